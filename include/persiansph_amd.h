@@ -1,0 +1,175 @@
+/*
+ * persiansph_amd — C ABI of the MI355X (gfx950) particle-interaction path of PersianSPH.
+ *
+ * The reference has no FFI: its hot path is the body of SPH::Domain::Solve
+ * (Source/Domain.cpp:1735-1797), reached from user code through the C++ Domain API
+ * (Source/Domain.h:44-166).  This header is the seam that a drop-in Domain::Solve binds:
+ * plain C, POD structs, int status codes, no exceptions across the boundary, no torch or
+ * HIP types in any signature.  Each entry point names the reference code it replaces.
+ *
+ * Particle order at the boundary is the reference's particle order (the index into
+ * Domain::Particles).  Inside the library particles live on the GPU in cell-sorted
+ * structure-of-arrays form; upload/download convert.
+ */
+#ifndef PERSIANSPH_AMD_H
+#define PERSIANSPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (0 = ok, negative = error; message via psph_last_error) ---- */
+#define PSPH_OK               0
+#define PSPH_ERR_INVALID     -1  /* bad argument or enum value (reference: abort() in Functions.cpp switches) */
+#define PSPH_ERR_UNSUPPORTED -2  /* feature outside this path (periodic/in-out-flow BCs, FSI, multi-material pairs) */
+#define PSPH_ERR_HIP         -3  /* HIP runtime failure */
+#define PSPH_ERR_TIMESTEP    -4  /* adaptive dt collapsed below deltatint/1e5 (Domain.cpp:150-151) */
+#define PSPH_ERR_CFL         -5  /* user dt above the CFL/viscous limit (Domain.cpp:1721-1733) */
+#define PSPH_ERR_NO_DEVICE   -6  /* no GPU visible */
+#define PSPH_ERR_STATE       -7  /* call out of order (e.g. run before upload) */
+#define PSPH_ERR_CAPACITY    -8  /* output buffer too small / size limit exceeded */
+
+/* ---- precision tiers ---- */
+#define PSPH_FP64        0  /* every quantity in FP64 (parity tier) */
+#define PSPH_FP32_MIXED  1  /* FP32 pair arithmetic on cell-relative coordinates, Kahan-compensated
+                               sums; FP64 positions, velocities, densities and integration */
+
+/* ---- why psph_run returned ---- */
+#define PSPH_STOP_STEPS   0  /* max_steps steps completed */
+#define PSPH_STOP_TEND    1  /* Time >= tf at the head of a step (Domain.cpp:1762 loop condition) */
+#define PSPH_STOP_OUTPUT  2  /* Time >= tout after the force pass (Domain.cpp:1773): the state is
+                                mid-step (forces of this step present, Move pending); the next
+                                psph_run call finishes the step first */
+
+/* Solver-wide settings: the Domain fields read inside Solve (Domain.h:89-164). */
+typedef struct psph_config {
+    int32_t dimension;       /* Domain::Dimension (2 or 3) */
+    int32_t kernel_type;     /* Kernel_Set: 0 cubic spline, 1 Wendland "Quintic", 2 quintic spline */
+    int32_t visc_eq;         /* Viscosity_Eq_Set: 0 Morris, 1 Shao, 2 incompressible, 3 Takeda */
+    int32_t gradient_type;   /* Gradient_Approach_Set: 0 squared density, 1 multiplied density */
+    int32_t scheme;          /* Domain::Scheme: 0 modified Verlet, 1 Leapfrog */
+    int32_t precision;       /* PSPH_FP64 or PSPH_FP32_MIXED */
+    int32_t periodic[3];     /* Boundary::Periodic (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    int32_t in_out_flow;     /* Boundary::InOutFlow (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    int32_t swi_type;        /* Domain::SWIType (only the single-material meaning is used) */
+    int32_t fsi;             /* Domain::FSI (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    double  cellfac;         /* Domain::Cellfac (2 for kernels 0/1, 3 for kernel 2) */
+    double  gravity[3];      /* Domain::Gravity */
+    double  xsph;            /* Domain::XSPH */
+    double  sqrt_h_a;        /* Domain::sqrt_h_a (default 0.0025) */
+    double  dom_max[3];      /* Domain::DomMax (default -1e11 per component) */
+    double  dom_min[3];      /* Domain::DomMin (default +1e11 per component) */
+    double  trpr[3];         /* Domain::TRPR on entry (all zero with BLPF: computed from particles) */
+    double  blpf[3];         /* Domain::BLPF on entry */
+    double  R;               /* Domain::R (periodic margin only) */
+    double  mu_max;          /* Domain::MuMax on entry (the reference never initialises it; pass 0) */
+    double  cs_max;          /* Domain::CsMax on entry (likewise; pass 0) */
+} psph_config;
+
+/* One particle: the fields of SPH::Particle (Source/Particle.h:29-149) that the path reads
+ * or writes, FP64 tensors row-major 3x3.  Defaults are the Particle constructor's
+ * (Source/Particle.cpp:25-117). */
+typedef struct psph_particle {
+    double x[3], v[3], va[3], vb[3], a[3], NSv[3], VXSPH[3];
+    double Density, Densitya, Densityb, dDensity, Pressure, Mass;
+    double ZWab, SumDen, SumKernel;
+    double Mu, ShearRate, SBar;
+    double StrainRate[9], RotationRate[9];
+    double ShearStress[9], ShearStressa[9], ShearStressb[9];
+    double Sigma[9], Sigmaa[9], Sigmab[9];
+    double Strain[9], Straina[9], Strainb[9];
+    double TIR[9];
+    double h, Cs, P0, RefDensity, FPMassC;
+    double Alpha, Beta, MuRef, T0, m, CSmag;
+    double TI, TIn, TIInitDist;
+    double G, K, Sigmay, c, phi, psi;
+    int32_t IsFree, NoSlip, Shepard, LES;
+    int32_t Material, PresEq, VisM, Fail;
+    int32_t ID, ct, FirstStep, InOut;
+    int32_t ShepardCounter, ShepardStep;
+    int32_t reserved[2];
+} psph_particle;
+
+/* Scalars of the Solve loop after a psph_run call. */
+typedef struct psph_status {
+    double  time;            /* Domain::Time */
+    double  deltat;          /* Domain::deltat (the dt of the last completed Move) */
+    double  deltatmin;       /* Domain::deltatmin of the last force pass */
+    int64_t steps;           /* steps completed by this call */
+    int64_t pairs;           /* in-support pairs (rij/h <= Cellfac) of the last force pass */
+    int64_t deleted;         /* particles removed by the leave check during this call */
+    size_t  n;               /* particle count after the call */
+    int32_t stop_reason;     /* PSPH_STOP_* */
+    int32_t mid_step;        /* 1 if stopped after the force pass with Move pending */
+} psph_status;
+
+/* The cell grid built by CellInitiate (Domain.cpp:549-660). */
+typedef struct psph_grid {
+    double  blpf[3], trpr[3], cell_size[3];
+    double  hmax, rhomax, mu_max, cs_max;
+    int32_t cell_no[3];
+    int32_t reserved;
+} psph_grid;
+
+typedef struct psph_handle psph_handle;
+
+/* Creates a solver bound to the current HIP device, with its own stream.
+ * Replaces Domain::Domain's solver state (Domain.cpp:54-102) for the Solve path. */
+int psph_create(const psph_config *cfg, psph_handle **out);
+
+/* Loads n particles (reference order) and runs Solve's initialisation: deltat = deltatint =
+ * deltatmin = dt; InitialChecks (EOS pressure); CellInitiate; TimestepCheck (PSPH_ERR_CFL).
+ * Replaces Domain.cpp:1735-1754 (InitialChecks :1662-1719, CellInitiate :549-660,
+ * ListGenerate :662-784, TimestepCheck :1721-1733).  time is Domain::Time on entry. */
+int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, double dt);
+
+/* Runs the Solve loop body (Domain.cpp:1762-1793) on the GPU: StartAcceleration,
+ * MainNeighbourSearch, PrimaryComputeAcceleration, LastComputeAcceleration,
+ * AdaptiveTimeStep, Move, Time += deltat, CheckParticleLeave, CellReset/ListGenerate.
+ * Stops after max_steps steps, when Time >= tf at a step head, or when Time >= tout after
+ * a force pass (then mid-step; pass tout = +inf to disable).  Particles leaving
+ * [BLPF, TRPR] are deleted in order-preserving fashion as Array::DelItems does. */
+int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_status *st);
+
+/* Copies the particle state back in reference order (cap = capacity of p). */
+int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n);
+
+/* Test hook: the neighbour pair set of the current positions, as the reference's
+ * MainNeighbourSearch + the rij/h <= Cellfac test in CalcForce11/2233 (Interaction.cpp:32,195)
+ * would form it: unordered pairs (P1, P2) in reference particle indices with the reference's
+ * orientation (P1 in the cell with the smaller key, or the larger index within one cell),
+ * at least one free, same material.  pairs holds 2*cap int64 values. */
+int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs);
+
+/* The grid computed at upload. */
+int psph_get_grid(psph_handle *h, psph_grid *g);
+
+/* Profiling: when enabled, every kernel launch is bracketed by HIP events on the solver's
+ * stream; psph_kernel_stats returns per-kernel launch counts and mean durations (ms). */
+int psph_set_timing(psph_handle *h, int enable);
+int psph_kernel_stats(psph_handle *h, char *names, size_t names_len,
+                      double *mean_ms, int64_t *launches, size_t cap, size_t *nkernels);
+
+/* The HIP stream the solver launches on (hipStream_t as an opaque pointer). */
+int psph_stream(psph_handle *h, void **stream);
+
+/* Bytes of algorithmic (compulsory) HBM traffic per step for the current particle set,
+ * split per kernel class, as used for roofline figures. */
+int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_bytes);
+
+int psph_destroy(psph_handle *h);
+
+/* Copies the calling thread's last error message (NUL-terminated); returns its length. */
+int psph_last_error(char *buf, size_t len);
+
+/* ABI version: major*100 + minor. */
+int psph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PERSIANSPH_AMD_H */

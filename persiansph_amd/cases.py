@@ -1,0 +1,311 @@
+"""Input generation for the benchmark and test workloads.
+
+`add_box_length` restates SPH::Domain::AddBoxLength (Source/Domain.cpp:294-495) for
+non-random boxes, and the dam-break builders follow the per-particle setup of the
+reference drivers 3-DamBreak.cpp:25-105 and 4-DamBreak3D.cpp:25-126 (walls fixed and
+no-slip, fluid above the column deleted, hydrostatic initial density).  Where a workload
+asks for lattice jitter, it is drawn from a seeded numpy generator and applied to the fluid
+only, identically for every solver the inputs are fed to.
+"""
+import math
+
+import numpy as np
+
+from .abi import default_config, new_particles
+
+
+def add_box_length(dim, V, Lx, Ly, Lz, r, density, h, ptype=1, rotation=0, fixed=False):
+    """Lattice box of particles in the order AddBoxLength pushes them (Domain.cpp:294-495).
+
+    ptype 1 is cubic packing, ptype 0 hexagonal close packing (rotation 0 only in 3-D).
+    Returns a structured array with x, Mass, Density, RefDensity, h, IsFree set.
+    """
+    V = [float(v) for v in V]
+    xs = []
+    if dim == 3:
+        if ptype == 1:
+            k = 0
+            zp = V[2]
+            while zp <= V[2] + Lz - r:
+                j = 0
+                yp = V[1]
+                while yp <= V[1] + Ly - r:
+                    i = 0
+                    xp = V[0]
+                    while xp <= V[0] + Lx - r:
+                        xs.append((V[0] + (2.0*i+1)*r, V[1] + (2.0*j+1)*r, V[2] + (2.0*k+1)*r))
+                        i += 1
+                        xp = V[0] + (2*i+1)*r
+                    j += 1
+                    yp = V[1] + (2.0*j+1)*r
+                k += 1
+                zp = V[2] + (2.0*k+1)*r
+        else:
+            k = 0
+            zp = V[2]
+            while zp <= V[2] + Lz - r:
+                j = 0
+                yp = V[1]
+                while yp <= V[1] + Ly - r:
+                    i = 0
+                    xp = V[0]
+                    while xp <= V[0] + Lx - r:
+                        if (k % 2 != 0) and (j % 2 != 0):
+                            x = V[0] + (2*i+(j % 2)+(k % 2)-1)*r
+                        else:
+                            x = V[0] + (2*i+(j % 2)+(k % 2)+1)*r
+                        y = V[1] + (math.sqrt(3.0)*(j+(1.0/3.0)*(k % 2))+1)*r
+                        z = V[2] + ((2*math.sqrt(6.0)/3)*k+1)*r
+                        xs.append((x, y, z))
+                        i += 1
+                        if (k % 2 != 0) and (j % 2 != 0):
+                            xp = V[0] + (2*i+(j % 2)+(k % 2)-1)*r
+                        else:
+                            xp = V[0] + (2*i+(j % 2)+(k % 2)+1)*r
+                    j += 1
+                    yp = V[1] + (math.sqrt(3.0)*(j+(1.0/3.0)*(k % 2))+1)*r
+                k += 1
+                zp = V[2] + ((2*math.sqrt(6.0)/3)*k+1)*r
+        p = new_particles(len(xs))
+        p["x"] = np.array(xs, dtype=np.float64)
+        # particle mass in 3-D (Domain.cpp:401-418)
+        mx = list(V)
+        for q in range(len(xs)):
+            for c in range(3):
+                if p["x"][q, c] > mx[c]:
+                    mx[c] = p["x"][q, c]
+        mx = [m + r for m in mx]
+        t = [mx[c] - V[c] for c in range(3)]
+        p["Mass"] = t[0]*t[1]*t[2]*density/len(xs)
+    elif dim == 2:
+        masses = []
+        if ptype == 0 and rotation == 0:
+            j = 0
+            yp = V[1]
+            while yp <= V[1] + Ly - r:
+                i = 0
+                xp = V[0]
+                while xp <= V[0] + Lx - r:
+                    xs.append((V[0] + (2*i+(j % 2)+1)*r, V[1] + (math.sqrt(3.0)*j+1)*r, 0.0))
+                    masses.append((math.sqrt(3.0)*r*r)*density)
+                    i += 1
+                    xp = V[0] + (2*i+(j % 2)+1)*r
+                j += 1
+                yp = V[1] + (math.sqrt(3.0)*j+1)*r
+        elif ptype == 0:
+            i = 0
+            xp = V[0]
+            while xp <= V[0] + Lx - r:
+                j = 0
+                yp = V[1]
+                while yp <= V[1] + Ly - r:
+                    xs.append((V[0] + (math.sqrt(3.0)*i+1)*r, V[1] + (2*j+(i % 2)+1)*r, 0.0))
+                    masses.append((math.sqrt(3.0)*r*r)*density)
+                    j += 1
+                    yp = V[1] + (2*j+(i % 2)+1)*r
+                i += 1
+                xp = V[0] + (math.sqrt(3.0)*i+1)*r
+        else:
+            j = 0
+            yp = V[1]
+            while yp <= V[1] + Ly - r:
+                i = 0
+                xp = V[0]
+                while xp <= V[0] + Lx - r:
+                    xs.append((V[0] + (2*i+1)*r, V[1] + (2*j+1)*r, 0.0))
+                    masses.append(2.0*r*2.0*r*density)
+                    i += 1
+                    xp = V[0] + (2*i+1)*r
+                j += 1
+                yp = V[1] + (2*j+1)*r
+        p = new_particles(len(xs))
+        p["x"] = np.array(xs, dtype=np.float64).reshape(-1, 3)
+        p["Mass"] = np.array(masses)
+    else:
+        raise ValueError("dimension must be 2 or 3")
+    p["Density"] = density
+    p["RefDensity"] = density
+    p["h"] = h
+    p["IsFree"] = 0 if fixed else 1
+    p["ID"] = 1
+    return p
+
+
+def _fast_cubic_box(dim, V, counts, r, density, h):
+    """Vectorised cubic-packing box for large 3-D cases; identical positions to
+    add_box_length(ptype=1) (x fastest, then y, then z, as the reference's loops nest)."""
+    nx, ny, nz = counts
+    i = np.arange(nx, dtype=np.float64)
+    j = np.arange(ny, dtype=np.float64)
+    k = np.arange(nz, dtype=np.float64)
+    X = V[0] + (2.0*i+1)*r
+    Y = V[1] + (2.0*j+1)*r
+    Z = V[2] + (2.0*k+1)*r
+    n = nx*ny*nz
+    p = new_particles(n)
+    x = np.empty((n, 3))
+    x[:, 0] = np.tile(X, ny*nz)
+    x[:, 1] = np.tile(np.repeat(Y, nx), nz)
+    x[:, 2] = np.repeat(Z, nx*ny)
+    p["x"] = x
+    mx = [max(V[c], x[:, c].max()) + r for c in range(3)]
+    t = [mx[c] - V[c] for c in range(3)]
+    p["Mass"] = t[0]*t[1]*t[2]*density/n
+    p["Density"] = density
+    p["RefDensity"] = density
+    p["h"] = h
+    p["ID"] = 1
+    return p
+
+
+def _loop_count(v0, L, r):
+    """Number of lattice sites the reference's `while (xp <= V+L-r)` loop visits
+    (xp starts at V, then V + (2i+1) r; Domain.cpp:371-384)."""
+    n = 0
+    xp = v0
+    while xp <= v0 + L - r:
+        n += 1
+        xp = v0 + (2*n+1)*r
+    return n
+
+
+def dam_break(dim, dx, tank, column, h_factor=1.3, rho0=1000.0, g=9.81, mu=1.0e-3,
+              alpha=0.05, cs=None, scheme=1, layers=3, jitter=0.0, seed=12345,
+              precision=0, kernel_type=0):
+    """Weakly-compressible dam break following 3-DamBreak.cpp / 4-DamBreak3D.cpp.
+
+    tank = (TL, TH) in 2-D or (TL, TH, TW) in 3-D (length x, height y, width z);
+    column = (L, H) or (L, H, W).  Walls are `layers` lattice layers outside the tank
+    on the left, right, bottom (and front/back in 3-D), no lid.  Returns
+    (config, particles, dt) with dt = 0.2 h / c0.
+    """
+    H = column[1]
+    h = h_factor*dx
+    if cs is None:
+        cs = 10.0*math.sqrt(2.0*g*H)
+    r = dx/2.0
+    lay = float(layers)
+    if dim == 2:
+        TL, TH = tank
+        L = column[0]
+        V = (-lay*dx, -lay*dx, 0.0)
+        p = add_box_length(2, V, 2*lay*dx + TL + dx/10.0, lay*dx + TH + dx/10.0, 0.0, r, rho0, h, 1, 0, False)
+    else:
+        TL, TH, TW = tank
+        L, H, W = column
+        V = (-lay*dx, -lay*dx, -lay*dx)
+        Ls = (2*lay*dx + TL + dx/10.0, lay*dx + TH + dx/10.0, 2*lay*dx + TW + dx/10.0)
+        counts = tuple(_loop_count(V[c], Ls[c], r) for c in range(3))
+        p = _fast_cubic_box(3, V, counts, r, rho0, h)
+    x = p["x"]
+    wall = (x[:, 0] < 0.0) | (x[:, 1] < 0.0) | (x[:, 0] > TL)
+    if dim == 3:
+        wall |= (x[:, 2] < 0.0) | (x[:, 2] > TW)
+    fluid_keep = ~wall & (x[:, 1] <= H) & (x[:, 0] <= L)
+    if dim == 3:
+        fluid_keep &= (x[:, 2] <= W)
+    keep = wall | fluid_keep          # DelParticles(11), order preserving (array.h:726-738)
+    p = p[keep]
+    wall = wall[keep]
+    p["Cs"] = cs
+    p["PresEq"] = 1
+    p["Mu"] = mu
+    p["MuRef"] = mu
+    p["Material"] = 1
+    p["Shepard"] = 1
+    p["Alpha"] = alpha
+    p["IsFree"] = np.where(wall, 0, 1)
+    p["NoSlip"] = np.where(wall, 1, 0)
+    p["ID"] = np.where(wall, 2, 1)
+    fl = ~wall
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter*dx, jitter*dx, size=(int(fl.sum()), 3))
+        if dim == 2:
+            d[:, 2] = 0.0
+        p["x"][fl] = p["x"][fl] + d
+    yb = p["x"][fl, 1]
+    rho = rho0*np.power(1.0 + 7.0*g*(H - yb)/(cs*cs), 1.0/7.0)
+    p["Density"][fl] = rho
+    p["Densityb"][fl] = rho
+    cfg = default_config(dimension=dim, kernel_type=kernel_type, visc_eq=0, gradient_type=0,
+                         scheme=scheme, precision=precision, gravity=(0.0, -g, 0.0))
+    dt = 0.2*h/cs
+    return cfg, p, dt
+
+
+def dam_break_2d(dx=0.02, scheme=1, **kw):
+    """2-D dam break: tank 4.0 x 3.0 m, column 1.0 x 2.0 m, 3 wall layers."""
+    return dam_break(2, dx, (4.0, 3.0), (1.0, 2.0), scheme=scheme, **kw)
+
+
+def dam_break_3d(dx=0.01, scheme=1, jitter=0.01, seed=12345, tank=(3.2, 1.0, 1.0),
+                 column=(1.2, 0.6, 1.0), cs=10.0*math.sqrt(2.0*9.81*2.0), **kw):
+    """3-D dam break: tank 3.2 x 1.0 x 1.0 m (x, y up, z), column 1.2 long, 0.6 high and
+    1.0 wide, 3 wall layers, no lid, fluid jitter +-0.01 dx.  The sound speed is the 2-D
+    case's c0 = 10 sqrt(2 g 2.0) = 62.64 m/s, so dt = 0.2 h / c0 = 4.15e-5 s at dx = 0.01."""
+    return dam_break(3, dx, tank, column, scheme=scheme, jitter=jitter, seed=seed, cs=cs, **kw)
+
+
+def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor=(1.0, 0.6), rho=2650.0,
+                    phi_deg=30.0, cohesion=0.0, psi_deg=0.0, E=5.0e6, nu=0.3, h_factor=1.5,
+                    cs=None, ti=0.5, tin=2.55, alpha=0.1, beta=0.1, scheme=1, layers=3,
+                    g=9.81, precision=0, fail=3):
+    """Granular column collapse with the Drucker-Prager soil model (Material 3).
+
+    The column (size = (Lx, Ly up, Lz)) stands in the corner x in [0, Lx], z in [0, Lz]
+    on a rigid floor of `layers` lattice layers covering x in [0, floor[0]] and
+    z in [-(floor[1]-Lz)/2 ... ] (centred on the column).  Parameters follow
+    Old_Tests/Dam_Scouring.cpp:187-202 (Fail 3, TI 0.5, TIn 2.55, TIInitDist dx) and
+    Old_Tests/ReposeAngle.cpp:93-99 (alpha = beta = 0.1).
+    """
+    h = h_factor*dx
+    r = dx/2.0
+    K = E/(3.0*(1.0-2.0*nu))
+    G = E/(2.0*(1.0+nu))
+    if cs is None:
+        cs = math.sqrt(K/rho)
+    Lx, Ly, Lz = size
+    if dim == 3:
+        nx, ny, nz = (int(round(Lx/dx)), int(round(Ly/dx)), int(round(Lz/dx)))
+        soil = _fast_cubic_box(3, (0.0, 0.0, 0.0), (nx, ny, nz), r, rho, h)
+        fx = int(round(floor[0]/dx))
+        fz = int(round(floor[1]/dx))
+        z0 = Lz/2.0 - floor[1]/2.0
+        fl = _fast_cubic_box(3, (-(layers*dx), -layers*dx, z0), (fx + layers, layers, fz), r, rho, h)
+    else:
+        nx, ny = (int(round(Lx/dx)), int(round(Ly/dx)))
+        soil = add_box_length(2, (0.0, 0.0, 0.0), nx*dx, ny*dx, 0.0, r, rho, h, 1, 0, False)
+        fx = int(round(floor[0]/dx))
+        fl = add_box_length(2, (-layers*dx, -layers*dx, 0.0), (fx+layers)*dx, layers*dx, 0.0, r, rho, h, 1, 0, True)
+    fl["IsFree"] = 0
+    fl["ID"] = 2
+    p = np.concatenate([fl, soil])
+    p["Material"] = 3
+    p["Fail"] = fail
+    p["phi"] = math.radians(phi_deg)
+    p["psi"] = math.radians(psi_deg)
+    p["c"] = cohesion
+    p["K"] = K
+    p["G"] = G
+    p["Cs"] = cs
+    p["Alpha"] = alpha
+    p["Beta"] = beta
+    p["TI"] = ti
+    p["TIn"] = tin
+    p["TIInitDist"] = dx
+    p["Shepard"] = 0
+    # geostatic initial stress in the soil: sigma_yy = -rho g (H - y), lateral K0 nu/(1-nu)
+    free = p["IsFree"] == 1
+    y = p["x"][:, 1]
+    syy = np.where(free, -rho*g*(Ly - y), 0.0)
+    k0 = nu/(1.0-nu)
+    sig = np.zeros((len(p), 9))
+    sig[:, 4] = syy
+    sig[:, 0] = k0*syy
+    sig[:, 8] = k0*syy if dim == 3 else 0.0
+    p["Sigma"] = sig
+    cfg = default_config(dimension=dim, kernel_type=0, visc_eq=0, gradient_type=0,
+                         scheme=scheme, precision=precision, gravity=(0.0, -g, 0.0))
+    dt = 0.2*h/cs
+    return cfg, p, dt

@@ -1,0 +1,946 @@
+// C ABI of persiansph_amd (include/persiansph_amd.h): host side of the GPU Solve path.
+//
+// Upload restates Solve's initialisation (Source/Domain.cpp:1735-1754): InitialChecks,
+// CellInitiate, TimestepCheck, then builds the per-particle class table and the device
+// structure-of-arrays.  psph_run queues steps on the solver's stream in batches; the device
+// control block (Ctl) carries Time, dt and the stop conditions, so the host only reads it back
+// once per batch.  Deletion of particles leaving the domain (CheckParticleLeave,
+// Domain.cpp:517-547) is handled here by an order-preserving compaction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/persiansph_amd.h"
+#include "psph_dev.h"
+
+namespace psph {
+void launch_head(const Dev &D, const Params &P, hipStream_t s);
+void launch_keys(const Dev &D, const Params &P, hipStream_t s);
+void launch_scan(const Dev &D, const Params &P, hipStream_t s);
+void launch_scatter(const Dev &D, const Params &P, hipStream_t s);
+void launch_cellsort(const Dev &D, const Params &P, hipStream_t s);
+void launch_reorder(const Dev &D, const Params &P, hipStream_t s);
+void launch_wall(const Dev &D, const Params &P, hipStream_t s);
+void launch_force(const Dev &D, const Params &P, hipStream_t s);
+void launch_integrate(const Dev &D, const Params &P, hipStream_t s);
+void launch_tail(const Dev &D, const Params &P, hipStream_t s);
+void launch_pairs(const Dev &D, const Params &P, hipStream_t s, int pass, const long long *off,
+                  int *cnt, long long *out);
+int scan_blocks(const Params &P);
+}  // namespace psph
+
+using namespace psph;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(PSPH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+enum KernelId { KHead, KKeys, KScan, KScatter, KCellsort, KReorder, KWall, KForce, KIntegrate, KTail, KCount };
+const char *kKernelNames[KCount] = {"head", "keys", "scan", "scatter", "cellsort", "reorder",
+                                    "wall", "force", "integrate", "tail"};
+
+struct Column { void *p = nullptr; size_t bytes = 0; };
+
+}  // namespace
+
+struct psph_handle {
+    psph_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Params P{};
+    Dev D[2]{};                 // D[0]: persistent A -> B, D[1]: B -> A
+    std::vector<void *> allocs;
+    Ctl *ctl_dev = nullptr;
+    Ctl *ctl_host = nullptr;    // pinned
+    ClassParams *cls_dev = nullptr;
+    std::vector<ClassParams> classes;
+    std::vector<psph_particle> host;   // reference-order mirror (fields the GPU does not keep)
+    size_t cap = 0;
+    bool uploaded = false;
+    bool broken = false;
+    psph_grid grid{};
+    long long parity_base = 0;         // reorders already reflected in D[] selection
+    bool trans_ok = false;             // transient columns match the current order
+    // timing
+    bool timing = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    double t_ms[KCount] = {0};
+    long long t_n[KCount] = {0};
+    // scratch for the pair query
+    int *pair_cnt = nullptr;
+    long long *pair_off = nullptr;
+};
+
+// ------------------------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------------------------
+static int dev_alloc(psph_handle *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    HIPCHECK(hipMalloc(p, bytes));
+    HIPCHECK(hipMemset(*p, 0, bytes));
+    h->allocs.push_back(*p);
+    return 0;
+}
+
+static void free_all(psph_handle *h)
+{
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) hipFree(p);
+    h->allocs.clear();
+    if (h->ctl_host) { hipHostFree(h->ctl_host); h->ctl_host = nullptr; }
+    for (auto &e : h->pending) { hipEventDestroy(e.second.first); hipEventDestroy(e.second.second); }
+    h->pending.clear();
+    for (auto e : h->event_pool) hipEventDestroy(e);
+    h->event_pool.clear();
+}
+
+static hipEvent_t get_event(psph_handle *h)
+{
+    if (!h->event_pool.empty()) {
+        hipEvent_t e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+template <class F>
+static void timed(psph_handle *h, int id, F &&f)
+{
+    if (!h->timing) { f(); return; }
+    hipEvent_t a = get_event(h), b = get_event(h);
+    hipEventRecord(a, h->stream);
+    f();
+    hipEventRecord(b, h->stream);
+    h->pending.push_back({id, {a, b}});
+}
+
+static void collect_timing(psph_handle *h)
+{
+    for (auto &e : h->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+            h->t_ms[e.first] += ms;
+            h->t_n[e.first] += 1;
+        }
+        h->event_pool.push_back(e.second.first);
+        h->event_pool.push_back(e.second.second);
+    }
+    h->pending.clear();
+}
+
+static inline double norm3(const double *a) { return std::sqrt(a[0]*a[0] + a[1]*a[1] + a[2]*a[2]); }
+
+// EOS, Functions.cpp:188-212 (host copy for InitialChecks)
+static double host_eos(int EQ, double Cs0, double P00, double Density, double Density0)
+{
+    switch (EQ) {
+    case 0: return P00+(Cs0*Cs0)*(Density-Density0);
+    case 1: return P00+(Density0*Cs0*Cs0/7.0)*(std::pow(Density/Density0,7.0)-1);
+    default: return (Cs0*Cs0)*Density;
+    }
+}
+
+static void fill_derived(ClassParams &c)
+{
+    double tp = std::tan(c.phi);
+    c.dp_alpha2 = tp / std::sqrt(9.0+12.0*std::tan(c.phi)*std::tan(c.phi));
+    c.dp_kf2 = 3.0 * c.c / std::sqrt(9.0+12.0*std::tan(c.phi)*std::tan(c.phi));
+    c.dp_alpha3 = (2.0* std::sin(c.phi)) / (std::sqrt(3.0)*(3.0-std::sin(c.phi)));
+    c.dp_kf3 = (6.0*c.c*std::cos(c.phi)) / (std::sqrt(3.0)*(3.0-std::sin(c.phi)));
+    c.sin_psi = std::sin(c.psi);
+}
+
+static ClassParams class_of(const psph_particle &p)
+{
+    ClassParams c;
+    std::memset(&c, 0, sizeof c);
+    c.h = p.h; c.Cs = p.Cs; c.P0 = p.P0; c.RefDensity = p.RefDensity; c.FPMassC = p.FPMassC;
+    c.Alpha = p.Alpha; c.Beta = p.Beta; c.MuRef = p.MuRef; c.T0 = p.T0; c.m = p.m; c.CSmag = p.CSmag;
+    c.TI = p.TI; c.TIn = p.TIn; c.TIInitDist = p.TIInitDist;
+    c.G = p.G; c.K = p.K; c.Sigmay = p.Sigmay; c.c = p.c; c.phi = p.phi; c.psi = p.psi;
+    c.IsFree = p.IsFree ? 1 : 0; c.NoSlip = p.NoSlip ? 1 : 0; c.Shepard = p.Shepard ? 1 : 0;
+    c.LES = p.LES ? 1 : 0; c.Material = p.Material; c.PresEq = p.PresEq; c.VisM = p.VisM;
+    c.Fail = p.Fail; c.ID = p.ID; c.ShepardStep = p.ShepardStep; c.InOut = p.InOut;
+    fill_derived(c);
+    return c;
+}
+
+struct ClassLess {
+    bool operator()(const ClassParams &a, const ClassParams &b) const
+    { return std::memcmp(&a, &b, sizeof a) < 0; }
+};
+
+static inline void to_sym(const double *M, double *s)
+{
+    s[0] = M[0]; s[1] = M[1]; s[2] = M[2]; s[3] = M[4]; s[4] = M[5]; s[5] = M[8];
+}
+
+static inline void from_sym(const double *s, double *M)
+{
+    M[0] = s[0]; M[1] = s[1]; M[2] = s[2];
+    M[3] = s[1]; M[4] = s[3]; M[5] = s[4];
+    M[6] = s[2]; M[7] = s[4]; M[8] = s[5];
+}
+
+// CellInitiate, Domain.cpp:549-660 (non-periodic)
+static int cell_initiate(psph_handle *h, const psph_particle *p, size_t n)
+{
+    psph_grid &g = h->grid;
+    std::memset(&g, 0, sizeof g);
+    double TRPR[3], BLPF[3];
+    std::memcpy(TRPR, h->cfg.trpr, sizeof TRPR);
+    std::memcpy(BLPF, h->cfg.blpf, sizeof BLPF);
+    double hmax = 0.0, rhomax = 0.0, MuMax = h->cfg.mu_max, CsMax = h->cfg.cs_max;
+    if (!(norm3(TRPR) > 0.0) && !(norm3(BLPF) > 0.0)) {
+        std::memcpy(BLPF, p[0].x, sizeof BLPF);
+        std::memcpy(TRPR, p[0].x, sizeof TRPR);
+        hmax = p[0].h;
+        rhomax = p[0].Density;
+        for (size_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                if (p[i].x[c] > TRPR[c]) TRPR[c] = p[i].x[c];
+                if (p[i].x[c] < BLPF[c]) BLPF[c] = p[i].x[c];
+            }
+            if (p[i].h > hmax) hmax = p[i].h;
+            if (p[i].Density > rhomax) rhomax = p[i].Density;
+            if (p[i].Mu > MuMax) MuMax = p[i].Mu;
+            if (p[i].Cs > CsMax) CsMax = p[i].Cs;
+        }
+    } else {
+        return fail(PSPH_ERR_UNSUPPORTED, "a preset TRPR/BLPF box (second Solve on one Domain) is not supported");
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (h->cfg.dom_max[c] > TRPR[c]) TRPR[c] = h->cfg.dom_max[c];
+        if (h->cfg.dom_min[c] < BLPF[c]) BLPF[c] = h->cfg.dom_min[c];
+    }
+    for (int c = 0; c < 3; ++c) { TRPR[c] += hmax/2; BLPF[c] -= hmax/2; }
+    double Cellfac = h->cfg.cellfac;
+    int dims = h->cfg.dimension;
+    int CellNo[3] = {1, 1, 1};
+    double CellSize[3] = {0, 0, 0};
+    for (int c = 0; c < dims; ++c) {
+        double L = (TRPR[c]-BLPF[c])/(Cellfac*hmax);
+        if ((double)(std::ceil(L) - L) < (hmax/10.0)) CellNo[c] = (int)std::ceil(L);
+        else CellNo[c] = (int)std::floor(L);
+        if (CellNo[c] < 1) return fail(PSPH_ERR_INVALID, "domain smaller than one cell in dimension %d", c);
+        CellSize[c] = (TRPR[c]-BLPF[c])/CellNo[c];
+    }
+    double ncells = (double)CellNo[0]*CellNo[1]*CellNo[2];
+    if (ncells > 2.0e9) return fail(PSPH_ERR_CAPACITY, "too many cells (%g)", ncells);
+    std::memcpy(g.blpf, BLPF, sizeof BLPF);
+    std::memcpy(g.trpr, TRPR, sizeof TRPR);
+    std::memcpy(g.cell_size, CellSize, sizeof CellSize);
+    std::memcpy(g.cell_no, CellNo, sizeof CellNo);
+    g.hmax = hmax; g.rhomax = rhomax; g.mu_max = MuMax; g.cs_max = CsMax;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// device allocation and state transfer
+// ------------------------------------------------------------------------------------------
+static int allocate(psph_handle *h, size_t n)
+{
+    Params &P = h->P;
+    Dev A{};
+    auto al = [&](void **p, size_t b) { return dev_alloc(h, p, b); };
+    int rc = 0;
+#define AL(ptr, bytes) do { rc = al((void **)&(ptr), (bytes)); if (rc) return rc; } while (0)
+    size_t N = n ? n : 1;
+    Dev &a = A;
+    AL(h->ctl_dev, sizeof(Ctl));
+    AL(h->cls_dev, sizeof(ClassParams)*kMaxClasses);
+    // persistent, two buffers
+    double4 *pos[2], *vel[2], *vh[2];
+    double2 *mmu[2];
+    uint32_t *flags[2];
+    int32_t *orig[2];
+    double *sig[2] = {nullptr, nullptr}, *sigh[2] = {nullptr, nullptr}, *tau[2] = {nullptr, nullptr},
+           *tauh[2] = {nullptr, nullptr}, *eps[2] = {nullptr, nullptr}, *epsh[2] = {nullptr, nullptr};
+    for (int b = 0; b < 2; ++b) {
+        AL(pos[b], sizeof(double4)*N);
+        AL(vel[b], sizeof(double4)*N);
+        AL(vh[b], sizeof(double4)*N);
+        AL(mmu[b], sizeof(double2)*N);
+        AL(flags[b], sizeof(uint32_t)*N);
+        AL(orig[b], sizeof(int32_t)*N);
+        if (P.any_solid) {
+            AL(sig[b], sizeof(double)*6*N);
+            AL(sigh[b], sizeof(double)*6*N);
+            AL(tau[b], sizeof(double)*6*N);
+            AL(tauh[b], sizeof(double)*6*N);
+            AL(eps[b], sizeof(double)*6*N);
+            AL(epsh[b], sizeof(double)*6*N);
+        }
+    }
+    AL(a.aux, sizeof(double4)*N);
+    AL(a.wal, sizeof(double4)*N);
+    AL(a.acc, sizeof(double4)*N);
+    AL(a.xsp, sizeof(double4)*N);
+    AL(a.shep, sizeof(double2)*N);
+    if (P.any_solid || P.any_sr) AL(a.sr, sizeof(double)*6*N);
+    if (P.any_solid) { AL(a.rr, sizeof(double)*3*N); AL(a.tir, sizeof(double)*6*N); }
+    if (P.any_les_t0) AL(a.shr, sizeof(double)*2*N);
+    AL(a.key, sizeof(int32_t)*N);
+    AL(a.rank, sizeof(int32_t)*N);
+    AL(a.key_b, sizeof(int32_t)*N);
+    AL(a.src, sizeof(int32_t)*N);
+    AL(a.cstart, sizeof(int32_t)*(P.ncells + 1));
+    AL(a.ccount, sizeof(int32_t)*(P.ncells + 1));
+    AL(a.bsum, sizeof(int32_t)*(scan_blocks(P) + 1));
+    AL(a.free_list, sizeof(int32_t)*N);
+    AL(a.fixed_list, sizeof(int32_t)*N);
+    AL(h->pair_cnt, sizeof(int)*N);
+    AL(h->pair_off, sizeof(long long)*N);
+#undef AL
+    a.ctl = h->ctl_dev;
+    a.cls = h->cls_dev;
+    for (int d = 0; d < 2; ++d) {
+        Dev x = a;
+        int s = d, t = 1 - d;
+        x.pos = pos[s]; x.vel = vel[s]; x.vh = vh[s]; x.mmu = mmu[s]; x.flags = flags[s]; x.orig = orig[s];
+        x.sig = sig[s]; x.sigh = sigh[s]; x.tau = tau[s]; x.tauh = tauh[s]; x.eps = eps[s]; x.epsh = epsh[s];
+        x.pos_b = pos[t]; x.vel_b = vel[t]; x.vh_b = vh[t]; x.mmu_b = mmu[t]; x.flags_b = flags[t]; x.orig_b = orig[t];
+        x.sig_b = sig[t]; x.sigh_b = sigh[t]; x.tau_b = tau[t]; x.tauh_b = tauh[t]; x.eps_b = eps[t]; x.epsh_b = epsh[t];
+        h->D[d] = x;
+    }
+    if (!h->ctl_host) HIPCHECK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+    h->cap = N;
+    return 0;
+}
+
+// Index of the Dev variant whose *source* buffers hold the current state.
+static int cur_src(psph_handle *h)
+{
+    long long r = h->ctl_host->reorders;
+    return (int)((r - h->parity_base) & 1);
+}
+
+// Writes particles (reference order) into the current buffers; classes must be built.
+static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf)
+{
+    Params &P = h->P;
+    std::vector<double4> pos(n), vel(n), vh(n);
+    std::vector<double2> mmu(n);
+    std::vector<uint32_t> flags(n);
+    std::vector<int32_t> orig(n);
+    std::vector<double> sig, sigh, tau, tauh, eps, epsh;
+    if (P.any_solid) { sig.resize(6*n); sigh.resize(6*n); tau.resize(6*n); tauh.resize(6*n); eps.resize(6*n); epsh.resize(6*n); }
+    std::map<ClassParams, int, ClassLess> idx;
+    for (size_t k = 0; k < h->classes.size(); ++k) idx[h->classes[k]] = (int)k;
+    for (size_t i = 0; i < n; ++i) {
+        const psph_particle &q = p[i];
+        ClassParams c = class_of(q);
+        auto it = idx.find(c);
+        if (it == idx.end()) return fail(PSPH_ERR_STATE, "particle %zu has an unknown class", i);
+        int ci = it->second;
+        pos[i] = make_double4(q.x[0], q.x[1], q.x[2], q.Density);
+        vel[i] = make_double4(q.v[0], q.v[1], q.v[2], q.Pressure);
+        bool lf = (P.scheme == 1) && q.IsFree;
+        if (lf) vh[i] = make_double4(q.va[0], q.va[1], q.va[2], q.Densitya);
+        else vh[i] = make_double4(q.vb[0], q.vb[1], q.vb[2], q.Densityb);
+        mmu[i] = make_double2(q.Mass, q.Mu);
+        flags[i] = (uint32_t)ci | (q.IsFree ? kFree : 0u) | (q.NoSlip ? kNoSlip : 0u) |
+                   ((uint32_t)(q.Material & 3) << kMatShift);
+        orig[i] = (int32_t)i;
+        if (P.any_solid) {
+            to_sym(q.Sigma, &sig[6*i]);
+            to_sym(P.scheme == 1 ? q.Sigmaa : q.Sigmab, &sigh[6*i]);
+            to_sym(q.ShearStress, &tau[6*i]);
+            to_sym(P.scheme == 1 ? q.ShearStressa : q.ShearStressb, &tauh[6*i]);
+            to_sym(q.Strain, &eps[6*i]);
+            to_sym(P.scheme == 1 ? q.Straina : q.Strainb, &epsh[6*i]);
+        }
+    }
+    Dev &D = h->D[buf];
+    HIPCHECK(hipMemcpy(D.pos, pos.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.vel, vel.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.vh, vh.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.mmu, mmu.data(), sizeof(double2)*n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.flags, flags.data(), sizeof(uint32_t)*n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.orig, orig.data(), sizeof(int32_t)*n, hipMemcpyHostToDevice));
+    if (P.any_solid) {
+        HIPCHECK(hipMemcpy(D.sig, sig.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(D.sigh, sigh.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(D.tau, tau.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(D.tauh, tauh.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(D.eps, eps.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(D.epsh, epsh.data(), sizeof(double)*6*n, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// Reads the current device state into h->host (reference order by the orig column).
+// After a completed step the transient columns share the current order.
+static int read_state(psph_handle *h)
+{
+    Params &P = h->P;
+    size_t n = (size_t)P.n;
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    HIPCHECK(hipMemcpy(h->ctl_host, h->ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost));
+    int b = cur_src(h);
+    // after a reorder the state sits in the reorder destination of the variant that ran it
+    const Dev &D = h->D[b];
+    std::vector<double4> pos(n), vel(n), vh(n), aux(n), wal(n), acc(n), xsp(n);
+    std::vector<double2> mmu(n), shep(n);
+    std::vector<uint32_t> flags(n);
+    std::vector<int32_t> orig(n);
+    HIPCHECK(hipMemcpy(pos.data(), D.pos, sizeof(double4)*n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vel.data(), D.vel, sizeof(double4)*n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vh.data(), D.vh, sizeof(double4)*n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(mmu.data(), D.mmu, sizeof(double2)*n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(flags.data(), D.flags, sizeof(uint32_t)*n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(orig.data(), D.orig, sizeof(int32_t)*n, hipMemcpyDeviceToHost));
+    bool have_transients = h->trans_ok;
+    if (have_transients) {
+        HIPCHECK(hipMemcpy(wal.data(), D.wal, sizeof(double4)*n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(acc.data(), D.acc, sizeof(double4)*n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(xsp.data(), D.xsp, sizeof(double4)*n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(shep.data(), D.shep, sizeof(double2)*n, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> sig, sigh, tau, tauh, eps, epsh, sr, rr, tir, shr;
+    auto rd = [&](std::vector<double> &v, const double *src, size_t k) -> int {
+        v.resize(k*n);
+        if (src) HIPCHECK(hipMemcpy(v.data(), src, sizeof(double)*k*n, hipMemcpyDeviceToHost));
+        return 0;
+    };
+    if (P.any_solid) {
+        if (rd(sig, D.sig, 6) || rd(sigh, D.sigh, 6) || rd(tau, D.tau, 6) || rd(tauh, D.tauh, 6) ||
+            rd(eps, D.eps, 6) || rd(epsh, D.epsh, 6))
+            return PSPH_ERR_HIP;
+    }
+    if (have_transients) {
+        if ((P.any_solid || P.any_sr) && rd(sr, D.sr, 6)) return PSPH_ERR_HIP;
+        if (P.any_solid && (rd(rr, D.rr, 3) || rd(tir, D.tir, 6))) return PSPH_ERR_HIP;
+        if (P.any_les_t0 && rd(shr, D.shr, 2)) return PSPH_ERR_HIP;
+    }
+    bool shep_on = P.any_shepard;
+    for (size_t i = 0; i < n; ++i) {
+        int o = orig[i];
+        if (o < 0 || (size_t)o >= h->host.size()) return fail(PSPH_ERR_STATE, "corrupt particle index %d", o);
+        psph_particle &q = h->host[o];
+        bool fr = flag_free(flags[i]);
+        q.x[0] = pos[i].x; q.x[1] = pos[i].y; q.x[2] = pos[i].z; q.Density = pos[i].w;
+        q.v[0] = vel[i].x; q.v[1] = vel[i].y; q.v[2] = vel[i].z; q.Pressure = vel[i].w;
+        if (P.scheme == 1 && fr) { q.va[0] = vh[i].x; q.va[1] = vh[i].y; q.va[2] = vh[i].z; q.Densitya = vh[i].w; }
+        else { q.vb[0] = vh[i].x; q.vb[1] = vh[i].y; q.vb[2] = vh[i].z; q.Densityb = vh[i].w; }
+        q.Mass = mmu[i].x; q.Mu = mmu[i].y;
+        if (have_transients) {
+            if (fr) {
+                q.a[0] = acc[i].x; q.a[1] = acc[i].y; q.a[2] = acc[i].z; q.dDensity = acc[i].w;
+                q.SumKernel = 0.0;
+                if (P.any_xsph) { q.VXSPH[0] = xsp[i].x; q.VXSPH[1] = xsp[i].y; q.VXSPH[2] = xsp[i].z; }
+                else { q.VXSPH[0] = q.VXSPH[1] = q.VXSPH[2] = 0.0; }
+                if (shep_on) { q.ZWab = xsp[i].w; q.SumDen = shep[i].x; }
+            } else {
+                // the pair sums of fixed particles are not formed; their a stays the gravity reset
+                for (int c = 0; c < 3; ++c) q.a[c] = h->cfg.gravity[c];
+                q.dDensity = 0.0;
+                q.NSv[0] = wal[i].x; q.NSv[1] = wal[i].y; q.NSv[2] = wal[i].z;
+                q.SumKernel = shep[i].y;
+            }
+            if (!sr.empty()) { from_sym(&sr[6*i], q.StrainRate); }
+            if (!rr.empty()) {
+                double *R = q.RotationRate;
+                R[0] = 0.0; R[1] = rr[3*i]; R[2] = rr[3*i+1];
+                R[3] = -rr[3*i]; R[4] = 0.0; R[5] = rr[3*i+2];
+                R[6] = -rr[3*i+1]; R[7] = -rr[3*i+2]; R[8] = 0.0;
+            }
+            if (!tir.empty()) from_sym(&tir[6*i], q.TIR);
+            if (!shr.empty() && fr) { q.ShearRate = shr[2*i]; q.SBar = shr[2*i+1]; }
+        }
+        if (P.any_solid) {
+            from_sym(&sig[6*i], q.Sigma);
+            from_sym(&tau[6*i], q.ShearStress);
+            from_sym(&eps[6*i], q.Strain);
+            if (P.scheme == 1) { from_sym(&sigh[6*i], q.Sigmaa); from_sym(&tauh[6*i], q.ShearStressa); from_sym(&epsh[6*i], q.Straina); }
+            else { from_sym(&sigh[6*i], q.Sigmab); from_sym(&tauh[6*i], q.ShearStressb); from_sym(&epsh[6*i], q.Strainb); }
+        }
+        if (fr) {
+            q.FirstStep = h->ctl_host->first_step;
+            if (P.scheme == 0) { q.ct = h->ctl_host->ct; q.ShepardCounter = h->ctl_host->shep_counter; }
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// step queueing
+// ------------------------------------------------------------------------------------------
+static void enqueue_step(psph_handle *h, int d, bool finish_only)
+{
+    const Params &P = h->P;
+    const Dev &D = h->D[d];
+    if (!finish_only) {
+        timed(h, KScan, [&] { launch_scan(D, P, h->stream); });
+        timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
+        timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
+        timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
+        timed(h, KWall, [&] { launch_wall(D, P, h->stream); });
+        timed(h, KForce, [&] { launch_force(D, P, h->stream); });
+    }
+    timed(h, KIntegrate, [&] { launch_integrate(D, P, h->stream); });
+    timed(h, KTail, [&] { launch_tail(D, P, h->stream); });
+}
+
+static int sync_ctl(psph_handle *h)
+{
+    HIPCHECK(hipMemcpyAsync(h->ctl_host, h->ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    if (h->timing) collect_timing(h);
+    return 0;
+}
+
+static int push_ctl(psph_handle *h)
+{
+    HIPCHECK(hipMemcpyAsync(h->ctl_dev, h->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Order-preserving deletion of the particles outside [BLPF, TRPR] (CheckParticleLeave with
+// Array::DelItems, External/array.h:726-738); the survivors are re-uploaded.
+static int delete_leavers(psph_handle *h, long long *ndel)
+{
+    int rc = read_state(h);
+    if (rc) return rc;
+    const double *T = h->grid.trpr, *B = h->grid.blpf;
+    std::vector<psph_particle> keep;
+    keep.reserve(h->host.size());
+    long long del = 0;
+    for (auto &q : h->host) {
+        if ((q.x[0] > T[0]) || (q.x[1] > T[1]) || (q.x[2] > T[2]) ||
+            (q.x[0] < B[0]) || (q.x[1] < B[1]) || (q.x[2] < B[2])) { ++del; continue; }
+        keep.push_back(q);
+    }
+    h->host.swap(keep);
+    Params &P = h->P;
+    P.n = (int)h->host.size();
+    int nf = 0;
+    for (auto &q : h->host) nf += q.IsFree ? 1 : 0;
+    P.nfree = nf;
+    P.nfixed = P.n - nf;
+    // the state now lives in the source buffers of variant 0
+    h->parity_base = h->ctl_host->reorders;
+    rc = write_state(h, h->host.data(), h->host.size(), 0);
+    if (rc) return rc;
+    HIPCHECK(hipMemsetAsync(h->D[0].ccount, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
+    h->ctl_host->leave = 0;
+    h->ctl_host->halt = kRun;
+    h->trans_ok = false;
+    rc = push_ctl(h);
+    if (rc) return rc;
+    launch_keys(h->D[0], P, h->stream);
+    *ndel = del;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int psph_version(void) { return 100; }
+
+int psph_last_error(char *buf, size_t len)
+{
+    if (buf && len) {
+        std::snprintf(buf, len, "%s", g_err.c_str());
+    }
+    return (int)g_err.size();
+}
+
+int psph_create(const psph_config *cfg, psph_handle **out)
+{
+    if (!cfg || !out) return fail(PSPH_ERR_INVALID, "null argument");
+    if (cfg->dimension != 2 && cfg->dimension != 3)
+        return fail(PSPH_ERR_INVALID, "Please correct the dimension (2=>2D or 3=>3D)");
+    if (cfg->kernel_type < 0 || cfg->kernel_type > 2) return fail(PSPH_ERR_INVALID, "Kernel Type No is out of range");
+    if (cfg->visc_eq < 0 || cfg->visc_eq > 3) return fail(PSPH_ERR_INVALID, "Viscosity Equation No is out of range");
+    if (cfg->gradient_type < 0 || cfg->gradient_type > 1) return fail(PSPH_ERR_INVALID, "Gradient type out of range");
+    if (cfg->scheme < 0 || cfg->scheme > 1) return fail(PSPH_ERR_INVALID, "Scheme must be 0 (modified Verlet) or 1 (Leapfrog)");
+    if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_MIXED)
+        return fail(PSPH_ERR_INVALID, "precision must be PSPH_FP64 or PSPH_FP32_MIXED");
+    if (cfg->periodic[0] || cfg->periodic[1] || cfg->periodic[2])
+        return fail(PSPH_ERR_UNSUPPORTED, "periodic boundaries are outside this path");
+    if (cfg->in_out_flow) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries are outside this path");
+    if (cfg->fsi) return fail(PSPH_ERR_UNSUPPORTED, "fluid-structure interaction is outside this path");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PSPH_ERR_NO_DEVICE, "no HIP device visible");
+    psph_handle *h = new psph_handle();
+    h->cfg = *cfg;
+    hipGetDevice(&h->device);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return fail(PSPH_ERR_HIP, "hipStreamCreate failed");
+    }
+    *out = h;
+    return 0;
+}
+
+int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, double dt)
+{
+    if (!h || !p) return fail(PSPH_ERR_INVALID, "null argument");
+    if (n == 0) return fail(PSPH_ERR_INVALID, "no particles");
+    if (n > (size_t)0x7fffffff) return fail(PSPH_ERR_CAPACITY, "too many particles");
+    if (h->cfg.precision == PSPH_FP32_MIXED)
+        return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier is not built yet");
+    free_all(h);
+    h->uploaded = false;
+    h->broken = false;
+    h->host.assign(p, p + n);
+    // InitialChecks, Domain.cpp:1662-1719: EOS pressure of fluid and solid particles
+    for (auto &q : h->host)
+        if (q.Material < 3) q.Pressure = host_eos(q.PresEq, q.Cs, q.P0, q.Density, q.RefDensity);
+    int rc = cell_initiate(h, h->host.data(), n);
+    if (rc) return rc;
+    // TimestepCheck, Domain.cpp:1721-1733
+    {
+        double t1 = 0.25*h->grid.hmax/(h->grid.cs_max), t2;
+        if (h->grid.mu_max > 0.0) t2 = 0.125*h->grid.hmax*h->grid.hmax*h->grid.rhomax/h->grid.mu_max; else t2 = 1000000.0;
+        if (dt > std::min(t1, t2))
+            return fail(PSPH_ERR_CFL, "Please decrease the time step to the allowable range (dt=%g > %g)", dt, std::min(t1, t2));
+    }
+    // classes and uniformity checks
+    std::map<ClassParams, int, ClassLess> idx;
+    h->classes.clear();
+    int mat0 = h->host[0].Material;
+    bool have_free = false;
+    psph_particle ref{};
+    int nfree = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const psph_particle &q = h->host[i];
+        if (q.Material < 1 || q.Material > 3)
+            return fail(PSPH_ERR_INVALID, "Material type %d out of range (1 fluid, 2 solid, 3 soil)", q.Material);
+        if (q.Material != mat0)
+            return fail(PSPH_ERR_UNSUPPORTED, "particles of different materials (CalcForce12/13 pairs) are outside this path");
+        if (q.InOut != 0) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow particles are outside this path");
+        if (q.PresEq < 0 || q.PresEq > 2) return fail(PSPH_ERR_INVALID, "Pressure Equation No %d out of range", q.PresEq);
+        if (q.Material == 3 && q.Fail > 1 && q.Fail != 2 && q.Fail != 3) return fail(PSPH_ERR_INVALID, "Failure Type %d out of range", q.Fail);
+        if (q.Material == 2 && q.Fail > 1) return fail(PSPH_ERR_INVALID, "Undefined failure criteria for solids");
+        if (q.Material == 1 && q.T0 > 0.0 && (q.VisM < 0 || q.VisM > 1)) return fail(PSPH_ERR_INVALID, "Non-Newtonian Viscosity Type out of range");
+        ClassParams c = class_of(q);
+        if (!idx.count(c)) {
+            if ((int)h->classes.size() >= kMaxClasses)
+                return fail(PSPH_ERR_CAPACITY, "more than %d distinct particle parameter sets", kMaxClasses);
+            idx[c] = (int)h->classes.size();
+            h->classes.push_back(c);
+        }
+        if (q.IsFree) {
+            ++nfree;
+            if (!have_free) { ref = q; have_free = true; }
+            else {
+                if (q.PresEq != ref.PresEq || q.Cs != ref.Cs || q.P0 != ref.P0 || q.RefDensity != ref.RefDensity)
+                    return fail(PSPH_ERR_UNSUPPORTED, "free particles with different equations of state");
+                if (q.FirstStep != ref.FirstStep || q.ct != ref.ct || q.ShepardCounter != ref.ShepardCounter ||
+                    q.ShepardStep != ref.ShepardStep)
+                    return fail(PSPH_ERR_UNSUPPORTED, "free particles with different integrator counters");
+            }
+        }
+    }
+    Params &P = h->P;
+    std::memset(&P, 0, sizeof P);
+    for (int c = 0; c < 3; ++c) {
+        P.blpf[c] = h->grid.blpf[c]; P.trpr[c] = h->grid.trpr[c]; P.csize[c] = h->grid.cell_size[c];
+        P.nc[c] = h->grid.cell_no[c]; P.gravity[c] = h->cfg.gravity[c];
+    }
+    P.ncells = P.nc[0]*P.nc[1]*P.nc[2];
+    P.hmax = h->grid.hmax;
+    P.cellfac = h->cfg.cellfac;
+    P.sqrt_h_a = h->cfg.sqrt_h_a;
+    P.xsph = h->cfg.xsph;
+    P.n = (int)n;
+    P.nfree = nfree;
+    P.nfixed = (int)n - nfree;
+    P.nclasses = (int)h->classes.size();
+    P.dim = h->cfg.dimension;
+    P.kernel_type = h->cfg.kernel_type;
+    P.visc_eq = h->cfg.visc_eq;
+    P.gradient_type = h->cfg.gradient_type;
+    P.scheme = h->cfg.scheme;
+    P.precision = h->cfg.precision;
+    if (have_free) {
+        P.eos_eq = ref.PresEq; P.eos_cs = ref.Cs; P.eos_p0 = ref.P0; P.eos_rho0 = ref.RefDensity;
+        P.shepard_step = ref.ShepardStep;
+    }
+    P.uniform_h = 1;
+    P.h_uniform = h->classes[0].h;
+    for (auto &c : h->classes) {
+        if (c.h != P.h_uniform) P.uniform_h = 0;
+        if (c.TI > 0.0) P.any_ti = 1;
+        if (c.Material == 1 && (c.T0 > 0.0 || c.LES)) { P.any_sr = 1; P.any_les_t0 = 1; }
+        if (c.Material > 1) P.any_solid = 1;
+        if (c.IsFree && c.Shepard && P.scheme == 0) P.any_shepard = 1;
+    }
+    P.any_xsph = (h->cfg.xsph != 0.0 && P.scheme == 0) ? 1 : 0;
+    rc = allocate(h, n);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
+    h->parity_base = 0;
+    std::memset(h->ctl_host, 0, sizeof(Ctl));
+    Ctl &C = *h->ctl_host;
+    C.time = time;
+    C.deltat = C.deltatint = C.deltatmin = dt;
+    C.tf = INFINITY;
+    C.tout = INFINITY;
+    double di = dt;
+    std::memcpy(&C.dtmin_bits, &di, sizeof di);
+    C.halt = kRun;
+    if (have_free) { C.ct = ref.ct; C.shep_counter = ref.ShepardCounter; C.first_step = ref.FirstStep ? 1 : 0; }
+    else C.first_step = 1;
+    rc = write_state(h, h->host.data(), n, 0);
+    if (rc) return rc;
+    rc = push_ctl(h);
+    if (rc) return rc;
+    launch_keys(h->D[0], P, h->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    h->uploaded = true;
+    return 0;
+}
+
+int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_status *st)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "psph_run before psph_upload");
+    if (h->broken) return fail(PSPH_ERR_STATE, "the solver stopped on an earlier error: %s", g_err.c_str());
+    psph_status s;
+    std::memset(&s, 0, sizeof s);
+    // read the control block, set the run limits
+    int rc = sync_ctl(h);
+    if (rc) return rc;
+    Ctl &C = *h->ctl_host;
+    C.tf = tf;
+    C.tout = tout;
+    C.max_steps = max_steps;
+    C.halt = kRun;
+    rc = push_ctl(h);
+    if (rc) return rc;
+    launch_head(h->D[0], h->P, h->stream);
+    long long queued = 0;
+    bool pending_finish = C.mid_step != 0 && max_steps > 0;
+    const long long batch = 16;
+    for (;;) {
+        // queue a batch; every kernel is a no-op once the device has halted
+        int d = cur_src(h);
+        long long nq = std::min<long long>(batch, max_steps - queued);
+        if (pending_finish && nq < 1) nq = 1;
+        for (long long k = 0; k < nq; ++k) {
+            if (pending_finish) {
+                // the pending step's Move runs on the variant whose reorder produced the state
+                enqueue_step(h, d ^ 1, true);
+                pending_finish = false;
+            } else {
+                enqueue_step(h, d, false);
+                d ^= 1;
+            }
+            ++queued;
+        }
+        HIPCHECK(hipGetLastError());
+        rc = sync_ctl(h);
+        if (rc) return rc;
+        if (C.reorders > 0) h->trans_ok = true;
+        if (C.halt == kRun) {
+            if (queued >= max_steps) break;   // cannot happen: the tail halts at max_steps
+            continue;
+        }
+        if (C.halt == kHaltLeave) {
+            long long nd = 0;
+            rc = delete_leavers(h, &nd);
+            if (rc) { h->broken = true; return rc; }
+            s.deleted += nd;
+            if (C.steps_done >= max_steps || !(C.time < tf)) {
+                C.halt = (C.steps_done >= max_steps) ? kHaltSteps : kHaltTend;
+                break;
+            }
+            queued = C.steps_done;
+            continue;
+        }
+        break;
+    }
+    s.time = C.time;
+    s.deltat = C.deltat;
+    s.deltatmin = C.deltatmin;
+    s.steps = C.steps_done;
+    s.pairs = (int64_t)C.last_pairs;
+    s.n = (size_t)h->P.n;
+    s.mid_step = C.mid_step;
+    s.stop_reason = (C.halt == kHaltOutput) ? PSPH_STOP_OUTPUT : (C.halt == kHaltTend) ? PSPH_STOP_TEND : PSPH_STOP_STEPS;
+    if (st) *st = s;
+    if (C.halt == kHaltError) {
+        h->broken = true;
+        if (C.error == PSPH_ERR_TIMESTEP)
+            return fail(PSPH_ERR_TIMESTEP, "Too small time step, please choose a smaller time step initially to make the simulation more stable");
+        return fail(C.error ? C.error : PSPH_ERR_UNSUPPORTED,
+                    "pairs between particles of different materials (CalcForce12/13) are outside this path");
+    }
+    return 0;
+}
+
+// The pending-finish variant: a step halted for output ran its reorder on variant d, leaving
+// the state in that variant's destination buffers; cur_src() counts that reorder, so the
+// finishing integrate must run on the variant whose *destination* is current, i.e. d = cur^1.
+
+int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "download before upload");
+    if (n) *n = (size_t)h->P.n;
+    if (!p) return 0;
+    if (cap < (size_t)h->P.n) return fail(PSPH_ERR_CAPACITY, "need room for %d particles", h->P.n);
+    int rc = read_state(h);
+    if (rc) return rc;
+    std::memcpy(p, h->host.data(), sizeof(psph_particle)*h->host.size());
+    return 0;
+}
+
+int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs)
+{
+    if (!h || !npairs) return fail(PSPH_ERR_INVALID, "null argument");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "pairs before upload");
+    int rc = sync_ctl(h);
+    if (rc) return rc;
+    if (h->ctl_host->mid_step) return fail(PSPH_ERR_STATE, "pair query while a step is pending");
+    const Params &P = h->P;
+    int d = cur_src(h);
+    const Dev &D = h->D[d];
+    // a sort of the current positions (the cell structure after Move; the order is canonical, so
+    // an extra sort changes no later result)
+    Ctl save = *h->ctl_host;
+    h->ctl_host->halt = kRun;
+    rc = push_ctl(h);
+    if (rc) return rc;
+    launch_scan(D, P, h->stream);
+    launch_scatter(D, P, h->stream);
+    launch_cellsort(D, P, h->stream);
+    launch_reorder(D, P, h->stream);
+    const Dev &E = h->D[d];   // reorder wrote into D's destination buffers
+    launch_pairs(E, P, h->stream, 0, nullptr, h->pair_cnt, nullptr);
+    std::vector<int> cnt(P.n);
+    HIPCHECK(hipMemcpyAsync(cnt.data(), h->pair_cnt, sizeof(int)*P.n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    std::vector<long long> off(P.n);
+    long long tot = 0;
+    for (int i = 0; i < P.n; ++i) { off[i] = tot; tot += cnt[i]; }
+    *npairs = (size_t)tot;
+    long long *out_dev = nullptr;
+    if (pairs && (size_t)tot <= cap && tot > 0) {
+        HIPCHECK(hipMemcpy(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc((void **)&out_dev, sizeof(long long)*2*tot));
+        launch_pairs(E, P, h->stream, 1, h->pair_off, nullptr, out_dev);
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipMemcpy(pairs, out_dev, sizeof(long long)*2*tot, hipMemcpyDeviceToHost));
+        hipFree(out_dev);
+    }
+    // restore the control block (the reorder counter keeps the new parity) and the next keys
+    rc = sync_ctl(h);
+    if (rc) return rc;
+    long long reorders = h->ctl_host->reorders;
+    *h->ctl_host = save;
+    h->ctl_host->reorders = reorders;
+    rc = push_ctl(h);
+    if (rc) return rc;
+    launch_keys(h->D[cur_src(h)], P, h->stream);
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    h->trans_ok = false;
+    if (pairs && (size_t)tot > cap) return fail(PSPH_ERR_CAPACITY, "need room for %lld pairs", tot);
+    return 0;
+}
+
+int psph_get_grid(psph_handle *h, psph_grid *g)
+{
+    if (!h || !g) return fail(PSPH_ERR_INVALID, "null argument");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "grid before upload");
+    *g = h->grid;
+    return 0;
+}
+
+int psph_set_timing(psph_handle *h, int enable)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    h->timing = enable != 0;
+    for (int k = 0; k < KCount; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
+    return 0;
+}
+
+int psph_kernel_stats(psph_handle *h, char *names, size_t names_len, double *mean_ms,
+                      int64_t *launches, size_t cap, size_t *nkernels)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (nkernels) *nkernels = KCount;
+    std::string all;
+    for (int k = 0; k < KCount; ++k) {
+        if (k) all += ",";
+        all += kKernelNames[k];
+        if ((size_t)k < cap) {
+            if (mean_ms) mean_ms[k] = h->t_n[k] ? h->t_ms[k]/h->t_n[k] : 0.0;
+            if (launches) launches[k] = h->t_n[k];
+        }
+    }
+    if (names && names_len) std::snprintf(names, names_len, "%s", all.c_str());
+    return 0;
+}
+
+int psph_stream(psph_handle *h, void **stream)
+{
+    if (!h || !stream) return fail(PSPH_ERR_INVALID, "null argument");
+    *stream = (void *)h->stream;
+    return 0;
+}
+
+// Compulsory HBM bytes per step, each array read or written once per kernel, FP64 fluid:
+// keys 32, sort 16, cell table 8/cell, reorder 2 x 112, wall 64 + 32 f_fix, force 112 + 24 f_fix,
+// integrate 192 f_free (per particle).
+int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_bytes)
+{
+    if (!h || !h->uploaded) return fail(PSPH_ERR_STATE, "traffic model before upload");
+    double N = h->P.n, Nf = h->P.nfree, Nx = h->P.nfixed, C = h->P.ncells;
+    // force gather: read x, v, rho, P, m, flags of every particle once (80 B) and NSv of the
+    // walls (24 B); write a and dDensity of the free particles (32 B)
+    double force = 80.0*N + 24.0*Nx + 32.0*Nf;
+    double wall = 64.0*N + 32.0*Nx;
+    double total = 32.0*N + 16.0*N + 8.0*C + 224.0*N + wall + force + 192.0*Nf;
+    if (bytes_per_step) *bytes_per_step = total;
+    if (force_bytes) *force_bytes = force;
+    return 0;
+}
+
+int psph_destroy(psph_handle *h)
+{
+    if (!h) return 0;
+    free_all(h);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+}  // extern "C"

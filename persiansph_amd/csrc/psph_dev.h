@@ -1,0 +1,123 @@
+// Internal device-side data structures of the persiansph_amd GPU path (gfx950).
+//
+// Particles live in HBM as structure-of-arrays, sorted by the reference's cell key
+// kx + nx*(ky + ny*kz) (the CellInitiate grid, Source/Domain.cpp:549-660), and inside a
+// cell by original (reference) index.  Every step re-sorts: counting sort on the cell key
+// (a one-digit radix sort whose digit is the whole key), cell-start table, per-cell order by
+// original index, then a gather of the persistent columns into the new order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psph {
+
+constexpr int kMaxClasses = 64;
+
+// Constant Particle fields shared by a group of particles (Source/Particle.h:29-149).
+// Built on the host at upload from the distinct tuples present; particles carry an index.
+struct ClassParams {
+    double h, Cs, P0, RefDensity, FPMassC;
+    double Alpha, Beta, MuRef, T0, m, CSmag;
+    double TI, TIn, TIInitDist;
+    double G, K, Sigmay, c, phi, psi;
+    // derived constants (computed once on the host from the fields above)
+    double dp_alpha2, dp_kf2, dp_alpha3, dp_kf3;   // Drucker-Prager alpha, kf (plane strain, 3-D)
+    double sin_psi;                                // sin(psi) for the non-associated flow rule
+    int32_t IsFree, NoSlip, Shepard, LES, Material, PresEq, VisM, Fail, ID, ShepardStep, InOut, pad;
+};
+
+// Flags word per particle.
+constexpr uint32_t kClsMask = 0xffffu;
+constexpr uint32_t kFree = 1u << 16;
+constexpr uint32_t kNoSlip = 1u << 17;
+constexpr int kMatShift = 18;      // 2 bits: Material 1..3
+
+__host__ __device__ inline int flag_cls(uint32_t f) { return (int)(f & kClsMask); }
+__host__ __device__ inline bool flag_free(uint32_t f) { return (f & kFree) != 0; }
+__host__ __device__ inline bool flag_noslip(uint32_t f) { return (f & kNoSlip) != 0; }
+__host__ __device__ inline int flag_mat(uint32_t f) { return (int)((f >> kMatShift) & 3u); }
+
+// Halt reasons written by the device into Ctl::halt.
+enum : int {
+    kRun = 0,
+    kHaltSteps = 1,    // max_steps reached
+    kHaltTend = 2,     // Time >= tf at a step head
+    kHaltOutput = 3,   // Time >= tout after the force pass (mid-step)
+    kHaltLeave = 4,    // particles outside [BLPF, TRPR] after Move
+    kHaltError = 5,    // see Ctl::error
+};
+
+// Solve-loop scalars, resident in device memory so steps run without host round trips.
+struct Ctl {
+    double time, deltat, deltatint, deltatmin;
+    double tf, tout;
+    unsigned long long dtmin_bits;        // atomic min of sqrt_h_a*sqrt(h/|a|) (positive doubles)
+    long long steps_done, max_steps;
+    unsigned long long pairs_ff, pairs_fw;  // directed free-free and free-fixed in-support pairs
+    unsigned long long last_pairs;        // unordered in-support pairs of the last force pass
+    int halt, error, mid_step, resume;
+    int leave, xmat, nfree_seen, nfixed_seen;
+    int ct, shep_counter, first_step, pad;
+    long long reorders;                 // completed reorders (buffer parity)
+};
+
+// Run-wide constants.
+struct Params {
+    double blpf[3], trpr[3], csize[3];
+    double hmax, cellfac, sqrt_h_a, xsph;
+    double gravity[3];
+    double eos_cs, eos_p0, eos_rho0;   // the free particles' EOS (uniform, checked at upload)
+    int eos_eq;
+    int nc[3], ncells;
+    int n, nfree, nfixed, nclasses;
+    int dim, kernel_type, visc_eq, gradient_type, scheme, precision;
+    int shepard_step;                  // uniform over free particles
+    int any_ti, any_sr, any_xsph, any_shepard, any_solid, uniform_h, any_les_t0;
+    double h_uniform;
+};
+
+// Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
+struct Dev {
+    Ctl *ctl;
+    ClassParams *cls;
+    // persistent (current order)
+    double4 *pos;      // x, y, z, Density
+    double4 *vel;      // vx, vy, vz, Pressure
+    double2 *mmu;      // Mass, Mu
+    double4 *vh;       // Leapfrog: va, Densitya (free) / MVerlet: vb, Densityb; fixed: vb, Densityb
+    uint32_t *flags;
+    int32_t *orig;
+    double *sig;       // Sigma (6: xx xy xz yy yz zz)
+    double *sigh;      // Leapfrog Sigmaa / MVerlet Sigmab
+    double *tau;       // ShearStress (6)
+    double *tauh;      // Leapfrog ShearStressa / MVerlet ShearStressb
+    double *eps;       // Strain (6)
+    double *epsh;      // Leapfrog Straina / MVerlet Strainb
+    // persistent, other buffer (reorder destination)
+    double4 *pos_b, *vel_b, *vh_b;
+    double2 *mmu_b;
+    uint32_t *flags_b;
+    int32_t *orig_b;
+    double *sig_b, *sigh_b, *tau_b, *tauh_b, *eps_b, *epsh_b;
+    // transient (current order)
+    double4 *aux;      // Mass, Mu, sound speed, P/d^2  (fixed: wall density-based values)
+    double4 *wal;      // fixed: NSv x,y,z, wall density d_w
+    double4 *acc;      // a x,y,z, dDensity
+    double4 *xsp;      // VXSPH x,y,z, ZWab
+    double2 *shep;     // SumDen, SumKernel
+    double *sr;        // StrainRate (6)
+    double *rr;        // RotationRate (3: xy xz yz)
+    double *tir;       // TIR (6)
+    double *shr;       // ShearRate, SBar (2)
+    int32_t *key;      // cell key of each particle in the current order
+    int32_t *rank;     // arrival rank within its cell (counting sort)
+    int32_t *key_b;    // key in the new order
+    int32_t *src;      // new position -> old position
+    int32_t *cstart;   // cell start table (ncells + 1)
+    int32_t *ccount;   // per-cell counts (zeroed by the scan)
+    int32_t *bsum;     // scan block sums
+    int32_t *free_list, *fixed_list;
+};
+
+}  // namespace psph

@@ -1,0 +1,1126 @@
+// HIP kernels of the PersianSPH Solve hot path for MI355X (gfx950, wave64).
+//
+// One time step (the loop body of Domain::Solve, Source/Domain.cpp:1762-1793) is:
+//   scan       cell counts -> cell-start table            (replaces ListGenerate's HOC chains)
+//   scatter    counting-sort scatter by cell key
+//   cellsort   order inside each cell by original index   (deterministic, history-free order)
+//   reorder    gather persistent columns into the new order, target lists, TIR of free soil
+//   wall       fixed-particle smoothing gather            (PrimaryComputeAcceleration)
+//   force      free-particle pair gather                  (LastComputeAcceleration, CalcForce11/2233)
+//   integrate  AdaptiveTimeStep + Move + Mat1/Mat2/Mat3 + leave check + next-step cell keys
+//   tail       Time += dt, step counters, stop conditions
+// Every kernel returns at once when the device control block says the run has halted, so the
+// host may queue many steps ahead without round trips.
+//
+// Pair sets: a free particle gathers every candidate in the 27 (3-D) or 9 (2-D) cells of the
+// reference grid around its own and keeps those with rij/h <= Cellfac, exactly the pairs
+// MainNeighbourSearch + CalcForce* evaluate.  Each pair contribution is computed with the
+// gathering particle as P1; the reference's pair formulas are antisymmetric to the bit, so this
+// equals its contribution whichever particle the reference's search made P1.  The one
+// orientation-dependent term, the hydrostatic wall term of PrimaryComputeAcceleration
+// (Domain.cpp:1045,1054), is oriented by the reference rule (smaller cell key, or larger
+// original index within a cell, is P1).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "psph_dev.h"
+#include "sph_math.h"
+
+namespace psph {
+
+constexpr int kBlock = 256;
+constexpr int kForceBlock = 128;
+constexpr int kCap = 24;            // per-lane neighbour list slots in LDS before a flush
+constexpr int kScanTile = 2048;     // cells per scan block (256 threads x 8)
+
+__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+__device__ inline int wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// ------------------------------------------------------------------------------------------
+// cell keys (ListGenerate, Domain.cpp:662-784): floor((x-BLPF)/CellSize), clamped to the grid
+// for particles within hmax outside it
+// ------------------------------------------------------------------------------------------
+__device__ inline int cell_key(const Params &P, double x, double y, double z, int &kx, int &ky, int &kz)
+{
+    double xs[3] = {x, y, z};
+    int ijk[3] = {0, 0, 0};
+    for (int c = 0; c < P.dim; ++c) {
+        int i = (int)(floor((xs[c] - P.blpf[c]) / P.csize[c]));
+        if (i < 0) i = 0;
+        if (i >= P.nc[c]) i = P.nc[c] - 1;
+        ijk[c] = i;
+    }
+    kx = ijk[0]; ky = ijk[1]; kz = ijk[2];
+    return ijk[0] + P.nc[0]*(ijk[1] + P.nc[1]*ijk[2]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    double4 p = D.pos[i];
+    int kx, ky, kz;
+    int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
+    D.key[i] = key;
+    D.rank[i] = atomicAdd(&D.ccount[key], 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// exclusive scan of the cell counts -> cell-start table; counts are cleared for the next step
+// ------------------------------------------------------------------------------------------
+__device__ inline int block_exclusive_scan(int v, int *sh, int &total)
+{
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { int t = sh[k]; sh[k] = s; s += t; }
+        sh[16] = s;
+    }
+    __syncthreads();
+    total = sh[16];
+    int r = x - v + sh[w];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ int sh[17];
+    int base = blockIdx.x*kScanTile + threadIdx.x*8;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { int c = base + k; if (c < P.ncells) s += D.ccount[c]; }
+    int total;
+    block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) D.bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(Dev D, Params P, int nblocks)
+{
+    if (D.ctl->halt) return;
+    __shared__ int sh[17];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblocks; b0 += 1024) {
+        int b = b0 + threadIdx.x;
+        int v = b < nblocks ? D.bsum[b] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, sh, total);
+        if (b < nblocks) D.bsum[b] = ex + carry;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        D.cstart[P.ncells] = carry;
+        D.ctl->nfree_seen = 0;
+        D.ctl->nfixed_seen = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ int sh[17];
+    int base = blockIdx.x*kScanTile + threadIdx.x*8;
+    int v[8];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int c = base + k;
+        v[k] = (c < P.ncells) ? D.ccount[c] : 0;
+        s += v[k];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, sh, total) + D.bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int c = base + k;
+        if (c < P.ncells) { D.cstart[c] = ex; D.ccount[c] = 0; }
+        ex += v[k];
+    }
+}
+
+// counting-sort scatter: new position = cell start + arrival rank
+__global__ void __launch_bounds__(kBlock) k_scatter(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    int k = D.key[i];
+    int pos = D.cstart[k] + D.rank[i];
+    D.src[pos] = i;
+    D.key_b[pos] = k;
+}
+
+// order inside each cell by original index (insertion sort; cells hold tens of particles)
+__global__ void __launch_bounds__(kBlock) k_cellsort(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    int c = blockIdx.x*kBlock + threadIdx.x;
+    if (c >= P.ncells) return;
+    int b = D.cstart[c], e = D.cstart[c+1];
+    for (int i = b + 1; i < e; ++i) {
+        int s = D.src[i];
+        int o = D.orig[s];
+        int j = i - 1;
+        while (j >= b) {
+            int sj = D.src[j];
+            if (D.orig[sj] <= o) break;
+            D.src[j+1] = sj;
+            --j;
+        }
+        D.src[j+1] = s;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// reorder: gather persistent columns into the new order; build target lists; per-particle
+// derived values used by the pair gathers; TIR of free soil/solid (StartAcceleration
+// Domain.cpp:951-988)
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    int i = blockIdx.x*kBlock + threadIdx.x;
+    bool act = i < P.n;
+    uint32_t f = 0;
+    if (act) {
+        int s = D.src[i];
+        double4 p = D.pos[s], v = D.vel[s], vh = D.vh[s];
+        double2 mm = D.mmu[s];
+        f = D.flags[s];
+        D.pos_b[i] = p;
+        D.vel_b[i] = v;
+        D.vh_b[i] = vh;
+        D.mmu_b[i] = mm;
+        D.flags_b[i] = f;
+        D.orig_b[i] = D.orig[s];
+        D.key[i] = D.key_b[i];
+        const ClassParams &c = cp[flag_cls(f)];
+        if (flag_free(f)) {
+            // sound speed and P/rho^2 of a free particle (CalcForce11 :67-72, :139)
+            double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
+            double pr = v.w/(p.w*p.w);
+            D.aux[i] = make_double4(mm.x, mm.y, C, pr);
+        } else {
+            D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
+        }
+        if (P.any_solid) {
+            double sg[6];
+            for (int q = 0; q < 6; ++q) {
+                sg[q] = D.sig[6*s+q];
+                D.sig_b[6*i+q] = sg[q];
+                D.sigh_b[6*i+q] = D.sigh[6*s+q];
+                D.tau_b[6*i+q] = D.tau[6*s+q];
+                D.tauh_b[6*i+q] = D.tauh[6*s+q];
+                D.eps_b[6*i+q] = D.eps[6*s+q];
+                D.epsh_b[6*i+q] = D.epsh[6*s+q];
+            }
+            if (flag_free(f)) {
+                // TIR is rebuilt every step; entries the 2-D formula leaves alone are zero
+                double t[6] = {0, 0, 0, 0, 0, 0};
+                if (c.Material > 1 && c.TI > 0.0) tir_tensor(P.dim, sg, c.TI, p.w, t);
+                for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t[q];
+            }
+        }
+    }
+    if (act && P.dim == 2) {
+        // StartAcceleration zeroes v(2) in 2-D (Domain.cpp:1011)
+        double4 v2 = D.vel_b[i];
+        v2.z = 0.0;
+        D.vel_b[i] = v2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl->reorders += 1;
+    // target lists (order within a list does not change any result)
+    bool isfree = act && flag_free(f);
+    bool isfix = act && !flag_free(f);
+    unsigned long long bf = __ballot(isfree), bx = __ballot(isfix);
+    int lane = threadIdx.x & 63;
+    unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int basef = 0, basex = 0;
+    if (lane == 0) {
+        if (bf) basef = atomicAdd(&D.ctl->nfree_seen, __popcll(bf));
+        if (bx) basex = atomicAdd(&D.ctl->nfixed_seen, __popcll(bx));
+    }
+    basef = __shfl(basef, 0, 64);
+    basex = __shfl(basex, 0, 64);
+    if (isfree) D.free_list[basef + __popcll(bf & lt)] = i;
+    if (isfix) D.fixed_list[basex + __popcll(bx & lt)] = i;
+}
+
+// ------------------------------------------------------------------------------------------
+// neighbour rows of a cell: 9 (3-D) or 3 (2-D) contiguous ranges of the sorted order
+// ------------------------------------------------------------------------------------------
+__device__ inline void row_range(const Dev &D, const Params &P, int kx, int ky, int kz, int r,
+                                 int &beg, int &end)
+{
+    int dy = (r % 3) - 1, dz = (P.dim == 3) ? (r / 3) - 1 : 0;
+    int yy = ky + dy, zz = kz + dz;
+    beg = end = 0;
+    if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) return;
+    int base = P.nc[0]*(yy + P.nc[1]*zz);
+    int x0 = kx > 0 ? kx - 1 : 0;
+    int x1 = kx + 1 < P.nc[0] ? kx + 1 : P.nc[0] - 1;
+    beg = D.cstart[base + x0];
+    end = D.cstart[base + x1 + 1];
+}
+
+__device__ inline void decode_key(const Params &P, int key, int &kx, int &ky, int &kz)
+{
+    kx = key % P.nc[0];
+    int t = key / P.nc[0];
+    ky = t % P.nc[1];
+    kz = t / P.nc[1];
+}
+
+__device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, int cj)
+{
+    return P.uniform_h ? P.h_uniform : (cp[ci].h + cp[cj].h)/2;
+}
+
+// ------------------------------------------------------------------------------------------
+// wall pass: PrimaryComputeAcceleration (Domain.cpp:1020-1059, 1163-1206) for fixed targets
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    int t = blockIdx.x*kBlock + threadIdx.x;
+    if (t >= P.nfixed) return;
+    int i = D.fixed_list[t];
+    uint32_t fi = D.flags_b[i];
+    int ci = flag_cls(fi);
+    const ClassParams &C = cp[ci];
+    int mat = flag_mat(fi);
+    double4 pi = D.pos_b[i];
+    int key_i = D.key[i];
+    int orig_i = D.orig_b[i];
+    int kx, ky, kz;
+    decode_key(P, key_i, kx, ky, kz);
+    double SK = 0.0, Pw = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    double sg[6] = {0, 0, 0, 0, 0, 0};
+    const double *G = P.gravity;
+    int nrows = (P.dim == 3) ? 9 : 3;
+    for (int r = 0; r < nrows; ++r) {
+        int beg, end;
+        row_range(D, P, kx, ky, kz, r, beg, end);
+        for (int j = beg; j < end; ++j) {
+            uint32_t fj = D.flags_b[j];
+            if (!flag_free(fj) || flag_mat(fj) != mat) continue;
+            double4 pj = D.pos_b[j];
+            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+            double h = pair_h(cp, P, ci, flag_cls(fj));
+            double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+            if (!((rij/h) <= P.cellfac)) continue;      // outside the reference's pair set
+            // orientation of the reference's pair (P1 = smaller key / larger index); xij is
+            // x_P1 - x_P2 (negation is exact, rij unchanged)
+            int kj = D.key[j];
+            bool wall_first = (key_i < kj) || (key_i == kj && orig_i > D.orig_b[j]);
+            if (!wall_first) { x0 = -x0; x1 = -x1; x2 = -x2; }
+            double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h);
+            SK += K;
+            if (mat < 3) {
+                double4 vj = D.vel_b[j];
+                Pw += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
+            }
+            if (mat > 1)
+                for (int q = 0; q < 6; ++q) sg[q] = sg[q] + D.sig_b[6*j+q]*K;
+            if (C.NoSlip) {
+                double4 vj = D.vel_b[j];
+                n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K;
+            }
+        }
+    }
+    // StartAcceleration resets fixed P, NSv, Sigma (Domain.cpp:990-996); then divide by SumKernel
+    if (SK != 0.0) {
+        if (mat < 3) Pw = Pw/SK;
+        if (mat > 1) for (int q = 0; q < 6; ++q) sg[q] = sg[q]*(1.0/SK);
+        if (C.NoSlip) { n0 = n0/SK; n1 = n1/SK; n2 = n2/SK; }
+        else { n0 = n1 = n2 = 0.0; }
+    } else {
+        Pw = 0.0; n0 = n1 = n2 = 0.0;
+        for (int q = 0; q < 6; ++q) sg[q] = 0.0;
+    }
+    double4 vi = D.vel_b[i];
+    vi.w = Pw;
+    D.vel_b[i] = vi;
+    D.shep[i] = make_double2(0.0, SK);
+    // wall density and sound speed as the free partner sees them (CalcForce11 :41-61, :69-70)
+    double dw = density_solid(P.eos_eq, P.eos_cs, P.eos_p0, Pw, P.eos_rho0);
+    double Cw = sound_speed(P.eos_eq, P.eos_cs, dw, P.eos_rho0);
+    double2 mm = D.mmu_b[i];
+    D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
+    D.wal[i] = make_double4(n0, n1, n2, dw);
+    if (P.any_solid) {
+        if (mat > 1) for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = sg[q];
+        else for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = 0.0;
+        for (int q = 0; q < 6; ++q) D.tau_b[6*i+q] = 0.0;
+        double t6[6] = {0, 0, 0, 0, 0, 0};
+        if (mat > 1 && C.TI > 0.0 && SK != 0.0) tir_tensor(P.dim, sg, C.TI, pi.w, t6);
+        for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t6[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// force pass: CalcForce11 / CalcForce2233 gathered per free particle
+// ------------------------------------------------------------------------------------------
+struct Acc {
+    double a0, a1, a2, dd;
+    double vx0, vx1, vx2, zw, sd;
+    double sr[6];
+    double rr[3];
+};
+
+// CalcForce11 (Interaction.cpp:25-184) contribution of neighbour j to free particle i as P1.
+__device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+                              uint32_t fi, double4 pi, double4 vi, double4 ai, Acc &A, int shep_on)
+{
+    uint32_t fj = D.flags_b[j];
+    int ci = flag_cls(fi), cj = flag_cls(fj);
+    const ClassParams &Ci_ = cp[ci];
+    const ClassParams &Cj_ = cp[cj];
+    double4 pj = D.pos_b[j];
+    double4 vj = D.vel_b[j];
+    double4 aj = D.aux[j];
+    bool freej = flag_free(fj);
+    double h = pair_h(cp, P, ci, cj);
+    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+    double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+    double Alpha = (Ci_.Alpha + Cj_.Alpha)/2.0;
+    double Beta = (Ci_.Beta + Cj_.Beta)/2.0;
+    double u0 = vi.x - vj.x, u1 = vi.y - vj.y, u2 = vi.z - vj.z;
+    double di = pi.w, mi = ai.x;
+    double dj, mj;
+    double4 wj = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (!freej) {
+        wj = D.wal[j];
+        dj = wj.w;
+        mj = Cj_.FPMassC * mi;
+    } else {
+        dj = pj.w;
+        mj = aj.x;
+    }
+    double q = rij/h;
+    double GK = grad_kernel<double>(P.dim, P.kernel_type, q, h);
+    double K = kernel_w<double>(P.dim, P.kernel_type, q, h);
+    double PIij = 0.0;
+    if (Alpha != 0.0 || Beta != 0.0) {
+        double Cii = ai.z, Cjj = aj.z;
+        double dvx = u0*x0 + u1*x1 + u2*x2;
+        double MUij = h*dvx/(rij*rij+0.01*h*h);
+        if (dvx < 0) PIij = (-Alpha*0.5*(Cii+Cjj)*MUij+Beta*MUij*MUij)/(0.5*(di+dj));
+    }
+    double TIij = 0.0;
+    if (Ci_.TI > 0.0 || Cj_.TI > 0.0) {
+        double Ri = 0.0, Rj = 0.0;
+        if ((vi.w+vj.w) < 0.0) {
+            if (vi.w < 0.0) Ri = -vi.w/(di*di);
+            if (vj.w < 0.0) Rj = -vj.w/(dj*dj);
+        }
+        TIij = (Ci_.TI*Ri + Cj_.TI*Rj)*pow((K/kernel_w<double>(P.dim, P.kernel_type, (Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*h), h)),(Ci_.TIn+Cj_.TIn)/2.0);
+    }
+    double VI0 = 0.0, VI1 = 0.0, VI2 = 0.0;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    bool sr_on = false;
+    double s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0;
+    if ((Ci_.NoSlip || Cj_.NoSlip) || freej) {
+        double Mu;
+        if (freej) {
+            b0 = u0; b1 = u1; b2 = u2;
+            Mu = 2.0*ai.y*aj.y/(ai.y+aj.y);
+        } else {
+            b0 = vi.x - (2.0*vj.x-wj.x);
+            b1 = vi.y - (2.0*vj.y-wj.y);
+            b2 = vi.z - (2.0*vj.z-wj.z);
+            Mu = ai.y;
+        }
+        if (Ci_.T0 > 0.0 || Cj_.T0 > 0.0 || (Ci_.LES && Cj_.LES)) {
+            sr_on = true;
+            double g = -GK;
+            s00 = (2.0*b0*x0)*g;
+            s01 = (b0*x1+b1*x0)*g;
+            s02 = (b0*x2+b2*x0)*g;
+            s11 = (2.0*b1*x1)*g;
+            s12 = (b1*x2+b2*x1)*g;
+            s22 = (2.0*b2*x2)*g;
+        }
+        viscous_force<double>(P.visc_eq, VI0, VI1, VI2, Mu, di, dj, GK, b0, b1, b2, P.dim,
+                              P.kernel_type, rij, h, x0, x1, x2, u0, u1, u2);
+    }
+    if (P.xsph != 0.0 && freej) {
+        double sx = P.xsph*mj/(0.5*(di+dj))*K;
+        A.vx0 += sx*-u0; A.vx1 += sx*-u1; A.vx2 += sx*-u2;
+    }
+    double s;
+    if (P.gradient_type == 0) s = -1.0*( ai.w + aj.w + PIij + TIij ) * GK;
+    else s = -1.0*( (vi.w + vj.w)/(di*dj) + PIij + TIij ) * GK;
+    double t0 = s*x0 + VI0, t1 = s*x1 + VI1, t2 = s*x2 + VI2;
+    if (P.dim == 2) t2 = 0.0;
+    double g0 = GK*x0, g1 = GK*x1, g2 = GK*x2;
+    double temp1 = u0*g0 + u1*g1 + u2*g2;
+    A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
+    A.dd += mj * (di/dj) * temp1;
+    if (sr_on && (Ci_.T0 > 0.0 || Ci_.LES)) {
+        double fq = mj/dj;
+        A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
+        A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
+    }
+    if (shep_on) {
+        A.zw += mj/dj* K;
+        A.sd += mj*K;
+    }
+}
+
+// CalcForce2233 (Interaction.cpp:186-377) contribution of neighbour j to free particle i as P1.
+__device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+                                uint32_t fi, double4 pi, double4 vi, double4 ai, const double *sgi,
+                                const double *tri, Acc &A, int shep_on)
+{
+    uint32_t fj = D.flags_b[j];
+    int ci = flag_cls(fi), cj = flag_cls(fj);
+    const ClassParams &Ci_ = cp[ci];
+    const ClassParams &Cj_ = cp[cj];
+    double4 pj = D.pos_b[j];
+    double4 vj = D.vel_b[j];
+    double4 aj = D.aux[j];
+    bool freej = flag_free(fj);
+    double h = pair_h(cp, P, ci, cj);
+    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+    double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+    double Alpha = (Ci_.Alpha + Cj_.Alpha)/2.0;
+    double Beta = (Ci_.Beta + Cj_.Beta)/2.0;
+    double di = pi.w, mi = ai.x, dj, mj;
+    bool soil = (Ci_.Material*Cj_.Material == 9);
+    double4 wj = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (!freej) wj = D.wal[j];
+    if (soil) {
+        if (!freej) { dj = pi.w; mj = Cj_.FPMassC * mi; } else { dj = pj.w; mj = aj.x; }
+    } else {
+        if (!freej) { dj = wj.w; mj = Cj_.FPMassC * mi; } else { dj = pj.w; mj = aj.x; }
+    }
+    double u0 = vi.x - vj.x, u1 = vi.y - vj.y, u2 = vi.z - vj.z;
+    double q = rij/h;
+    double GK = grad_kernel<double>(P.dim, P.kernel_type, q, h);
+    double K = kernel_w<double>(P.dim, P.kernel_type, q, h);
+    double PIs = 0.0;
+    bool pi_on = false;
+    if (Alpha != 0.0 || Beta != 0.0) {
+        double dvx = u0*x0 + u1*x1 + u2*x2;
+        double MUij = h*dvx/(rij*rij+0.01*h*h);
+        double Cij;
+        if (soil) Cij = 0.5*(Ci_.Cs+Cj_.Cs);
+        else Cij = 0.5*(ai.z+aj.z);
+        if (dvx < 0) { PIs = (Alpha*Cij*MUij+Beta*MUij*MUij)/(0.5*(di+dj)); pi_on = true; }
+    }
+    double TI6[6] = {0, 0, 0, 0, 0, 0};
+    if (Ci_.TI > 0.0 || Cj_.TI > 0.0) {
+        double sT = pow((K/kernel_w<double>(P.dim, P.kernel_type, (Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*h), h)),(Ci_.TIn+Cj_.TIn)/2.0);
+        for (int k = 0; k < 6; ++k) TI6[k] = (tri[k]+D.tir[6*j+k])*sT;
+    }
+    double b0, b1, b2;
+    if (freej) { b0 = u0; b1 = u1; b2 = u2; }
+    else if (Ci_.NoSlip || Cj_.NoSlip) {
+        b0 = vi.x - (2.0*vj.x-wj.x); b1 = vi.y - (2.0*vj.y-wj.y); b2 = vi.z - (2.0*vj.z-wj.z);
+    } else {
+        double4 vbj = D.vh_b[j];    // fixed particles keep vb in the half-step column
+        b0 = vi.x - vbj.x; b1 = vi.y - vbj.y; b2 = vi.z - vbj.z;
+    }
+    double f05 = -0.5 * GK;
+    double s00 = (2.0*b0*x0)*f05, s01 = (b0*x1+b1*x0)*f05, s02 = (b0*x2+b2*x0)*f05;
+    double s11 = (2.0*b1*x1)*f05, s12 = (b1*x2+b2*x1)*f05, s22 = (2.0*b2*x2)*f05;
+    double r01 = (b0*x1-b1*x0)*f05, r02 = (b0*x2-b2*x0)*f05, r12 = (b1*x2-b2*x1)*f05;
+    if (P.xsph != 0.0 && freej) {
+        double sx = P.xsph*mj/(0.5*(di+dj))*K;
+        A.vx0 += sx*-u0; A.vx1 += sx*-u1; A.vx2 += sx*-u2;
+    }
+    // M = Sigma_i/di^2 + Sigma_j/dj^2 + PI I + TI, then temp = (GK xij)^T M (matvec.h:806-811)
+    double Mf[9];
+    double sgj[6];
+    for (int k = 0; k < 6; ++k) sgj[k] = D.sig_b[6*j+k];
+    double Mi[9], Mj[9], T9[9];
+    sym_to_full(sgi, Mi);
+    sym_to_full(sgj, Mj);
+    sym_to_full(TI6, T9);
+    if (P.gradient_type == 0) {
+        double fa = 1.0/(di*di), fb = 1.0/(dj*dj);
+        for (int k = 0; k < 9; ++k) {
+            double pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0)*PIs : 0.0;
+            Mf[k] = Mi[k]*fa + Mj[k]*fb + pim + T9[k];
+        }
+    } else {
+        double fa = 1.0/(di*dj);
+        for (int k = 0; k < 9; ++k) {
+            double pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0)*PIs : 0.0;
+            Mf[k] = (Mi[k] + Mj[k])*fa + pim + T9[k];
+        }
+    }
+    double g0 = GK*x0, g1 = GK*x1, g2 = GK*x2;
+    double t0 = g0*Mf[0] + g1*Mf[3] + g2*Mf[6];
+    double t1 = g0*Mf[1] + g1*Mf[4] + g2*Mf[7];
+    double t2 = g0*Mf[2] + g1*Mf[5] + g2*Mf[8];
+    if (P.dim == 2) t2 = 0.0;
+    double temp1 = u0*g0 + u1*g1 + u2*g2;
+    A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
+    A.dd += mj * (di/dj) * temp1;
+    double fq = mj/dj;
+    A.zw += mj/dj* K;
+    A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
+    A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
+    A.rr[0] = A.rr[0] + r01*fq; A.rr[1] = A.rr[1] + r02*fq; A.rr[2] = A.rr[2] + r12*fq;
+    if (shep_on) A.sd += mj* K;
+}
+
+template <bool SOLID>
+__global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ ClassParams cp[kMaxClasses];
+    __shared__ int lst[kCap*kForceBlock];
+    for (int k = threadIdx.x; k < P.nclasses; k += kForceBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    const int tid = threadIdx.x;
+    int t = blockIdx.x*kForceBlock + tid;
+    bool act = t < P.nfree;
+    int i = act ? D.free_list[t] : 0;
+    uint32_t fi = D.flags_b[i];
+    int ci = flag_cls(fi);
+    int mat = flag_mat(fi);
+    double4 pi = D.pos_b[i];
+    double4 vi = D.vel_b[i];
+    double4 ai = D.aux[i];
+    int kx, ky, kz;
+    decode_key(P, D.key[i], kx, ky, kz);
+    int shep_on = P.any_shepard && cp[ci].Shepard && (D.ctl->shep_counter == cp[ci].ShepardStep);
+    int zw_on = shep_on;
+    Acc A;
+    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
+    A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0;
+    for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
+    A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
+    double sgi[6] = {0, 0, 0, 0, 0, 0}, tri[6] = {0, 0, 0, 0, 0, 0};
+    if (SOLID) {
+        for (int k = 0; k < 6; ++k) { sgi[k] = D.sig_b[6*i+k]; tri[k] = D.tir[6*i+k]; }
+    }
+    unsigned long long nff = 0, nfw = 0;
+    int xmat = 0;
+    int cnt = 0;
+    const int nrows = (P.dim == 3) ? 9 : 3;
+    for (int r = 0; r < nrows; ++r) {
+        int beg = 0, end = 0;
+        if (act) row_range(D, P, kx, ky, kz, r, beg, end);
+        for (int j = beg; j < end; ++j) {
+            if (j != i) {
+                uint32_t fj = D.flags_b[j];
+                if (flag_mat(fj) != mat) {
+                    xmat = 1;
+                } else {
+                    double4 pj = D.pos_b[j];
+                    double h = pair_h(cp, P, ci, flag_cls(fj));
+                    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                    double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+                    if ((rij/h) <= P.cellfac) {
+                        lst[cnt*kForceBlock + tid] = j;
+                        ++cnt;
+                        if (flag_free(fj)) ++nff; else ++nfw;
+                    }
+                }
+            }
+            if (wave_any(cnt == kCap)) {
+                for (int e = 0; e < cnt; ++e) {
+                    int jj = lst[e*kForceBlock + tid];
+                    if (SOLID) pair2233(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+                    else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+                }
+                cnt = 0;
+            }
+        }
+    }
+    for (int e = 0; e < cnt; ++e) {
+        int jj = lst[e*kForceBlock + tid];
+        if (SOLID) pair2233(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+        else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+    }
+    if (act) {
+        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        if (P.any_xsph || zw_on) D.xsp[i] = make_double4(A.vx0, A.vx1, A.vx2, A.zw);
+        if (shep_on) D.shep[i] = make_double2(A.sd, 0.0);
+        if (SOLID || P.any_sr) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
+        if (SOLID) for (int k = 0; k < 3; ++k) D.rr[3*i+k] = A.rr[k];
+        // dt control: min over free particles of sqrt_h_a*sqrt(h/|a|) (Domain.cpp:1281-1296)
+        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        double test = sqrt(cp[ci].h/na);
+        double cand = P.sqrt_h_a*test;
+        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+    }
+    // pair statistics and the cross-material check
+    unsigned long long sff = nff, sfw = nfw;
+    for (int o = 32; o > 0; o >>= 1) {
+        sff += __shfl_down(sff, o, 64);
+        sfw += __shfl_down(sfw, o, 64);
+    }
+    int anyx = wave_any(xmat != 0);
+    if ((tid & 63) == 0) {
+        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
+        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
+        if (anyx) atomicOr(&D.ctl->xmat, 1);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// integrate: AdaptiveTimeStep (Domain.cpp:133-152), Move (Particle.cpp:119-274, 561-596),
+// Mat1/Mat2/Mat3 (Particle.cpp:146-186, 276-482, 598-795), CheckParticleLeave test
+// (Domain.cpp:517-547) and the next step's cell keys
+// ------------------------------------------------------------------------------------------
+__device__ inline double adaptive_dt(double deltat, double deltatint, double deltatmin)
+{
+    if (deltatint > deltatmin) {
+        if (deltat < deltatmin) return 2.0*deltat*deltatmin/(deltat+deltatmin);
+        return deltatmin;
+    }
+    if (deltatint != deltat) return 2.0*deltat*deltatint/(deltat+deltatint);
+    return deltatint;
+}
+
+__device__ inline double contract6(const double *S)
+{
+    // S00*S00 + 2 S01 S10 + 2 S02 S20 + S11*S11 + 2 S12 S21 + S22*S22 (Particle.cpp:149-151)
+    return S[0]*S[0] + 2.0*S[1]*S[1] + 2.0*S[2]*S[2] + S[3]*S[3] + 2.0*S[4]*S[4] + S[5]*S[5];
+}
+
+// R = sum of the Jaumann rate terms for Mat2 (deviatoric, on S) or Mat3 (total, on Sigma)
+__device__ inline void jaumann_rate(bool total, const double *S6, const double *sr6, const double *rr3,
+                                    double G, double Kb, double R6[6])
+{
+    double S[9], W[9], WT[9], SRT[9], RS[9];
+    sym_to_full(S6, S);
+    W[0] = 0.0;     W[1] = rr3[0];  W[2] = rr3[1];
+    W[3] = -rr3[0]; W[4] = 0.0;     W[5] = rr3[2];
+    W[6] = -rr3[1]; W[7] = -rr3[2]; W[8] = 0.0;
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) WT[3*b+a] = W[3*a+b];
+    mat_mult(S, WT, SRT);
+    mat_mult(W, S, RS);
+    double SR[9];
+    sym_to_full(sr6, SR);
+    const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double R[9];
+    if (total) {
+        double I1s = SR[0]+SR[4]+SR[8];
+        double iso = I1s*Kb, third = 1.0/3.0*I1s;
+        for (int k = 0; k < 9; ++k) R[k] = I[k]*iso + (SR[k] - I[k]*third)*(2.0*G) + SRT[k] + RS[k];
+    } else {
+        double tr3 = 1.0/3.0*(SR[0]+SR[4]+SR[8]);
+        for (int k = 0; k < 9; ++k) R[k] = (SR[k] - I[k]*tr3)*(2.0*G) + SRT[k] + RS[k];
+    }
+    full_to_sym(R, R6);
+}
+
+__device__ inline double apex_cut6(double *S, double alpha, double kf)
+{
+    double I1 = S[0] + S[3] + S[5];
+    if ((kf-alpha*I1) < 0.0) {
+        double Ratio = (alpha == 0.0) ? 0.0 : kf/alpha;
+        S[0] -= 1.0/3.0*(I1-Ratio);
+        S[3] -= 1.0/3.0*(I1-Ratio);
+        S[5] -= 1.0/3.0*(I1-Ratio);
+        I1 = Ratio;
+    }
+    return I1;
+}
+
+// Drucker-Prager return mapping (Particle.cpp:341-430 / 661-752); St trial, Sn previous level.
+__device__ inline void dp_return(const ClassParams &c, double i22, double *St, const double *Sn,
+                                 const double *sr6, double fac, double *tau6)
+{
+    double alpha = (i22 == 0.0) ? c.dp_alpha2 : c.dp_alpha3;
+    double kf = (i22 == 0.0) ? c.dp_kf2 : c.dp_kf3;
+    const double I6[6] = {1, 0, 0, 1, 0, 1};
+    double I1 = apex_cut6(St, alpha, kf);
+    double S[6];
+    for (int k = 0; k < 6; ++k) S[k] = St[k] - I6[k]*(1.0/3.0*I1);
+    double J2 = 0.5*contract6(S);
+    if ((sqrt(J2)+alpha*I1-kf) > 0.0) {
+        double trn = Sn[0]+Sn[3]+Sn[5];
+        for (int k = 0; k < 6; ++k) S[k] = Sn[k] - I6[k]*(1.0/3.0*trn);
+        J2 = 0.5*contract6(S);
+        if (sqrt(J2) > 0.0) {
+            double I1s = sr6[0]+sr6[3]+sr6[5];
+            // sum of the Hadamard product over all 9 entries (abab, Functions.cpp:350-357)
+            double sum = S[0]*sr6[0]+S[1]*sr6[1]+S[2]*sr6[2]+S[1]*sr6[1]+S[3]*sr6[3]+S[4]*sr6[4]+S[2]*sr6[2]+S[4]*sr6[4]+S[5]*sr6[5];
+            double dLanda, s1;
+            double s2 = c.G/sqrt(J2);
+            if (c.Fail == 2) {
+                dLanda = 1.0/(9.0*alpha*alpha*c.K+c.G)*((3.0*alpha*c.K*I1s) + (c.G/sqrt(J2))*sum);
+                s1 = 3.0*alpha*c.K;
+            } else {
+                dLanda = 1.0/(9.0*alpha*c.K*3.0*c.sin_psi+c.G)*((3.0*alpha*c.K*I1s) + (c.G/sqrt(J2))*sum);
+                s1 = 3.0*3.0*c.sin_psi*c.K;
+            }
+            const double Iv[6] = {1, 0, 0, 1, 0, i22};
+            for (int k = 0; k < 6; ++k) {
+                double pl = Iv[k]*s1 + S[k]*s2;
+                St[k] = St[k] - (pl*dLanda)*fac;
+            }
+        }
+        I1 = apex_cut6(St, alpha, kf);
+        for (int k = 0; k < 6; ++k) S[k] = St[k] - I6[k]*(1.0/3.0*I1);
+        J2 = 0.5*contract6(S);
+        if ((sqrt(J2)+alpha*I1-kf) > 0.0 && sqrt(J2) > 0.0) {
+            double sc = (kf-alpha*I1)/sqrt(J2);
+            for (int k = 0; k < 6; ++k) St[k] = I6[k]*(I1/3.0) + S[k]*sc;
+        }
+    }
+    for (int k = 0; k < 6; ++k) tau6[k] = S[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
+{
+    Ctl *ctl = D.ctl;
+    if (ctl->halt) return;
+    if (!ctl->resume && ctl->time >= ctl->tout) return;       // output stop before Move
+    double deltatmin = bitsd(ctl->dtmin_bits);
+    double dt = adaptive_dt(ctl->deltat, ctl->deltatint, deltatmin);
+    if (dt < (ctl->deltatint/1.0e5)) return;                  // Fatal: reported by k_tail
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t f = D.flags_b[i];
+    double4 p = D.pos_b[i];
+    const ClassParams &c = cp[flag_cls(f)];
+    bool first = ctl->first_step != 0;
+    int ct = first ? 30 : ctl->ct;
+    if (flag_free(f)) {
+        double4 v = D.vel_b[i];
+        double4 vh = D.vh_b[i];
+        double4 a = D.acc[i];
+        double rho = p.w;
+        double P_ = v.w;
+        if (P.scheme == 1) {
+            // Move_Leapfrog, Particle.cpp:561-596 (vh = va, Densitya)
+            if (first) {
+                vh.w = rho - dt/2.0*a.w;
+                vh.x = v.x - dt/2.0*a.x; vh.y = v.y - dt/2.0*a.y; vh.z = v.z - dt/2.0*a.z;
+            }
+            double rb = vh.w;
+            vh.w += dt*a.w;
+            rho = (vh.w+rb)/2.0;
+            double b0 = vh.x, b1 = vh.y, b2 = vh.z;
+            vh.x += dt*a.x; vh.y += dt*a.y; vh.z += dt*a.z;
+            v.x = (vh.x + b0)/2.0; v.y = (vh.y + b1)/2.0; v.z = (vh.z + b2)/2.0;
+            p.x += dt*vh.x; p.y += dt*vh.y; p.z += dt*vh.z;
+        } else {
+            // Move_MVerlet, Particle.cpp:189-274 (vh = vb, Densityb)
+            double4 xs = P.any_xsph ? D.xsp[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+            double s = 0.5*dt*dt;
+            p.x += dt*(v.x+xs.x) + s*a.x;
+            p.y += dt*(v.y+xs.y) + s*a.y;
+            p.z += dt*(v.z+xs.z) + s*a.z;
+            bool shep = c.Shepard && (ctl->shep_counter == c.ShepardStep);
+            double zw = 0.0, sd = 0.0;
+            if (shep) { zw = D.xsp[i].w; sd = D.shep[i].x; }
+            if (ct == 30) {
+                if (shep && zw > 0.6) { vh.w = sd/zw; rho = sd/zw; }
+                else { vh.w = rho; rho += dt*a.w; }
+                vh.x = v.x; vh.y = v.y; vh.z = v.z;
+                v.x += dt*a.x; v.y += dt*a.y; v.z += dt*a.z;
+            } else {
+                if (shep && zw > 0.6) { vh.w = sd/zw; rho = sd/zw; }
+                else { double dens = rho; rho = vh.w + 2.0*dt*a.w; vh.w = dens; }
+                double t2 = 2*dt;
+                double o0 = v.x, o1 = v.y, o2 = v.z;
+                v.x = vh.x + t2*a.x; v.y = vh.y + t2*a.y; v.z = vh.z + t2*a.z;
+                vh.x = o0; vh.y = o1; vh.z = o2;
+            }
+        }
+        double2 mm = D.mmu_b[i];
+        if (c.Material == 1) {
+            // Mat1, Particle.cpp:146-186
+            P_ = eos(c.PresEq, c.Cs, c.P0, rho, c.RefDensity);
+            if (P.any_les_t0) {
+                double sr[6];
+                for (int k = 0; k < 6; ++k) sr[k] = D.sr[6*i+k];
+                double temp = contract6(sr);
+                double ShearRate = sqrt(0.5*temp);
+                double SBar = sqrt(2.0*temp);
+                D.shr[2*i] = ShearRate;
+                D.shr[2*i+1] = SBar;
+                if (c.LES) mm.y = c.MuRef + c.RefDensity*pow((c.CSmag*c.h),2.0)*SBar;
+                if (c.T0 > 0.0) {
+                    if (c.VisM == 0) {
+                        if (ShearRate != 0.0) mm.y = c.MuRef + c.T0*(1-exp(-c.m*ShearRate))/ShearRate;
+                        else mm.y = c.MuRef + c.T0*c.m;
+                    } else {
+                        mm.y = (1000.0*c.MuRef + c.MuRef*c.MuRef*1000.0/c.T0*ShearRate)/(1+1000.0*c.MuRef/c.T0*ShearRate);
+                    }
+                }
+                D.mmu_b[i] = mm;
+            }
+        } else {
+            double sr[6], rr[3], R[6];
+            for (int k = 0; k < 6; ++k) sr[k] = D.sr[6*i+k];
+            for (int k = 0; k < 3; ++k) rr[k] = D.rr[3*i+k];
+            double sg[6], sh[6], tau[6], th[6], ep[6], eh[6];
+            for (int k = 0; k < 6; ++k) {
+                sg[k] = D.sig_b[6*i+k]; sh[k] = D.sigh_b[6*i+k];
+                tau[k] = D.tau_b[6*i+k]; th[k] = D.tauh_b[6*i+k];
+                ep[k] = D.eps_b[6*i+k]; eh[k] = D.epsh_b[6*i+k];
+            }
+            double i22 = (P.dim == 2) ? 0.0 : 1.0;
+            const double I6[6] = {1, 0, 0, 1, 0, 1};
+            if (c.Material == 2) {
+                P_ = eos(c.PresEq, c.Cs, c.P0, rho, c.RefDensity);
+                jaumann_rate(false, tau, sr, rr, c.G, c.K, R);
+                if (P.scheme == 1) {
+                    // Mat2Leapfrog, Particle.cpp:598-639 (th = ShearStressa)
+                    if (first) for (int k = 0; k < 6; ++k) th[k] = R[k]*(-dt/2.0) + tau[k];
+                    double tb[6];
+                    for (int k = 0; k < 6; ++k) { tb[k] = th[k]; th[k] = R[k]*dt + th[k]; }
+                    if (c.Fail == 1) {
+                        double J2 = 0.5*contract6(th);
+                        double am = c.Sigmay/sqrt(3.0*J2);
+                        double sc = (1.0 < am) ? 1.0 : am;
+                        for (int k = 0; k < 6; ++k) th[k] = th[k]*sc;
+                    }
+                    for (int k = 0; k < 6; ++k) tau[k] = (th[k]+tb[k])*(1.0/2.0);
+                    for (int k = 0; k < 6; ++k) sg[k] = I6[k]*(-P_) + tau[k];
+                    if (first) for (int k = 0; k < 6; ++k) eh[k] = sr[k]*(-dt/2.0) + ep[k];
+                    double ebb[6];
+                    for (int k = 0; k < 6; ++k) { ebb[k] = eh[k]; eh[k] = sr[k]*dt + eh[k]; }
+                    for (int k = 0; k < 6; ++k) ep[k] = (eh[k]+ebb[k])*(1.0/2.0);
+                } else {
+                    // Mat2MVerlet, Particle.cpp:276-318 (th = ShearStressb)
+                    double st[6];
+                    for (int k = 0; k < 6; ++k) st[k] = tau[k];
+                    if (ct == 30) for (int k = 0; k < 6; ++k) tau[k] = R[k]*dt + tau[k];
+                    else for (int k = 0; k < 6; ++k) tau[k] = R[k]*(2.0*dt) + th[k];
+                    for (int k = 0; k < 6; ++k) th[k] = st[k];
+                    if (c.Fail == 1) {
+                        double J2 = 0.5*contract6(tau);
+                        double am = c.Sigmay/sqrt(3.0*J2);
+                        double sc = (1.0 < am) ? 1.0 : am;
+                        for (int k = 0; k < 6; ++k) tau[k] = tau[k]*sc;
+                    }
+                    for (int k = 0; k < 6; ++k) sg[k] = I6[k]*(-P_) + tau[k];
+                    for (int k = 0; k < 6; ++k) st[k] = ep[k];
+                    if (ct == 30) for (int k = 0; k < 6; ++k) ep[k] = sr[k]*dt + ep[k];
+                    else for (int k = 0; k < 6; ++k) ep[k] = sr[k]*(2.0*dt) + eh[k];
+                    for (int k = 0; k < 6; ++k) eh[k] = st[k];
+                }
+            } else {
+                jaumann_rate(true, sg, sr, rr, c.G, c.K, R);
+                if (P.scheme == 1) {
+                    // Mat3Leapfrog, Particle.cpp:641-795 (sh = Sigmaa)
+                    if (first) for (int k = 0; k < 6; ++k) sh[k] = R[k]*(-dt/2.0) + sg[k];
+                    double sb[6];
+                    for (int k = 0; k < 6; ++k) { sb[k] = sh[k]; sh[k] = R[k]*dt + sh[k]; }
+                    if (c.Fail > 1) dp_return(c, i22, sh, sg, sr, dt, tau);
+                    for (int k = 0; k < 6; ++k) sg[k] = (sh[k]+sb[k])*(1.0/2.0);
+                    if (first) for (int k = 0; k < 6; ++k) eh[k] = sr[k]*(-dt/2.0) + ep[k];
+                    double ebb[6];
+                    for (int k = 0; k < 6; ++k) { ebb[k] = eh[k]; eh[k] = sr[k]*dt + eh[k]; }
+                    for (int k = 0; k < 6; ++k) ep[k] = (eh[k]+ebb[k])*(1.0/2.0);
+                } else {
+                    // Mat3MVerlet, Particle.cpp:320-482 (sh = Sigmab)
+                    double st[6];
+                    for (int k = 0; k < 6; ++k) st[k] = sg[k];
+                    if (ct == 30) for (int k = 0; k < 6; ++k) sg[k] = R[k]*dt + sg[k];
+                    else for (int k = 0; k < 6; ++k) sg[k] = R[k]*(2.0*dt) + sh[k];
+                    for (int k = 0; k < 6; ++k) sh[k] = st[k];
+                    if (c.Fail > 1) dp_return(c, i22, sg, st, sr, (ct == 30) ? dt : 2.0*dt, tau);
+                    for (int k = 0; k < 6; ++k) st[k] = ep[k];
+                    if (ct == 30) for (int k = 0; k < 6; ++k) ep[k] = sr[k]*dt + ep[k];
+                    else for (int k = 0; k < 6; ++k) ep[k] = sr[k]*(2.0*dt) + eh[k];
+                    for (int k = 0; k < 6; ++k) eh[k] = st[k];
+                }
+            }
+            for (int k = 0; k < 6; ++k) {
+                D.sig_b[6*i+k] = sg[k]; D.sigh_b[6*i+k] = sh[k];
+                D.tau_b[6*i+k] = tau[k]; D.tauh_b[6*i+k] = th[k];
+                D.eps_b[6*i+k] = ep[k]; D.epsh_b[6*i+k] = eh[k];
+            }
+        }
+        p.w = rho;
+        v.w = P_;
+        D.pos_b[i] = p;
+        D.vel_b[i] = v;
+        D.vh_b[i] = vh;
+    }
+    // CheckParticleLeave test against [BLPF, TRPR]
+    if ((p.x > P.trpr[0]) || (p.y > P.trpr[1]) || (p.z > P.trpr[2]) ||
+        (p.x < P.blpf[0]) || (p.y < P.blpf[1]) || (p.z < P.blpf[2])) {
+        atomicAdd(&ctl->leave, 1);
+        return;
+    }
+    // next step's cell key (ListGenerate after Move)
+    int kx, ky, kz;
+    int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
+    D.key[i] = key;
+    D.rank[i] = atomicAdd(&D.ccount[key], 1);
+}
+
+// Time update and stop conditions (Solve loop, Domain.cpp:1762-1793); one thread.
+__global__ void k_tail(Dev D, Params P)
+{
+    Ctl *c = D.ctl;
+    if (c->halt) return;
+    c->last_pairs = c->pairs_ff/2 + c->pairs_fw;
+    if (c->xmat) { c->halt = kHaltError; c->error = -2; return; }
+    if (!c->resume && c->time >= c->tout) { c->halt = kHaltOutput; c->mid_step = 1; return; }
+    double deltatmin = bitsd(c->dtmin_bits);
+    double dt = adaptive_dt(c->deltat, c->deltatint, deltatmin);
+    c->deltatmin = deltatmin;
+    if (dt < (c->deltatint/1.0e5)) { c->halt = kHaltError; c->error = -4; c->deltat = dt; return; }
+    c->deltat = dt;
+    c->time += dt;
+    c->steps_done += 1;
+    c->resume = 0;
+    c->mid_step = 0;
+    if (P.scheme == 0) {
+        // Move_MVerlet's per-particle counters, uniform over free particles (Particle.cpp:191-195, 272-273)
+        if (c->first_step) c->ct = 30;
+        c->ct = (c->ct == 30) ? 0 : c->ct + 1;
+        c->shep_counter = (c->shep_counter == P.shepard_step) ? 0 : c->shep_counter + 1;
+    }
+    c->first_step = 0;
+    c->dtmin_bits = dbits(c->deltatint);
+    c->pairs_ff = 0;
+    c->pairs_fw = 0;
+    if (c->leave) { c->halt = kHaltLeave; return; }
+    if (c->steps_done >= c->max_steps) { c->halt = kHaltSteps; return; }
+    if (!(c->time < c->tf)) { c->halt = kHaltTend; return; }
+}
+
+// Head of a run call: stop at once if the loop condition fails (Domain.cpp:1762).
+__global__ void k_head(Dev D, Params P)
+{
+    Ctl *c = D.ctl;
+    c->halt = kRun;
+    c->steps_done = 0;
+    c->leave = 0;
+    c->xmat = 0;
+    if (c->max_steps <= 0) { c->halt = kHaltSteps; return; }
+    if (c->mid_step) { c->resume = 1; return; }
+    if (!(c->time < c->tf)) { c->halt = kHaltTend; return; }
+}
+
+// ------------------------------------------------------------------------------------------
+// test hook: the neighbour pair set with the reference's orientation (MainNeighbourSearch
+// Domain.cpp:808-945 + the rij/h <= Cellfac test of CalcForce11/2233).  Pass 0 counts pairs
+// per particle, pass 1 writes them at offsets from an exclusive scan of the counts.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_pairs(Dev D, Params P, int pass, const long long *off,
+                                                  int *cnt_out, long long *out)
+{
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t fi = D.flags_b[i];
+    double4 pi = D.pos_b[i];
+    int key_i = D.key[i], orig_i = D.orig_b[i];
+    int kx, ky, kz;
+    decode_key(P, key_i, kx, ky, kz);
+    int nrows = (P.dim == 3) ? 9 : 3;
+    long long w = (pass == 1) ? off[i] : 0;
+    int c = 0;
+    for (int r = 0; r < nrows; ++r) {
+        int beg, end;
+        row_range(D, P, kx, ky, kz, r, beg, end);
+        for (int j = beg; j < end; ++j) {
+            if (j == i) continue;
+            uint32_t fj = D.flags_b[j];
+            if (!(flag_free(fi) || flag_free(fj))) continue;
+            if (flag_mat(fi) != flag_mat(fj)) continue;
+            int kj = D.key[j], oj = D.orig_b[j];
+            bool first = (key_i < kj) || (key_i == kj && orig_i > oj);
+            if (!first) continue;
+            double4 pj = D.pos_b[j];
+            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+            double h = pair_h(cp, P, flag_cls(fi), flag_cls(fj));
+            double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+            if ((rij/h) <= P.cellfac) {
+                if (pass == 1) { out[2*w] = orig_i; out[2*w+1] = oj; ++w; }
+                ++c;
+            }
+        }
+    }
+    if (pass == 0) cnt_out[i] = c;
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side launchers (one per kernel so the host can bracket each with timing events)
+// ------------------------------------------------------------------------------------------
+static inline int nblk(int n, int b) { return (n + b - 1)/b; }
+
+int scan_blocks(const Params &P) { return nblk(P.ncells, kScanTile); }
+
+void launch_head(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_head, dim3(1), dim3(1), 0, s, D, P); }
+
+void launch_keys(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_keys, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
+
+void launch_scan(const Dev &D, const Params &P, hipStream_t s)
+{
+    int nb = scan_blocks(P);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, D, P);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, D, P, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, D, P);
+}
+
+void launch_scatter(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_scatter, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
+
+void launch_cellsort(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_cellsort, dim3(nblk(P.ncells, kBlock)), dim3(kBlock), 0, s, D, P); }
+
+void launch_reorder(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_reorder, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
+
+void launch_wall(const Dev &D, const Params &P, hipStream_t s)
+{
+    if (P.nfixed > 0)
+        hipLaunchKernelGGL(k_wall, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+}
+
+void launch_force(const Dev &D, const Params &P, hipStream_t s)
+{
+    if (P.nfree <= 0) return;
+    if (P.any_solid)
+        hipLaunchKernelGGL(k_force<true>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+    else
+        hipLaunchKernelGGL(k_force<false>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+}
+
+void launch_integrate(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_integrate, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
+
+void launch_tail(const Dev &D, const Params &P, hipStream_t s)
+{ hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, D, P); }
+
+void launch_pairs(const Dev &D, const Params &P, hipStream_t s, int pass, const long long *off,
+                  int *cnt, long long *out)
+{ hipLaunchKernelGGL(k_pairs, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P, pass, off, cnt, out); }
+
+}  // namespace psph

@@ -1,0 +1,144 @@
+"""ctypes binding of the C ABI (include/persiansph_amd.h) of the GPU path.
+
+The shared library is built in-tree (persiansph_amd/lib/libpersiansph_amd.so).  There is no
+CPU fallback: if the library is missing or no GPU is visible, construction raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import ERRORS, PARTICLE_DTYPE, Config, Grid, Status, grid_dict
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpersiansph_amd.so")
+
+EXPORTS = [
+    "psph_create", "psph_upload", "psph_run", "psph_download", "psph_neighbour_pairs",
+    "psph_get_grid", "psph_set_timing", "psph_kernel_stats", "psph_stream",
+    "psph_traffic_model", "psph_destroy", "psph_last_error", "psph_version",
+]
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"persiansph_amd HIP library not built: {LIB_PATH} missing "
+                               "(run __graft_entry__.build() or make -C persiansph_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        p, d, i64, sz = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_size_t
+        L.psph_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(p)]
+        L.psph_upload.argtypes = [p, p, sz, d, d]
+        L.psph_run.argtypes = [p, i64, d, d, ctypes.POINTER(Status)]
+        L.psph_download.argtypes = [p, p, sz, ctypes.POINTER(sz)]
+        L.psph_neighbour_pairs.argtypes = [p, p, sz, ctypes.POINTER(sz)]
+        L.psph_get_grid.argtypes = [p, ctypes.POINTER(Grid)]
+        L.psph_set_timing.argtypes = [p, ctypes.c_int]
+        L.psph_kernel_stats.argtypes = [p, ctypes.c_char_p, sz, p, p, sz, ctypes.POINTER(sz)]
+        L.psph_stream.argtypes = [p, ctypes.POINTER(p)]
+        L.psph_traffic_model.argtypes = [p, ctypes.POINTER(d), ctypes.POINTER(d)]
+        L.psph_destroy.argtypes = [p]
+        L.psph_last_error.argtypes = [ctypes.c_char_p, sz]
+        for f in EXPORTS:
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib().psph_last_error(buf, 1024)
+        raise SolverError(rc, buf.value.decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Solver:
+    """GPU Solve loop behind the C ABI; same interface as the CPU oracle's wrapper."""
+
+    def __init__(self, cfg, particles, dt, time=0.0):
+        h = ctypes.c_void_p()
+        _check(lib().psph_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.cfg = cfg
+        p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        try:
+            _check(lib().psph_upload(self._h, _ptr(p), len(p), time, dt))
+        except Exception:
+            self.close()
+            raise
+
+    def run(self, max_steps, tf=float("inf"), tout=float("inf")):
+        st = Status()
+        _check(lib().psph_run(self._h, max_steps, tf, tout, ctypes.byref(st)))
+        return st
+
+    def particles(self):
+        n = ctypes.c_size_t()
+        _check(lib().psph_download(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
+        _check(lib().psph_download(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        return out
+
+    def pairs(self):
+        n = ctypes.c_size_t()
+        _check(lib().psph_neighbour_pairs(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 2), dtype=np.int64)
+        _check(lib().psph_neighbour_pairs(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def grid(self):
+        g = Grid()
+        _check(lib().psph_get_grid(self._h, ctypes.byref(g)))
+        return g
+
+    def set_timing(self, on):
+        _check(lib().psph_set_timing(self._h, 1 if on else 0))
+
+    def kernel_stats(self):
+        names = ctypes.create_string_buffer(512)
+        cap = 32
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        nk = ctypes.c_size_t()
+        _check(lib().psph_kernel_stats(self._h, names, 512, _ptr(ms), _ptr(cnt), cap, ctypes.byref(nk)))
+        keys = names.value.decode().split(",")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+    def traffic_model(self):
+        t = ctypes.c_double()
+        f = ctypes.c_double()
+        _check(lib().psph_traffic_model(self._h, ctypes.byref(t), ctypes.byref(f)))
+        return t.value, f.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check(lib().psph_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    def grid_dict(self):
+        return grid_dict(self.grid())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().psph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
