@@ -332,10 +332,14 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
             double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
             if (!((rij/h) <= P.cellfac)) continue;      // outside the reference's pair set
             // orientation of the reference's pair (P1 = smaller key / larger index); xij is
-            // x_P1 - x_P2 (negation is exact, rij unchanged)
+            // x_P1 - x_P2.  Multiplying by -1 is an exact negation (rij unchanged).  The sign is
+            // formed without short-circuit branches: hipcc (ROCm 7.2, -O3) dropped the same-cell
+            // negation when this was written as `if (!(a || (b && c))) x = -x;`.
             int kj = D.key[j];
-            bool wall_first = (key_i < kj) || (key_i == kj && orig_i > D.orig_b[j]);
-            if (!wall_first) { x0 = -x0; x1 = -x1; x2 = -x2; }
+            int oj = D.orig_b[j];
+            int wall_first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
+            double sgn = wall_first ? 1.0 : -1.0;
+            x0 *= sgn; x1 *= sgn; x2 *= sgn;
             double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h);
             SK += K;
             if (mat < 3) {
@@ -984,11 +988,11 @@ __global__ void k_tail(Dev D, Params P)
     Ctl *c = D.ctl;
     if (c->halt) return;
     c->last_pairs = c->pairs_ff/2 + c->pairs_fw;
+    double deltatmin = bitsd(c->dtmin_bits);
+    c->deltatmin = deltatmin;
     if (c->xmat) { c->halt = kHaltError; c->error = -2; return; }
     if (!c->resume && c->time >= c->tout) { c->halt = kHaltOutput; c->mid_step = 1; return; }
-    double deltatmin = bitsd(c->dtmin_bits);
     double dt = adaptive_dt(c->deltat, c->deltatint, deltatmin);
-    c->deltatmin = deltatmin;
     if (dt < (c->deltatint/1.0e5)) { c->halt = kHaltError; c->error = -4; c->deltat = dt; return; }
     c->deltat = dt;
     c->time += dt;
@@ -1053,7 +1057,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev D, Params P, int pass, con
             if (!(flag_free(fi) || flag_free(fj))) continue;
             if (flag_mat(fi) != flag_mat(fj)) continue;
             int kj = D.key[j], oj = D.orig_b[j];
-            bool first = (key_i < kj) || (key_i == kj && orig_i > oj);
+            int first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
             if (!first) continue;
             double4 pj = D.pos_b[j];
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;

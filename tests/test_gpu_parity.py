@@ -72,7 +72,8 @@ def test_first_force_pass(case):
     assert sg.stop_reason == so.stop_reason == 2
     assert sg.mid_step == so.mid_step == 1
     assert sg.pairs == so.pairs
-    assert sg.deltatmin == so.deltatmin
+    # min over particles of sqrt_h_a*sqrt(h/|a|): |a| carries the reordered-sum rounding
+    assert abs(sg.deltatmin - so.deltatmin) <= 1e-12*so.deltatmin
     G, O = g.particles(), o.particles()
     sc = o.scales(len(O))
     fr = O["IsFree"] == 1
