@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -129,7 +130,7 @@ static hipEvent_t get_event(psph_handle *h)
         return e;
     }
     hipEvent_t e;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
 }
 
@@ -138,9 +139,9 @@ static void timed(psph_handle *h, int id, F &&f)
 {
     if (!h->timing) { f(); return; }
     hipEvent_t a = get_event(h), b = get_event(h);
-    hipEventRecord(a, h->stream);
+    (void)hipEventRecord(a, h->stream);
     f();
-    hipEventRecord(b, h->stream);
+    (void)hipEventRecord(b, h->stream);
     h->pending.push_back({id, {a, b}});
 }
 
@@ -303,6 +304,12 @@ static int allocate(psph_handle *h, size_t n)
         }
     }
     AL(a.aux, sizeof(double4)*N);
+    if (P.fast11) {
+        AL(a.g0, sizeof(double4)*N);
+        AL(a.g1, sizeof(double4)*N);
+        AL(a.g2, sizeof(double4)*N);
+        AL(a.g3, sizeof(double4)*N);
+    }
     AL(a.wal, sizeof(double4)*N);
     AL(a.acc, sizeof(double4)*N);
     AL(a.xsp, sizeof(double4)*N);
@@ -702,6 +709,36 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         if (c.IsFree && c.Shepard && P.scheme == 0) P.any_shepard = 1;
     }
     P.any_xsph = (h->cfg.xsph != 0.0 && P.scheme == 0) ? 1 : 0;
+    // fast fluid path: cubic spline, Morris, squared-density form, no artificial stress, no
+    // strain-rate viscosity models, no XSPH/Shepard terms, uniform h and free-particle mu
+    {
+        bool mu_uniform = true;
+        double mu0 = have_free ? ref.Mu : 0.0;
+        for (size_t i = 0; i < n; ++i)
+            if (h->host[i].IsFree && h->host[i].Mu != mu0) { mu_uniform = false; break; }
+        bool ok = mat0 == 1 && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
+                  !P.any_ti && !P.any_sr && !P.any_xsph && !P.any_shepard && P.uniform_h &&
+                  mu_uniform && P.nclasses <= 8 && have_free && !std::getenv("PSPH_GENERAL_KERNELS");
+        P.fast11 = ok ? 1 : 0;
+        if (ok) {
+            double hh = P.h_uniform;
+            P.mu_ff = 2.0*mu0*mu0/(mu0+mu0);
+            P.mu_ff_wall = mu0;
+            if (P.dim == 2) { P.kw = 10.0/(7.0*hh*hh*M_PI); P.kg = 10.0/(7.0*hh*hh*hh*M_PI); }
+            else { P.kw = 1.0/(hh*hh*hh*M_PI); P.kg = 1.0/(hh*hh*hh*hh*M_PI); }
+            P.kgh = P.kg/hh;
+            P.invh = 1.0/hh;
+            P.h2c = 0.01*hh*hh;
+            double sup = P.cellfac*hh;
+            P.sup2_lo = sup*sup*(1.0 - 1e-13);
+            P.sup2_hi = sup*sup*(1.0 + 1e-13);
+            for (int a = 0; a < P.nclasses; ++a)
+                for (int b = 0; b < P.nclasses; ++b) {
+                    P.pair_alpha[a*8+b] = (h->classes[a].Alpha + h->classes[b].Alpha)/2.0;
+                    P.pair_beta[a*8+b] = (h->classes[a].Beta + h->classes[b].Beta)/2.0;
+                }
+        }
+    }
     rc = allocate(h, n);
     if (rc) return rc;
     HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));

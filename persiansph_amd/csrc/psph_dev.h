@@ -75,6 +75,13 @@ struct Params {
     int shepard_step;                  // uniform over free particles
     int any_ti, any_sr, any_xsph, any_shepard, any_solid, uniform_h, any_les_t0;
     double h_uniform;
+    // fast fluid path (cubic spline, Morris, squared-density form, uniform h and viscosity)
+    int fast11;
+    double mu_ff;                      // 2 mu mu / (mu + mu) of two free particles
+    double mu_ff_wall;                 // the free particles' mu (a wall pair uses the free side's mu)
+    double kw, kg, kgh, invh, h2c;     // kernel constants: C of W, C of (1/r)dW/dr, C/h, 1/h, 0.01 h^2
+    double sup2_lo, sup2_hi;           // (Cellfac h)^2 band around the exact rij/h <= Cellfac test
+    double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
@@ -102,6 +109,11 @@ struct Dev {
     double *sig_b, *sigh_b, *tau_b, *tauh_b, *eps_b, *epsh_b;
     // transient (current order)
     double4 *aux;      // Mass, Mu, sound speed, P/d^2  (fixed: wall density-based values)
+    // gather views for the fast fluid path, written per step by reorder (free) and wall (fixed):
+    // g0 = x, y, z, d   g1 = v, P/d^2   g2 = v_visc, c   g3 = m (free) or FPMassC (fixed), 1/d, free, class
+    // (d, P, c: the density, pressure and sound speed the pair formulas use for that particle;
+    //  v_visc = v for free particles and 2 v - NSv for no-slip walls)
+    double4 *g0, *g1, *g2, *g3;
     double4 *wal;      // fixed: NSv x,y,z, wall density d_w
     double4 *acc;      // a x,y,z, dDensity
     double4 *xsp;      // VXSPH x,y,z, ZWab

@@ -206,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
         double4 p = D.pos[s], v = D.vel[s], vh = D.vh[s];
         double2 mm = D.mmu[s];
         f = D.flags[s];
+        if (P.dim == 2) v.z = 0.0;          // StartAcceleration zeroes v(2) in 2-D (Domain.cpp:1011)
         D.pos_b[i] = p;
         D.vel_b[i] = v;
         D.vh_b[i] = vh;
@@ -219,6 +220,12 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
             double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
             double pr = v.w/(p.w*p.w);
             D.aux[i] = make_double4(mm.x, mm.y, C, pr);
+            if (P.fast11) {
+                D.g0[i] = p;
+                D.g1[i] = make_double4(v.x, v.y, v.z, pr);
+                D.g2[i] = make_double4(v.x, v.y, v.z, C);
+                D.g3[i] = make_double4(mm.x, 1.0/p.w, 1.0, (double)flag_cls(f));
+            }
         } else {
             D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
         }
@@ -240,12 +247,6 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
                 for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t[q];
             }
         }
-    }
-    if (act && P.dim == 2) {
-        // StartAcceleration zeroes v(2) in 2-D (Domain.cpp:1011)
-        double4 v2 = D.vel_b[i];
-        v2.z = 0.0;
-        D.vel_b[i] = v2;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl->reorders += 1;
     // target lists (order within a list does not change any result)
@@ -374,6 +375,13 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     double2 mm = D.mmu_b[i];
     D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
     D.wal[i] = make_double4(n0, n1, n2, dw);
+    if (P.fast11) {
+        D.g0[i] = make_double4(pi.x, pi.y, pi.z, dw);
+        D.g1[i] = make_double4(vi.x, vi.y, vi.z, Pw/(dw*dw));
+        if (C.NoSlip) D.g2[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, Cw);
+        else D.g2[i] = make_double4(vi.x, vi.y, vi.z, Cw);
+        D.g3[i] = make_double4(C.FPMassC, 1.0/dw, 0.0, (double)ci);
+    }
     if (P.any_solid) {
         if (mat > 1) for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = sg[q];
         else for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = 0.0;
@@ -686,6 +694,147 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
         if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
         if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
         if (anyx) atomicOr(&D.ctl->xmat, 1);
+    }
+}
+
+
+// ------------------------------------------------------------------------------------------
+// fast fluid force pass: CalcForce11 for cubic spline, Morris viscosity, squared-density
+// momentum form, uniform h and viscosity (the dam-break configuration).  Neighbour data comes
+// from the gather views; divisions become refined reciprocals (each within ~2 ulp), the pair
+// test is decided on rij^2 except in a 1e-13 band around the support radius, where the exact
+// rij/h <= Cellfac test of the reference runs.
+// ------------------------------------------------------------------------------------------
+__device__ inline double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ inline double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double hx = 0.5*x;
+    double e = fma(-hx*y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-hx*y, y, 0.5);
+    return fma(y, e, y);
+}
+
+struct Acc11 { double a0, a1, a2, dd; };
+
+template <int DIM>
+__device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
+                                   double4 gi0, double4 gi1, double4 gi2, double4 gi3, int ci, int j,
+                                   Acc11 &A, unsigned &nff, unsigned &nfw)
+{
+    double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j], a3 = D.g3[j];
+    double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
+    double r2 = x0*x0 + x1*x1 + x2*x2;
+    double invr = rsq_nr(r2);
+    double rij = r2 > 0.0 ? r2*invr : 0.0;
+    double q = rij*P.invh;
+    double K, GK;
+    if (q < 1.0) {
+        K = P.kw*(1.0-1.5*q*q+0.75*q*q*q);
+        GK = P.kgh*(-3.0+2.25*q);
+    } else if (q < 2.0) {
+        double t = 2.0 - q;
+        K = P.kw*(0.25*t*t*t);
+        GK = P.kg*invr*(-0.75*t*t);
+    } else {
+        K = 0.0; GK = 0.0;
+    }
+    (void)K;
+    bool freej = a3.z != 0.0;
+    nff += freej ? 1u : 0u;
+    nfw += freej ? 0u : 1u;
+    int cj = (int)a3.w;
+    double alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
+    double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
+    double dvx = u0*x0 + u1*x1 + u2*x2;
+    double di = gi0.w, dj = a0.w;
+    double PIij = 0.0;
+    if ((alpha != 0.0 || beta != 0.0) && dvx < 0.0) {
+        double MU = P.h_uniform*dvx*rcp_nr(r2 + P.h2c);
+        PIij = (-alpha*0.5*(gi2.w + a2.w)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
+    }
+    double mi = gi3.x;
+    double mj = freej ? a3.x : a3.x*mi;
+    double mu = freej ? P.mu_ff : P.mu_ff_wall;
+    double vs = 2.0*mu*gi3.y*a3.y*GK;
+    double s = -(gi1.w + a1.w + PIij)*GK;
+    double b0 = gi1.x - a2.x, b1 = gi1.y - a2.y, b2 = gi1.z - a2.z;
+    double t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
+    if (DIM == 2) t2 = 0.0;
+    double temp1 = (u0*x0 + u1*x1 + u2*x2)*GK;
+    A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
+    A.dd += mj*(di*a3.y)*temp1;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ double pa[64], pb[64];
+    __shared__ int lst[kCap*kForceBlock];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    int t = blockIdx.x*kForceBlock + tid;
+    bool act = t < P.nfree;
+    int i = act ? D.free_list[t] : 0;
+    double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
+    int ci = (int)gi3.w;
+    int kx, ky, kz;
+    decode_key(P, D.key[i], kx, ky, kz);
+    Acc11 A;
+    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
+    unsigned nff = 0, nfw = 0;
+    int cnt = 0;
+    const int nrows = (DIM == 3) ? 9 : 3;
+    const double lo = P.sup2_lo, hi = P.sup2_hi;
+    for (int r = 0; r < nrows; ++r) {
+        int beg = 0, end = 0;
+        if (act) row_range(D, P, kx, ky, kz, r, beg, end);
+        for (int j = beg; j < end; ++j) {
+            if (j != i) {
+                double4 pj = D.g0[j];
+                double x0 = gi0.x - pj.x, x1 = gi0.y - pj.y, x2 = gi0.z - pj.z;
+                double r2 = x0*x0 + x1*x1 + x2*x2;
+                bool in = r2 <= lo;
+                if (!in && r2 <= hi) in = (sqrt(r2)/P.h_uniform) <= P.cellfac;
+                if (in) {
+                    lst[cnt*kForceBlock + tid] = j;
+                    ++cnt;
+                }
+            }
+            if (wave_any(cnt == kCap)) {
+                for (int e = 0; e < cnt; ++e)
+                    pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], A, nff, nfw);
+                cnt = 0;
+            }
+        }
+    }
+    for (int e = 0; e < cnt; ++e)
+        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], A, nff, nfw);
+    if (act) {
+        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+    }
+    unsigned long long sff = nff, sfw = nfw;
+    for (int o = 32; o > 0; o >>= 1) {
+        sff += __shfl_down(sff, o, 64);
+        sfw += __shfl_down(sfw, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
+        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
     }
 }
 
@@ -1111,6 +1260,13 @@ void launch_wall(const Dev &D, const Params &P, hipStream_t s)
 void launch_force(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfree <= 0) return;
+    if (P.fast11) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_force11_fast<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        else
+            hipLaunchKernelGGL(k_force11_fast<2>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        return;
+    }
     if (P.any_solid)
         hipLaunchKernelGGL(k_force<true>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
     else
