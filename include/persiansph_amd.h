@@ -90,7 +90,8 @@ typedef struct psph_particle {
     int32_t Material, PresEq, VisM, Fail;
     int32_t ID, ct, FirstStep, InOut;
     int32_t ShepardCounter, ShepardStep;
-    int32_t reserved[2];
+    int32_t Index;      /* out: position of this particle in the array given to psph_upload */
+    int32_t reserved;
 } psph_particle;
 
 /* Scalars of the Solve loop after a psph_run call. */

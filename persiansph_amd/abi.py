@@ -32,7 +32,7 @@ _PARTICLE_FIELDS = [
     ("Material", _I, 1), ("PresEq", _I, 1), ("VisM", _I, 1), ("Fail", _I, 1),
     ("ID", _I, 1), ("ct", _I, 1), ("FirstStep", _I, 1), ("InOut", _I, 1),
     ("ShepardCounter", _I, 1), ("ShepardStep", _I, 1),
-    ("reserved", _I, 2),
+    ("Index", _I, 1), ("reserved", _I, 1),
 ]
 
 PARTICLE_DTYPE = np.dtype([(n, t, (k,)) if k > 1 else (n, t) for n, t, k in _PARTICLE_FIELDS],

@@ -1,0 +1,510 @@
+// SPH::Domain drop-in for the MI355X path of PersianSPH.
+//
+// The user-facing C++ API of the reference (Source/Domain.h:44-166, Source/Particle.h:29-149,
+// enums Domain.h:38-40) with Domain::Solve running on the GPU through the C ABI of
+// persiansph_amd.h.  Model set-up (AddBoxLength, AddBoxNo, AddSingleParticle, DelParticles,
+// per-particle field writes) is host code as in the reference; Solve uploads the particles,
+// runs the time loop on the device, downloads at every output time and at the end.
+//
+// Vec3_t / Mat3_t are the parts of blitz TinyVector<double,3> / TinyMatrix<double,3,3> that
+// user code touches (element access, comma initialisation, arithmetic).  Errors surface as
+// `throw new Fatal(...)`, caught by MECHSYS_CATCH in main, as in the reference.
+//
+// Not available on the GPU path (Solve throws Fatal): periodic and in/out-flow boundaries,
+// FSI and multi-material pairs, and user hooks GeneralBefore/GeneralAfter other than the
+// default no-op.  XDMF output is not written; UserOutput is not called.
+#ifndef PERSIANSPH_AMD_DOMAIN_H
+#define PERSIANSPH_AMD_DOMAIN_H
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "persiansph_amd.h"
+
+// ----------------------------------------------------------------------------- Fatal
+class Fatal
+{
+public:
+    Fatal(char const *fmt, ...)
+    {
+        char buf[2048];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        msg_ = buf;
+    }
+    char const *Msg() const { return msg_.c_str(); }
+private:
+    std::string msg_;
+};
+
+#define MECHSYS_CATCH                                                                    \
+    catch (Fatal *e) { std::printf("Fatal: %s\n", e->Msg()); delete e; return 1; }      \
+    catch (char const *m) { std::printf("Fatal: %s\n", m); return 1; }                  \
+    catch (std::exception &e) { std::printf("Fatal: %s\n", e.what()); return 1; }
+
+// ----------------------------------------------------------------------------- Vec3_t / Mat3_t
+template <int N>
+class TinyArray
+{
+public:
+    // comma initialisation: v = 1.0, 2.0, 3.0;
+    class Init
+    {
+    public:
+        Init(double *d, int k) : d_(d), k_(k) {}
+        Init operator,(double x) { if (k_ < N) d_[k_] = x; return Init(d_, k_ + 1); }
+    private:
+        double *d_;
+        int k_;
+    };
+    double d[N];
+    double *data() { return d; }
+    const double *data() const { return d; }
+};
+
+class Vec3_t : public TinyArray<3>
+{
+public:
+    Vec3_t() { d[0] = d[1] = d[2] = 0.0; }
+    Vec3_t(double x, double y, double z) { d[0] = x; d[1] = y; d[2] = z; }
+    Init operator=(double x) { d[0] = x; d[1] = x; d[2] = x; return Init(d, 1); }
+    double &operator()(int i) { return d[i]; }
+    double operator()(int i) const { return d[i]; }
+    double &operator[](int i) { return d[i]; }
+    double operator[](int i) const { return d[i]; }
+    Vec3_t &operator+=(const Vec3_t &o) { for (int k = 0; k < 3; ++k) d[k] += o.d[k]; return *this; }
+    Vec3_t &operator-=(const Vec3_t &o) { for (int k = 0; k < 3; ++k) d[k] -= o.d[k]; return *this; }
+    Vec3_t &operator*=(double s) { for (int k = 0; k < 3; ++k) d[k] *= s; return *this; }
+    Vec3_t &operator/=(double s) { for (int k = 0; k < 3; ++k) d[k] /= s; return *this; }
+};
+
+inline Vec3_t operator+(const Vec3_t &a, const Vec3_t &b) { return Vec3_t(a(0)+b(0), a(1)+b(1), a(2)+b(2)); }
+inline Vec3_t operator-(const Vec3_t &a, const Vec3_t &b) { return Vec3_t(a(0)-b(0), a(1)-b(1), a(2)-b(2)); }
+inline Vec3_t operator-(const Vec3_t &a) { return Vec3_t(-a(0), -a(1), -a(2)); }
+inline Vec3_t operator*(double s, const Vec3_t &a) { return Vec3_t(s*a(0), s*a(1), s*a(2)); }
+inline Vec3_t operator*(const Vec3_t &a, double s) { return Vec3_t(a(0)*s, a(1)*s, a(2)*s); }
+inline Vec3_t operator/(const Vec3_t &a, double s) { return Vec3_t(a(0)/s, a(1)/s, a(2)/s); }
+inline double dot(const Vec3_t &a, const Vec3_t &b) { return a(0)*b(0) + a(1)*b(1) + a(2)*b(2); }
+inline double norm(const Vec3_t &a) { return std::sqrt(dot(a, a)); }
+inline std::ostream &operator<<(std::ostream &os, const Vec3_t &a)
+{ return os << "[ " << a(0) << " " << a(1) << " " << a(2) << " ]"; }
+
+class Mat3_t : public TinyArray<9>
+{
+public:
+    Mat3_t() { for (int k = 0; k < 9; ++k) d[k] = 0.0; }
+    Init operator=(double x) { for (int k = 0; k < 9; ++k) d[k] = x; return Init(d, 1); }
+    double &operator()(int i, int j) { return d[3*i+j]; }
+    double operator()(int i, int j) const { return d[3*i+j]; }
+};
+
+inline void set_to_zero(Vec3_t &v) { v = 0.0, 0.0, 0.0; }
+inline void set_to_zero(Mat3_t &m) { for (int k = 0; k < 9; ++k) m.d[k] = 0.0; }
+
+// ----------------------------------------------------------------------------- Array
+template <class T>
+class Array : public std::vector<T>
+{
+public:
+    size_t Size() const { return this->size(); }
+    void Push(const T &x) { this->push_back(x); }
+    void Resize(size_t n) { this->resize(n); }
+    void Clear() { this->clear(); }
+    bool Has(const T &x) const
+    {
+        for (const T &y : *this) if (y == x) return true;
+        return false;
+    }
+    // order-preserving removal of the given indices (External/array.h:726-738)
+    void DelItems(const Array<int> &idx)
+    {
+        std::vector<char> del(this->size(), 0);
+        for (int i : idx) if (i >= 0 && (size_t)i < del.size()) del[i] = 1;
+        size_t k = 0;
+        for (size_t i = 0; i < this->size(); ++i) if (!del[i]) (*this)[k++] = (*this)[i];
+        this->resize(k);
+    }
+};
+
+enum Kernels_Type { Qubic_Spline = 0, Quintic = 1, Quintic_Spline = 2 };
+enum Viscosity_Eq_Type { Morris = 0, Shao = 1, Incompressible_Full = 2, Takeda = 3 };
+enum Gradient_Type { Squared_density = 0, Multiplied_density = 1 };
+
+namespace SPH {
+
+// ----------------------------------------------------------------------------- Boundary
+class Boundary
+{
+public:
+    double inDensity = 0.0, outDensity = 0.0, allDensity = 0.0;
+    Vec3_t inv, outv, allv;
+    bool Periodic[3] = {false, false, false};
+    int InOutFlow = 0;
+    double InFlowLoc1 = 0.0, InFlowLoc2 = 0.0, InFlowLoc3 = 0.0, OutFlowLoc = 0.0;
+    double cellfac = 3.9;
+    int inoutcounter = 0;
+    bool MassConservation = false;
+};
+
+// ----------------------------------------------------------------------------- Particle
+class Particle
+{
+public:
+    bool Shepard = false;
+    size_t ShepardCounter = 0, ShepardStep = 40;
+    double ZWab = 0.0, SumDen = 0.0;
+    bool IsFree = true;
+    size_t InOut = 0;
+    bool NoSlip = false;
+    int ID = 0, Material = 0;
+    Vec3_t x, vb, va, v, NSv, VXSPH, a;
+    size_t PresEq = 0;
+    double Cs = 0.0, P0 = 0.0, Pressure = 0.0;
+    double Density = 0.0, Densitya = 0.0, Densityb = 0.0, dDensity = 0.0, RefDensity = 0.0;
+    double FPMassC = 1.0, Mass = 0.0;
+    Mat3_t StrainRate, RotationRate;
+    double ShearRate = 0.0, SBar = 0.0;
+    Mat3_t ShearStress, ShearStressa, ShearStressb;
+    Mat3_t Sigma, Sigmaa, Sigmab;
+    Mat3_t Strain, Straina, Strainb;
+    Mat3_t TIR;
+    double TI = 0.0, TIn = 4.0, TIInitDist = 0.0;
+    double Alpha = 0.0, Beta = 0.0, Mu = 0.0, MuRef = 0.0, T0 = 0.0, m = 300.0;
+    size_t VisM = 0;
+    bool LES = false;
+    double CSmag = 0.17;
+    double G = 0.0, K = 0.0, Sigmay = 0.0;
+    size_t Fail = 0;
+    double c = 0.0, phi = 0.0, psi = 0.0;
+    double h = 0.0;
+    int ct = 0;
+    double SumKernel = 0.0;
+    bool FirstStep = true;
+
+    // Particle::Particle, Source/Particle.cpp:25-117
+    Particle(int Tag, Vec3_t const &x0, Vec3_t const &v0, double Mass0, double Density0, double h0, bool Fixed = false)
+    {
+        ID = Tag; x = x0; v = v0; Mass = Mass0; Density = Density0; RefDensity = Density0;
+        h = h0; IsFree = !Fixed;
+    }
+};
+
+// ----------------------------------------------------------------------------- Domain
+class Domain;
+inline void General(Domain &) {}
+
+class Domain
+{
+public:
+    typedef void (*PtDom)(Domain &dom);
+
+    Array<Particle *> Particles;
+    double R = 0.0;
+    double sqrt_h_a = 0.0025;
+    int Dimension = 2;
+    double MuMax = 0.0, CsMax = 0.0;
+    Vec3_t Gravity;
+    Vec3_t TRPR, BLPF, CellSize;
+    int CellNo[3] = {0, 0, 0};
+    double hmax = 0.0, rhomax = 0.0;
+    size_t SWIType = 0;
+    bool FSI = false;
+    double XSPH = 0.0;
+    double InitialDist = 0.0;
+    size_t Nproc = 1;
+    Boundary BC;
+    Vec3_t DomMax, DomMin;
+    PtDom GeneralBefore = &General;
+    PtDom GeneralAfter = &General;
+    size_t Scheme = 0;
+    int Precision = PSPH_FP64;           // GPU arithmetic tier (this build's addition)
+
+    Domain()
+    {
+        DomMax = -100000000000.0;
+        DomMin = 100000000000.0;
+    }
+    ~Domain() { for (Particle *p : Particles) delete p; }
+
+    void Kernel_Set(Kernels_Type const &KT) { KernelType = KT; Cellfac = (KT == 2) ? 3.0 : 2.0; }
+    void Viscosity_Eq_Set(Viscosity_Eq_Type const &VQ) { VisEq = VQ; }
+    void Gradient_Approach_Set(Gradient_Type const &GT) { GradientType = GT; }
+
+    void AddSingleParticle(int tag, Vec3_t const &x, double Mass, double Density, double h, bool Fixed)
+    {
+        Particles.Push(new Particle(tag, x, Vec3_t(0, 0, 0), Mass, Density, h, Fixed));
+    }
+
+    // AddBoxLength, Source/Domain.cpp:294-495 (random jitter with srand(100)/rand() as there)
+    void AddBoxLength(int tag, Vec3_t const &V, double Lx, double Ly, double Lz, double r, double Density,
+                      double h, int type, int rotation, bool random, bool Fixed)
+    {
+        if (!(type == 0 || type == 1)) throw new Fatal("Packing Type is out of range (0 hexagonal, 1 cubic)");
+        if (!(rotation == 0 || rotation == 90)) throw new Fatal("Packing Rotation Angle is out of range (0 or 90)");
+        size_t PrePS = Particles.Size();
+        double qin = 0.03;
+        std::srand(100);
+        auto jit = [&](double c) { return random ? c + qin*r*double(std::rand())/RAND_MAX : c; };
+        if (Dimension == 3) {
+            size_t k = 0;
+            double zp = V(2);
+            if (type == 0) {
+                while (zp <= (V(2)+Lz-r)) {
+                    size_t j = 0; double yp = V(1);
+                    while (yp <= (V(1)+Ly-r)) {
+                        size_t i = 0; double xp = V(0);
+                        while (xp <= (V(0)+Lx-r)) {
+                            double x, y, z;
+                            if ((k%2 != 0) && (j%2 != 0)) x = V(0) + (2*i+(j%2)+(k%2)-1)*r; else x = V(0) + (2*i+(j%2)+(k%2)+1)*r;
+                            y = V(1) + (std::sqrt(3.0)*(j+(1.0/3.0)*(k%2))+1)*r;
+                            z = V(2) + ((2*std::sqrt(6.0)/3)*k+1)*r;
+                            double xx = jit(x), yy = jit(y), zz = jit(z);
+                            Particles.Push(new Particle(tag, Vec3_t(xx, yy, zz), Vec3_t(0, 0, 0), 0.0, Density, h, Fixed));
+                            i++;
+                            if ((k%2 != 0) && (j%2 != 0)) xp = V(0) + (2*i+(j%2)+(k%2)-1)*r; else xp = V(0) + (2*i+(j%2)+(k%2)+1)*r;
+                        }
+                        j++;
+                        yp = V(1) + (std::sqrt(3.0)*(j+(1.0/3.0)*(k%2))+1)*r;
+                    }
+                    k++;
+                    zp = V(2) + ((2*std::sqrt(6.0)/3)*k+1)*r;
+                }
+            } else {
+                while (zp <= (V(2)+Lz-r)) {
+                    size_t j = 0; double yp = V(1);
+                    while (yp <= (V(1)+Ly-r)) {
+                        size_t i = 0; double xp = V(0);
+                        while (xp <= (V(0)+Lx-r)) {
+                            double x = V(0) + (2.0*i+1)*r, y = V(1) + (2.0*j+1)*r, z = V(2) + (2.0*k+1)*r;
+                            double xx = jit(x), yy = jit(y), zz = jit(z);
+                            Particles.Push(new Particle(tag, Vec3_t(xx, yy, zz), Vec3_t(0, 0, 0), 0.0, Density, h, Fixed));
+                            i++;
+                            xp = V(0) + (2*i+1)*r;
+                        }
+                        j++;
+                        yp = V(1) + (2.0*j+1)*r;
+                    }
+                    k++;
+                    zp = V(2) + (2.0*k+1)*r;
+                }
+            }
+            // particle mass in 3-D (Domain.cpp:401-418)
+            Vec3_t Max = V;
+            for (size_t i = PrePS; i < Particles.Size(); i++)
+                for (int c = 0; c < 3; ++c) if (Particles[i]->x(c) > Max(c)) Max(c) = Particles[i]->x(c);
+            for (int c = 0; c < 3; ++c) Max(c) += r;
+            Vec3_t temp = Max - V;
+            double Mass = temp(0)*temp(1)*temp(2)*Density/(Particles.Size()-PrePS);
+            for (size_t i = PrePS; i < Particles.Size(); i++) Particles[i]->Mass = Mass;
+        } else if (Dimension == 2) {
+            if (type == 0 && rotation == 0) {
+                size_t j = 0; double yp = V(1);
+                while (yp <= (V(1)+Ly-r)) {
+                    size_t i = 0; double xp = V(0);
+                    while (xp <= (V(0)+Lx-r)) {
+                        double x = V(0) + (2*i+(j%2)+1)*r, y = V(1) + (std::sqrt(3.0)*j+1)*r;
+                        double xx = jit(x), yy = jit(y);
+                        Particles.Push(new Particle(tag, Vec3_t(xx, yy, 0.0), Vec3_t(0, 0, 0), (std::sqrt(3.0)*r*r)*Density, Density, h, Fixed));
+                        i++;
+                        xp = V(0) + (2*i+(j%2)+1)*r;
+                    }
+                    j++;
+                    yp = V(1) + (std::sqrt(3.0)*j+1)*r;
+                }
+            } else if (type == 0) {
+                size_t i = 0; double xp = V(0);
+                while (xp <= (V(0)+Lx-r)) {
+                    size_t j = 0; double yp = V(1);
+                    while (yp <= (V(1)+Ly-r)) {
+                        double x = V(0) + (std::sqrt(3.0)*i+1)*r, y = V(1) + (2*j+(i%2)+1)*r;
+                        double xx = jit(x), yy = jit(y);
+                        Particles.Push(new Particle(tag, Vec3_t(xx, yy, 0.0), Vec3_t(0, 0, 0), (std::sqrt(3.0)*r*r)*Density, Density, h, Fixed));
+                        j++;
+                        yp = V(1) + (2*j+(i%2)+1)*r;
+                    }
+                    i++;
+                    xp = V(0) + (std::sqrt(3.0)*i+1)*r;
+                }
+            } else {
+                size_t j = 0; double yp = V(1);
+                while (yp <= (V(1)+Ly-r)) {
+                    size_t i = 0; double xp = V(0);
+                    while (xp <= (V(0)+Lx-r)) {
+                        double x = V(0) + (2*i+1)*r, y = V(1) + (2*j+1)*r;
+                        // random cubic boxes get the hexagonal mass, as the reference does (Domain.cpp:482)
+                        if (random)
+                            Particles.Push(new Particle(tag, Vec3_t(jit(x), jit(y), 0.0), Vec3_t(0, 0, 0), (std::sqrt(3.0)*r*r)*Density, Density, h, Fixed));
+                        else
+                            Particles.Push(new Particle(tag, Vec3_t(x, y, 0.0), Vec3_t(0, 0, 0), 2.0*r*2.0*r*Density, Density, h, Fixed));
+                        i++;
+                        xp = V(0) + (2*i+1)*r;
+                    }
+                    j++;
+                    yp = V(1) + (2*j+1)*r;
+                }
+            }
+        }
+        R = r;
+    }
+
+    // DelParticles, Domain.cpp:497-515
+    void DelParticles(int const &Tags)
+    {
+        Array<int> idxs;
+        for (size_t i = 0; i < Particles.Size(); ++i) if (Particles[i]->ID == Tags) idxs.Push((int)i);
+        if (idxs.Size() < 1) throw new Fatal("Domain::DelParticles: Could not find any particles to delete");
+        for (int i : idxs) delete Particles[i];
+        Particles.DelItems(idxs);
+        std::cout << "\nParticle(s) with Tag No. " << Tags << " has been deleted" << std::endl;
+    }
+
+    // Solve, Domain.cpp:1735-1797, with the time loop on the GPU
+    void Solve(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx)
+    {
+        (void)TheFileKey;
+        std::cout << "\n--------------Solving---------------------------------------------------------------" << std::endl;
+        if (GeneralBefore != &General || GeneralAfter != &General)
+            throw new Fatal("Domain::Solve: user hooks GeneralBefore/GeneralAfter are not available on the GPU path");
+        if (Particles.Size() == 0) throw new Fatal("Domain::Solve: no particles");
+        psph_config cfg;
+        std::memset(&cfg, 0, sizeof cfg);
+        cfg.dimension = Dimension; cfg.kernel_type = (int)KernelType; cfg.visc_eq = (int)VisEq;
+        cfg.gradient_type = (int)GradientType; cfg.scheme = (int)Scheme; cfg.precision = Precision;
+        for (int c = 0; c < 3; ++c) {
+            cfg.periodic[c] = BC.Periodic[c] ? 1 : 0;
+            cfg.gravity[c] = Gravity(c); cfg.dom_max[c] = DomMax(c); cfg.dom_min[c] = DomMin(c);
+            cfg.trpr[c] = TRPR(c); cfg.blpf[c] = BLPF(c);
+        }
+        cfg.in_out_flow = BC.InOutFlow; cfg.swi_type = (int)SWIType; cfg.fsi = FSI ? 1 : 0;
+        cfg.cellfac = Cellfac; cfg.xsph = XSPH; cfg.sqrt_h_a = sqrt_h_a; cfg.R = R;
+        cfg.mu_max = MuMax; cfg.cs_max = CsMax;
+        std::vector<psph_particle> buf(Particles.Size());
+        for (size_t i = 0; i < Particles.Size(); ++i) to_abi(*Particles[i], buf[i]);
+        up_.assign(Particles.begin(), Particles.end());
+        alive_.assign(up_.size(), 1);
+        psph_handle *h = nullptr;
+        check(psph_create(&cfg, &h));
+        struct Guard { psph_handle *h; ~Guard() { psph_destroy(h); } } guard{h};
+        check(psph_upload(h, buf.data(), buf.size(), Time, dt));
+        psph_grid g;
+        check(psph_get_grid(h, &g));
+        for (int c = 0; c < 3; ++c) { TRPR(c) = g.trpr[c]; BLPF(c) = g.blpf[c]; CellSize(c) = g.cell_size[c]; CellNo[c] = g.cell_no[c]; }
+        hmax = g.hmax; rhomax = g.rhomax; MuMax = g.mu_max; CsMax = g.cs_max;
+        deltat = deltatint = deltatmin = dt;
+        size_t idx_out = 1;
+        double tout = Time;
+        psph_status st;
+        for (;;) {
+            if (!(Time < tf && idx_out <= maxidx)) break;
+            check(psph_run(h, INT64_MAX, tf, tout, &st));
+            Time = st.time; deltat = st.deltat; deltatmin = st.deltatmin;
+            if (st.stop_reason != PSPH_STOP_OUTPUT) break;
+            download(h);
+            std::cout << "\nOutput No. " << idx_out << " at " << Time << " has been generated" << std::endl;
+            std::cout << "Current Time Step = " << deltat << std::endl;
+            idx_out++;
+            tout += dtOut;
+            if (idx_out > maxidx) {                      // the current step still completes
+                check(psph_run(h, 1, tf, tout, &st));
+                Time = st.time; deltat = st.deltat; deltatmin = st.deltatmin;
+                break;
+            }
+        }
+        download(h);
+        std::cout << "\n--------------Solving is finished---------------------------------------------------" << std::endl;
+    }
+
+    double SimTime() const { return Time; }
+
+private:
+    size_t VisEq = 0, KernelType = 0, GradientType = 0;
+    double Cellfac = 2.0;
+    double Time = 0.0, deltat = 0.0, deltatmin = 0.0, deltatint = 0.0;
+
+    static void check(int rc)
+    {
+        if (rc != 0) {
+            char msg[1024];
+            psph_last_error(msg, sizeof msg);
+            throw new Fatal("%s (code %d)", msg, rc);
+        }
+    }
+
+    static void cp9(const Mat3_t &m, double *o) { for (int k = 0; k < 9; ++k) o[k] = m.d[k]; }
+    static void cp9(const double *o, Mat3_t &m) { for (int k = 0; k < 9; ++k) m.d[k] = o[k]; }
+    static void cp3(const Vec3_t &v, double *o) { for (int k = 0; k < 3; ++k) o[k] = v(k); }
+    static void cp3(const double *o, Vec3_t &v) { for (int k = 0; k < 3; ++k) v(k) = o[k]; }
+
+#define PSPH_FIELDS_SCALAR(X) X(Density) X(Densitya) X(Densityb) X(dDensity) X(Pressure) X(Mass) \
+    X(ZWab) X(SumDen) X(SumKernel) X(Mu) X(ShearRate) X(SBar) X(h) X(Cs) X(P0) X(RefDensity)           \
+    X(FPMassC) X(Alpha) X(Beta) X(MuRef) X(T0) X(m) X(CSmag) X(TI) X(TIn) X(TIInitDist) X(G) X(K)     \
+    X(Sigmay) X(c) X(phi) X(psi)
+#define PSPH_FIELDS_VEC(X) X(x) X(v) X(va) X(vb) X(a) X(NSv) X(VXSPH)
+#define PSPH_FIELDS_MAT(X) X(StrainRate) X(RotationRate) X(ShearStress) X(ShearStressa) X(ShearStressb) \
+    X(Sigma) X(Sigmaa) X(Sigmab) X(Strain) X(Straina) X(Strainb) X(TIR)
+
+    static void to_abi(const Particle &p, psph_particle &q)
+    {
+        std::memset(&q, 0, sizeof q);
+#define S_(f) q.f = p.f;
+#define V_(f) cp3(p.f, q.f);
+#define M_(f) cp9(p.f, q.f);
+        PSPH_FIELDS_SCALAR(S_) PSPH_FIELDS_VEC(V_) PSPH_FIELDS_MAT(M_)
+#undef S_
+#undef V_
+#undef M_
+        q.IsFree = p.IsFree; q.NoSlip = p.NoSlip; q.Shepard = p.Shepard; q.LES = p.LES;
+        q.Material = p.Material; q.PresEq = (int32_t)p.PresEq; q.VisM = (int32_t)p.VisM; q.Fail = (int32_t)p.Fail;
+        q.ID = p.ID; q.ct = p.ct; q.FirstStep = p.FirstStep; q.InOut = (int32_t)p.InOut;
+        q.ShepardCounter = (int32_t)p.ShepardCounter; q.ShepardStep = (int32_t)p.ShepardStep;
+    }
+
+    static void from_abi(const psph_particle &q, Particle &p)
+    {
+#define S_(f) p.f = q.f;
+#define V_(f) cp3(q.f, p.f);
+#define M_(f) cp9(q.f, p.f);
+        PSPH_FIELDS_SCALAR(S_) PSPH_FIELDS_VEC(V_) PSPH_FIELDS_MAT(M_)
+#undef S_
+#undef V_
+#undef M_
+        p.ct = q.ct; p.FirstStep = q.FirstStep != 0; p.ShepardCounter = (size_t)q.ShepardCounter;
+    }
+
+    // Copies the device state back; particles deleted by the leave check are removed from
+    // Particles in order (Array::DelItems semantics).
+    void download(psph_handle *h)
+    {
+        size_t n = 0;
+        check(psph_download(h, nullptr, 0, &n));
+        std::vector<psph_particle> buf(n);
+        check(psph_download(h, buf.data(), n, &n));
+        std::vector<char> keep(up_.size(), 0);
+        Array<Particle *> next;
+        for (size_t k = 0; k < n; ++k) {
+            Particle *p = up_[buf[k].Index];        // Index: position at upload
+            keep[buf[k].Index] = 1;
+            from_abi(buf[k], *p);
+            next.Push(p);
+        }
+        for (size_t i = 0; i < up_.size(); ++i)
+            if (!keep[i] && alive_[i]) { delete up_[i]; alive_[i] = 0; }
+        Particles.swap(next);
+    }
+
+    std::vector<Particle *> up_;     // Particles as uploaded by the current Solve
+    std::vector<char> alive_;
+};
+
+}  // namespace SPH
+
+#endif

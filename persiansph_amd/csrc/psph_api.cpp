@@ -81,6 +81,7 @@ struct psph_handle {
     ClassParams *cls_dev = nullptr;
     std::vector<ClassParams> classes;
     std::vector<psph_particle> host;   // reference-order mirror (fields the GPU does not keep)
+    std::vector<int32_t> upload_index; // position of each mirrored particle in the uploaded array
     size_t cap = 0;
     bool uploaded = false;
     bool broken = false;
@@ -304,7 +305,12 @@ static int allocate(psph_handle *h, size_t n)
         }
     }
     AL(a.aux, sizeof(double4)*N);
-    if (P.fast11) {
+    if (P.mixed32) {
+        AL(a.f0, sizeof(float4)*N);
+        AL(a.f1, sizeof(float4)*N);
+        AL(a.f2, sizeof(float4)*N);
+        AL(a.f3, sizeof(float4)*N);
+    } else if (P.fast11) {
         AL(a.g0, sizeof(double4)*N);
         AL(a.g1, sizeof(double4)*N);
         AL(a.g2, sizeof(double4)*N);
@@ -542,14 +548,18 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     if (rc) return rc;
     const double *T = h->grid.trpr, *B = h->grid.blpf;
     std::vector<psph_particle> keep;
+    std::vector<int32_t> keep_idx;
     keep.reserve(h->host.size());
     long long del = 0;
-    for (auto &q : h->host) {
+    for (size_t k = 0; k < h->host.size(); ++k) {
+        const psph_particle &q = h->host[k];
         if ((q.x[0] > T[0]) || (q.x[1] > T[1]) || (q.x[2] > T[2]) ||
             (q.x[0] < B[0]) || (q.x[1] < B[1]) || (q.x[2] < B[2])) { ++del; continue; }
         keep.push_back(q);
+        keep_idx.push_back(h->upload_index[k]);
     }
     h->host.swap(keep);
+    h->upload_index.swap(keep_idx);
     Params &P = h->P;
     P.n = (int)h->host.size();
     int nf = 0;
@@ -619,12 +629,12 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     if (!h || !p) return fail(PSPH_ERR_INVALID, "null argument");
     if (n == 0) return fail(PSPH_ERR_INVALID, "no particles");
     if (n > (size_t)0x7fffffff) return fail(PSPH_ERR_CAPACITY, "too many particles");
-    if (h->cfg.precision == PSPH_FP32_MIXED)
-        return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier is not built yet");
     free_all(h);
     h->uploaded = false;
     h->broken = false;
     h->host.assign(p, p + n);
+    h->upload_index.resize(n);
+    for (size_t i = 0; i < n; ++i) h->upload_index[i] = (int32_t)i;
     // InitialChecks, Domain.cpp:1662-1719: EOS pressure of fluid and solid particles
     for (auto &q : h->host)
         if (q.Material < 3) q.Pressure = host_eos(q.PresEq, q.Cs, q.P0, q.Density, q.RefDensity);
@@ -738,6 +748,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                     P.pair_beta[a*8+b] = (h->classes[a].Beta + h->classes[b].Beta)/2.0;
                 }
         }
+    }
+    if (P.precision == PSPH_FP32_MIXED) {
+        if (!P.fast11)
+            return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier covers fluid runs with the cubic spline, Morris "
+                        "viscosity, the squared-density form, uniform h and mu, no artificial stress, "
+                        "no XSPH/Shepard terms");
+        P.mixed32 = 1;
     }
     rc = allocate(h, n);
     if (rc) return rc;
@@ -857,6 +874,7 @@ int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n)
     int rc = read_state(h);
     if (rc) return rc;
     std::memcpy(p, h->host.data(), sizeof(psph_particle)*h->host.size());
+    for (size_t i = 0; i < h->host.size(); ++i) p[i].Index = h->upload_index[i];
     return 0;
 }
 

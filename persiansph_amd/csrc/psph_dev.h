@@ -82,6 +82,8 @@ struct Params {
     double kw, kg, kgh, invh, h2c;     // kernel constants: C of W, C of (1/r)dW/dr, C/h, 1/h, 0.01 h^2
     double sup2_lo, sup2_hi;           // (Cellfac h)^2 band around the exact rij/h <= Cellfac test
     double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
+    // FP32-mixed tier (fast fluid configuration only)
+    int mixed32;
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
@@ -114,6 +116,9 @@ struct Dev {
     // (d, P, c: the density, pressure and sound speed the pair formulas use for that particle;
     //  v_visc = v for free particles and 2 v - NSv for no-slip walls)
     double4 *g0, *g1, *g2, *g3;
+    // FP32-mixed gather views: f0 = position relative to the particle's cell origin, d;
+    // f1 = v, P/d^2; f2 = v_visc, c; f3 = m or FPMassC, 1/d, free, class
+    float4 *f0, *f1, *f2, *f3;
     double4 *wal;      // fixed: NSv x,y,z, wall density d_w
     double4 *acc;      // a x,y,z, dDensity
     double4 *xsp;      // VXSPH x,y,z, ZWab

@@ -187,6 +187,24 @@ __global__ void __launch_bounds__(kBlock) k_cellsort(Dev D, Params P)
     }
 }
 
+__device__ inline void decode_key(const Params &P, int key, int &kx, int &ky, int &kz)
+{
+    kx = key % P.nc[0];
+    int t = key / P.nc[0];
+    ky = t % P.nc[1];
+    kz = t / P.nc[1];
+}
+
+// Origin (lower corner) of a cell of the reference grid: BLPF + k CellSize (FP64).
+__device__ inline void cell_origin(const Params &P, int key, double &o0, double &o1, double &o2)
+{
+    int kx, ky, kz;
+    decode_key(P, key, kx, ky, kz);
+    o0 = P.blpf[0] + kx*P.csize[0];
+    o1 = P.blpf[1] + ky*P.csize[1];
+    o2 = (P.dim == 3) ? P.blpf[2] + kz*P.csize[2] : 0.0;
+}
+
 // ------------------------------------------------------------------------------------------
 // reorder: gather persistent columns into the new order; build target lists; per-particle
 // derived values used by the pair gathers; TIR of free soil/solid (StartAcceleration
@@ -220,7 +238,14 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
             double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
             double pr = v.w/(p.w*p.w);
             D.aux[i] = make_double4(mm.x, mm.y, C, pr);
-            if (P.fast11) {
+            if (P.mixed32) {
+                double o0, o1, o2;
+                cell_origin(P, D.key_b[i], o0, o1, o2);
+                D.f0[i] = make_float4((float)(p.x - o0), (float)(p.y - o1), (float)(p.z - o2), (float)p.w);
+                D.f1[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)pr);
+                D.f2[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)C);
+                D.f3[i] = make_float4((float)mm.x, (float)(1.0/p.w), 1.0f, (float)flag_cls(f));
+            } else if (P.fast11) {
                 D.g0[i] = p;
                 D.g1[i] = make_double4(v.x, v.y, v.z, pr);
                 D.g2[i] = make_double4(v.x, v.y, v.z, C);
@@ -283,13 +308,6 @@ __device__ inline void row_range(const Dev &D, const Params &P, int kx, int ky, 
     end = D.cstart[base + x1 + 1];
 }
 
-__device__ inline void decode_key(const Params &P, int key, int &kx, int &ky, int &kz)
-{
-    kx = key % P.nc[0];
-    int t = key / P.nc[0];
-    ky = t % P.nc[1];
-    kz = t / P.nc[1];
-}
 
 __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, int cj)
 {
@@ -375,7 +393,15 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     double2 mm = D.mmu_b[i];
     D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
     D.wal[i] = make_double4(n0, n1, n2, dw);
-    if (P.fast11) {
+    if (P.mixed32) {
+        double o0, o1, o2;
+        cell_origin(P, key_i, o0, o1, o2);
+        D.f0[i] = make_float4((float)(pi.x - o0), (float)(pi.y - o1), (float)(pi.z - o2), (float)dw);
+        D.f1[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)(Pw/(dw*dw)));
+        if (C.NoSlip) D.f2[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), (float)Cw);
+        else D.f2[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)Cw);
+        D.f3[i] = make_float4((float)C.FPMassC, (float)(1.0/dw), 0.0f, (float)ci);
+    } else if (P.fast11) {
         D.g0[i] = make_double4(pi.x, pi.y, pi.z, dw);
         D.g1[i] = make_double4(vi.x, vi.y, vi.z, Pw/(dw*dw));
         if (C.NoSlip) D.g2[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, Cw);
@@ -838,6 +864,150 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
     }
 }
 
+
+// ------------------------------------------------------------------------------------------
+// FP32-mixed fluid force pass: the fast-path formulas in FP32 on cell-relative coordinates
+// (x_i - x_j = (r_i - r_j) - dk CellSize, r measured from each particle's cell origin), FP32
+// accumulation of the momentum sum and a Kahan-compensated FP32 continuity sum; results are
+// widened to FP64 for the FP64 integration.
+// ------------------------------------------------------------------------------------------
+struct Acc32 { float a0, a1, a2, dd, ddc; };
+
+template <int DIM>
+__device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
+                                  float4 gi0, float4 gi1, float4 gi2, float4 gi3, int ci, int j,
+                                  float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
+                                  float kw, float kg, float kgh, float invh, float h, float h2c,
+                                  float mu_ff, float mu_w)
+{
+    float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j], a3 = D.f3[j];
+    float x0 = (gi0.x - a0.x) - o0, x1 = (gi0.y - a0.y) - o1, x2 = (gi0.z - a0.z) - o2;
+    float r2 = x0*x0 + x1*x1 + x2*x2;
+    float invr = __builtin_amdgcn_rsqf(r2);
+    float rij = r2 > 0.0f ? r2*invr : 0.0f;
+    float q = rij*invh;
+    float GK;
+    if (q < 1.0f) {
+        GK = kgh*(-3.0f+2.25f*q);
+    } else if (q < 2.0f) {
+        float t = 2.0f - q;
+        GK = kg*invr*(-0.75f*t*t);
+    } else {
+        GK = 0.0f;
+    }
+    (void)kw;
+    bool freej = a3.z != 0.0f;
+    nff += freej ? 1u : 0u;
+    nfw += freej ? 0u : 1u;
+    int cj = (int)a3.w;
+    float alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
+    float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
+    float dvx = u0*x0 + u1*x1 + u2*x2;
+    float di = gi0.w, dj = a0.w;
+    float PIij = 0.0f;
+    if ((alpha != 0.0f || beta != 0.0f) && dvx < 0.0f) {
+        float MU = h*dvx*__builtin_amdgcn_rcpf(r2 + h2c);
+        PIij = (-alpha*0.5f*(gi2.w + a2.w)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
+    }
+    float mi = gi3.x;
+    float mj = freej ? a3.x : a3.x*mi;
+    float mu = freej ? mu_ff : mu_w;
+    float vs = 2.0f*mu*gi3.y*a3.y*GK;
+    float s = -(gi1.w + a1.w + PIij)*GK;
+    float b0 = gi1.x - a2.x, b1 = gi1.y - a2.y, b2 = gi1.z - a2.z;
+    float t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
+    if (DIM == 2) t2 = 0.0f;
+    A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
+    // Kahan-compensated continuity sum
+    float c = mj*(di*a3.y)*(dvx*GK);
+    float y = c - A.ddc;
+    float tt = A.dd + y;
+    A.ddc = (tt - A.dd) - y;
+    A.dd = tt;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_f32(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ float pa[64], pb[64];
+    __shared__ int lst[kCap*kForceBlock];
+    __shared__ unsigned char lcell[kCap*kForceBlock];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    int t = blockIdx.x*kForceBlock + tid;
+    bool act = t < P.nfree;
+    int i = act ? D.free_list[t] : 0;
+    float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
+    int ci = (int)gi3.w;
+    int kx, ky, kz;
+    decode_key(P, D.key[i], kx, ky, kz);
+    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
+    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const float sup2 = (float)(P.cellfac*P.h_uniform*P.cellfac*P.h_uniform);
+    Acc32 A;
+    A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
+    unsigned nff = 0, nfw = 0;
+    int cnt = 0;
+    const int ncell = (DIM == 3) ? 27 : 9;
+    for (int c = 0; c < ncell; ++c) {
+        int dxk = (c % 3) - 1, dyk = ((c / 3) % 3) - 1, dzk = (DIM == 3) ? (c / 9) - 1 : 0;
+        int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+        int beg = 0, end = 0;
+        if (act && xx >= 0 && xx < P.nc[0] && yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
+            int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
+            beg = D.cstart[cell];
+            end = D.cstart[cell + 1];
+        }
+        float o0 = dxk*csx, o1 = dyk*csy, o2 = dzk*csz;
+        for (int j = beg; j < end; ++j) {
+            if (j != i) {
+                float4 pj = D.f0[j];
+                float x0 = (gi0.x - pj.x) - o0, x1 = (gi0.y - pj.y) - o1, x2 = (gi0.z - pj.z) - o2;
+                float r2 = x0*x0 + x1*x1 + x2*x2;
+                if (r2 <= sup2) {
+                    lst[cnt*kForceBlock + tid] = j;
+                    lcell[cnt*kForceBlock + tid] = (unsigned char)c;
+                    ++cnt;
+                }
+            }
+            if (wave_any(cnt == kCap)) {
+                for (int e = 0; e < cnt; ++e) {
+                    int ce = lcell[e*kForceBlock + tid];
+                    float p0 = ((ce % 3) - 1)*csx, p1 = (((ce / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((ce / 9) - 1)*csz : 0.0f;
+                    pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], p0, p1, p2,
+                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+                }
+                cnt = 0;
+            }
+        }
+    }
+    for (int e = 0; e < cnt; ++e) {
+        int ce = lcell[e*kForceBlock + tid];
+        float p0 = ((ce % 3) - 1)*csx, p1 = (((ce / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((ce / 9) - 1)*csz : 0.0f;
+        pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], p0, p1, p2,
+                        A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+    }
+    if (act) {
+        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
+        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
+        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+    }
+    unsigned long long sff = nff, sfw = nfw;
+    for (int o = 32; o > 0; o >>= 1) {
+        sff += __shfl_down(sff, o, 64);
+        sfw += __shfl_down(sfw, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
+        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // integrate: AdaptiveTimeStep (Domain.cpp:133-152), Move (Particle.cpp:119-274, 561-596),
 // Mat1/Mat2/Mat3 (Particle.cpp:146-186, 276-482, 598-795), CheckParticleLeave test
@@ -1260,6 +1430,13 @@ void launch_wall(const Dev &D, const Params &P, hipStream_t s)
 void launch_force(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfree <= 0) return;
+    if (P.mixed32) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_force11_f32<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        else
+            hipLaunchKernelGGL(k_force11_f32<2>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        return;
+    }
     if (P.fast11) {
         if (P.dim == 3)
             hipLaunchKernelGGL(k_force11_fast<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
