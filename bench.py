@@ -65,7 +65,19 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events in the timed region")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe*1024*1024//4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4*n, "copy_probe_bytes_written": 4*n, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)

@@ -329,6 +329,8 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.src, sizeof(int32_t)*N);
     AL(a.cstart, sizeof(int32_t)*(P.ncells + 1));
     AL(a.ccount, sizeof(int32_t)*(P.ncells + 1));
+    AL(a.cfcnt, sizeof(int32_t)*(P.ncells + 1));
+    AL(a.cfree, sizeof(int32_t)*(P.ncells + 1));
     AL(a.bsum, sizeof(int32_t)*(scan_blocks(P) + 1));
     AL(a.free_list, sizeof(int32_t)*N);
     AL(a.fixed_list, sizeof(int32_t)*N);
@@ -571,6 +573,7 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     rc = write_state(h, h->host.data(), h->host.size(), 0);
     if (rc) return rc;
     HIPCHECK(hipMemsetAsync(h->D[0].ccount, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
+    HIPCHECK(hipMemsetAsync(h->D[0].cfcnt, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
     h->ctl_host->leave = 0;
     h->ctl_host->halt = kRun;
     h->trans_ok = false;

@@ -133,6 +133,8 @@ struct Dev {
     int32_t *src;      // new position -> old position
     int32_t *cstart;   // cell start table (ncells + 1)
     int32_t *ccount;   // per-cell counts (zeroed by the scan)
+    int32_t *cfcnt;    // per-cell free-particle counts being accumulated for the next step
+    int32_t *cfree;    // per-cell free-particle counts of the current step
     int32_t *bsum;     // scan block sums
     int32_t *free_list, *fixed_list;
 };

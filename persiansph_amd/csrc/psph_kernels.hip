@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
     int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
     D.key[i] = key;
     D.rank[i] = atomicAdd(&D.ccount[key], 1);
+    if (flag_free(D.flags[i])) atomicAdd(&D.cfcnt[key], 1);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         int c = base + k;
-        if (c < P.ncells) { D.cstart[c] = ex; D.ccount[c] = 0; }
+        if (c < P.ncells) { D.cstart[c] = ex; D.ccount[c] = 0; D.cfree[c] = D.cfcnt[c]; D.cfcnt[c] = 0; }
         ex += v[k];
     }
 }
@@ -309,6 +310,36 @@ __device__ inline void row_range(const Dev &D, const Params &P, int kx, int ky, 
 }
 
 
+// Neighbour row r of the target's cell restricted to the cells that intersect the support
+// sphere around the target at (px, py, pz): cells whose box lies farther than the support
+// radius cannot hold a pair.  The kept cells of a row are contiguous (the middle cell is the
+// nearest in x).  sup2m is the squared support radius with a small safety margin.
+__device__ inline void row_range_culled(const Dev &D, const Params &P, int kx, int ky, int kz, int r,
+                                        double px, double py, double pz, double sup2m,
+                                        int &beg, int &end)
+{
+    int dy = (r % 3) - 1, dz = (P.dim == 3) ? (r / 3) - 1 : 0;
+    int yy = ky + dy, zz = kz + dz;
+    beg = end = 0;
+    if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) return;
+    double oy = P.blpf[1] + ky*P.csize[1], oz = P.blpf[2] + kz*P.csize[2];
+    double ey = (dy < 0) ? py - oy : (dy > 0) ? oy + P.csize[1] - py : 0.0;
+    double ez = (dz < 0) ? pz - oz : (dz > 0) ? oz + P.csize[2] - pz : 0.0;
+    ey = ey > 0.0 ? ey : 0.0;
+    ez = ez > 0.0 ? ez : 0.0;
+    double dyz = ey*ey + ez*ez;
+    if (dyz > sup2m) return;
+    double ox = P.blpf[0] + kx*P.csize[0];
+    double el = px - ox, er = ox + P.csize[0] - px;
+    el = el > 0.0 ? el : 0.0;
+    er = er > 0.0 ? er : 0.0;
+    int x0 = (kx > 0 && dyz + el*el <= sup2m) ? kx - 1 : kx;
+    int x1 = (kx + 1 < P.nc[0] && dyz + er*er <= sup2m) ? kx + 1 : kx;
+    int base = P.nc[0]*(yy + P.nc[1]*zz);
+    beg = D.cstart[base + x0];
+    end = D.cstart[base + x1 + 1];
+}
+
 __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, int cj)
 {
     return P.uniform_h ? P.h_uniform : (cp[ci].h + cp[cj].h)/2;
@@ -338,10 +369,22 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     double SK = 0.0, Pw = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
     double sg[6] = {0, 0, 0, 0, 0, 0};
     const double *G = P.gravity;
-    int nrows = (P.dim == 3) ? 9 : 3;
-    for (int r = 0; r < nrows; ++r) {
-        int beg, end;
-        row_range(D, P, kx, ky, kz, r, beg, end);
+    const double sup = P.cellfac*P.hmax;
+    const double sup2m = sup*sup*(1.0 + 1e-9);
+    const double ox = P.blpf[0] + kx*P.csize[0], oy = P.blpf[1] + ky*P.csize[1], oz = P.blpf[2] + kz*P.csize[2];
+    const int ncell = (P.dim == 3) ? 27 : 9;
+    for (int cc = 0; cc < ncell; ++cc) {
+        int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (P.dim == 3) ? (cc / 9) - 1 : 0;
+        int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+        if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+        int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
+        if (D.cfree[cell] == 0) continue;                  // no free particle to gather from
+        double ex = (dxk < 0) ? pi.x - ox : (dxk > 0) ? ox + P.csize[0] - pi.x : 0.0;
+        double ey = (dyk < 0) ? pi.y - oy : (dyk > 0) ? oy + P.csize[1] - pi.y : 0.0;
+        double ez = (dzk < 0) ? pi.z - oz : (dzk > 0) ? oz + P.csize[2] - pi.z : 0.0;
+        ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
+        if (ex*ex + ey*ey + ez*ez > sup2m) continue;       // the cell lies outside the support
+        int beg = D.cstart[cell], end = D.cstart[cell + 1];
         for (int j = beg; j < end; ++j) {
             uint32_t fj = D.flags_b[j];
             if (!flag_free(fj) || flag_mat(fj) != mat) continue;
@@ -823,12 +866,15 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
     int cnt = 0;
     const int nrows = (DIM == 3) ? 9 : 3;
     const double lo = P.sup2_lo, hi = P.sup2_hi;
+    const double sup2m = P.sup2_hi*(1.0 + 1e-9);
     for (int r = 0; r < nrows; ++r) {
         int beg = 0, end = 0;
-        if (act) row_range(D, P, kx, ky, kz, r, beg, end);
+        if (act) row_range_culled(D, P, kx, ky, kz, r, gi0.x, gi0.y, gi0.z, sup2m, beg, end);
+        double4 nx = D.g0[beg < end ? beg : i];
         for (int j = beg; j < end; ++j) {
+            double4 pj = nx;
+            if (j + 1 < end) nx = D.g0[j + 1];
             if (j != i) {
-                double4 pj = D.g0[j];
                 double x0 = gi0.x - pj.x, x1 = gi0.y - pj.y, x2 = gi0.z - pj.z;
                 double r2 = x0*x0 + x1*x1 + x2*x2;
                 bool in = r2 <= lo;
@@ -956,15 +1002,23 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32(Dev D, Params P)
         int dxk = (c % 3) - 1, dyk = ((c / 3) % 3) - 1, dzk = (DIM == 3) ? (c / 9) - 1 : 0;
         int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
         int beg = 0, end = 0;
-        if (act && xx >= 0 && xx < P.nc[0] && yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
+        // distance from the target (cell-relative position) to the neighbour cell's box
+        float ex = (dxk < 0) ? gi0.x : (dxk > 0) ? csx - gi0.x : 0.0f;
+        float ey = (dyk < 0) ? gi0.y : (dyk > 0) ? csy - gi0.y : 0.0f;
+        float ez = (dzk < 0) ? gi0.z : (dzk > 0) ? csz - gi0.z : 0.0f;
+        ex = ex > 0.0f ? ex : 0.0f; ey = ey > 0.0f ? ey : 0.0f; ez = ez > 0.0f ? ez : 0.0f;
+        bool near = ex*ex + ey*ey + ez*ez <= sup2*1.0001f;
+        if (act && near && xx >= 0 && xx < P.nc[0] && yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
             int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
             beg = D.cstart[cell];
             end = D.cstart[cell + 1];
         }
         float o0 = dxk*csx, o1 = dyk*csy, o2 = dzk*csz;
+        float4 nx = D.f0[beg < end ? beg : i];
         for (int j = beg; j < end; ++j) {
+            float4 pj = nx;
+            if (j + 1 < end) nx = D.f0[j + 1];
             if (j != i) {
-                float4 pj = D.f0[j];
                 float x0 = (gi0.x - pj.x) - o0, x1 = (gi0.y - pj.y) - o1, x2 = (gi0.z - pj.z) - o2;
                 float r2 = x0*x0 + x1*x1 + x2*x2;
                 if (r2 <= sup2) {
@@ -1299,6 +1353,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
     int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
     D.key[i] = key;
     D.rank[i] = atomicAdd(&D.ccount[key], 1);
+    if (flag_free(f)) atomicAdd(&D.cfcnt[key], 1);
 }
 
 // Time update and stop conditions (Solve loop, Domain.cpp:1762-1793); one thread.
