@@ -225,7 +225,10 @@ def dam_break(dim, dx, tank, column, h_factor=1.3, rho0=1000.0, g=9.81, mu=1.0e-
             d[:, 2] = 0.0
         p["x"][fl] = p["x"][fl] + d
     yb = p["x"][fl, 1]
-    rho = rho0*np.power(1.0 + 7.0*g*(H - yb)/(cs*cs), 1.0/7.0)
+    # scalar libm pow, as the reference drivers' std::pow (numpy's vectorised power may differ
+    # in the last bit)
+    base = 1.0 + 7.0*g*(H - yb)/(cs*cs)
+    rho = rho0*np.array([math.pow(b, 1.0/7.0) for b in base.tolist()])
     p["Density"][fl] = rho
     p["Densityb"][fl] = rho
     cfg = default_config(dimension=dim, kernel_type=kernel_type, visc_eq=0, gradient_type=0,
@@ -247,17 +250,29 @@ def dam_break_3d(dx=0.01, scheme=1, jitter=0.01, seed=12345, tank=(3.2, 1.0, 1.0
     return dam_break(3, dx, tank, column, scheme=scheme, jitter=jitter, seed=seed, cs=cs, **kw)
 
 
-def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor=(1.0, 0.6), rho=2650.0,
+def _lattice(dim, V, counts, r, density, h):
+    """Cubic lattice box, x fastest (AddBoxLength order); 2-D boxes have z = 0 and mass 4 r^2 rho."""
+    if dim == 3:
+        return _fast_cubic_box(3, V, counts, r, density, h)
+    nx, ny = counts[0], counts[1]
+    p = _fast_cubic_box(3, (V[0], V[1], 0.0), (nx, ny, 1), r, density, h)
+    p["x"][:, 2] = 0.0
+    p["Mass"] = 2.0*r*2.0*r*density
+    return p
+
+
+def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor_half=0.5, rho=2650.0,
                     phi_deg=30.0, cohesion=0.0, psi_deg=0.0, E=5.0e6, nu=0.3, h_factor=1.5,
                     cs=None, ti=0.5, tin=2.55, alpha=0.1, beta=0.1, scheme=1, layers=3,
-                    g=9.81, precision=0, fail=3):
-    """Granular column collapse with the Drucker-Prager soil model (Material 3).
+                    g=9.81, precision=0, fail=3, geostatic=True):
+    """Granular column collapse with the Drucker-Prager soil model (Material 3, Fail 3).
 
-    The column (size = (Lx, Ly up, Lz)) stands in the corner x in [0, Lx], z in [0, Lz]
-    on a rigid floor of `layers` lattice layers covering x in [0, floor[0]] and
-    z in [-(floor[1]-Lz)/2 ... ] (centred on the column).  Parameters follow
-    Old_Tests/Dam_Scouring.cpp:187-202 (Fail 3, TI 0.5, TIn 2.55, TIInitDist dx) and
-    Old_Tests/ReposeAngle.cpp:93-99 (alpha = beta = 0.1).
+    The column (size = (Lx, Ly up, Lz)) stands on a rigid floor of `layers` lattice layers
+    extending floor_half beyond the column centre in x (and z in 3-D).  Parameters follow
+    Old_Tests/Dam_Scouring.cpp:187-202 (Fail 3, TI 0.5, TIn 2.55, TIInitDist = dx) and
+    Old_Tests/ReposeAngle.cpp:93-99 (alpha = beta = 0.1); K = E/(3(1-2nu)), G = E/(2(1+nu)).
+    The sound speed for artificial viscosity defaults to sqrt(K/rho) and dt = 0.2 h/Cs.
+    The initial stress is geostatic: sigma_yy = -rho g (Ly - y), lateral K0 = nu/(1-nu).
     """
     h = h_factor*dx
     r = dx/2.0
@@ -266,18 +281,15 @@ def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor=(1.0, 0.6), rho
     if cs is None:
         cs = math.sqrt(K/rho)
     Lx, Ly, Lz = size
+    nx, ny, nz = int(round(Lx/dx)), int(round(Ly/dx)), int(round(Lz/dx))
+    soil = _lattice(dim, (0.0, 0.0, 0.0), (nx, ny, nz), r, rho, h)
+    nf = int(round(2*floor_half/dx))
+    fx0 = Lx/2.0 - nf*dx/2.0
     if dim == 3:
-        nx, ny, nz = (int(round(Lx/dx)), int(round(Ly/dx)), int(round(Lz/dx)))
-        soil = _fast_cubic_box(3, (0.0, 0.0, 0.0), (nx, ny, nz), r, rho, h)
-        fx = int(round(floor[0]/dx))
-        fz = int(round(floor[1]/dx))
-        z0 = Lz/2.0 - floor[1]/2.0
-        fl = _fast_cubic_box(3, (-(layers*dx), -layers*dx, z0), (fx + layers, layers, fz), r, rho, h)
+        fz0 = Lz/2.0 - nf*dx/2.0
+        fl = _lattice(3, (fx0, -layers*dx, fz0), (nf, layers, nf), r, rho, h)
     else:
-        nx, ny = (int(round(Lx/dx)), int(round(Ly/dx)))
-        soil = add_box_length(2, (0.0, 0.0, 0.0), nx*dx, ny*dx, 0.0, r, rho, h, 1, 0, False)
-        fx = int(round(floor[0]/dx))
-        fl = add_box_length(2, (-layers*dx, -layers*dx, 0.0), (fx+layers)*dx, layers*dx, 0.0, r, rho, h, 1, 0, True)
+        fl = _lattice(2, (fx0, -layers*dx, 0.0), (nf, layers, 1), r, rho, h)
     fl["IsFree"] = 0
     fl["ID"] = 2
     p = np.concatenate([fl, soil])
@@ -294,18 +306,24 @@ def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor=(1.0, 0.6), rho
     p["TI"] = ti
     p["TIn"] = tin
     p["TIInitDist"] = dx
-    p["Shepard"] = 0
-    # geostatic initial stress in the soil: sigma_yy = -rho g (H - y), lateral K0 nu/(1-nu)
-    free = p["IsFree"] == 1
-    y = p["x"][:, 1]
-    syy = np.where(free, -rho*g*(Ly - y), 0.0)
-    k0 = nu/(1.0-nu)
-    sig = np.zeros((len(p), 9))
-    sig[:, 4] = syy
-    sig[:, 0] = k0*syy
-    sig[:, 8] = k0*syy if dim == 3 else 0.0
-    p["Sigma"] = sig
+    if geostatic:
+        free = p["IsFree"] == 1
+        y = p["x"][:, 1]
+        syy = np.where(free, -rho*g*(Ly - y), 0.0)
+        k0 = nu/(1.0-nu)
+        sig = np.zeros((len(p), 9))
+        sig[:, 4] = syy
+        sig[:, 0] = k0*syy
+        if dim == 3:
+            sig[:, 8] = k0*syy
+        p["Sigma"] = sig
     cfg = default_config(dimension=dim, kernel_type=0, visc_eq=0, gradient_type=0,
                          scheme=scheme, precision=precision, gravity=(0.0, -g, 0.0))
     dt = 0.2*h/cs
     return cfg, p, dt
+
+
+def runout(p):
+    """Runout of a granular column: the largest x reached by a free particle."""
+    free = p["IsFree"] == 1
+    return float(p["x"][free, 0].max())
