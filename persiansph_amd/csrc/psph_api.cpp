@@ -97,6 +97,8 @@ struct psph_handle {
     // scratch for the pair query
     int *pair_cnt = nullptr;
     long long *pair_off = nullptr;
+    // neighbour lists of the fast fluid path (capacity in Params::nl_cap)
+    int32_t *nl = nullptr;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -109,6 +111,45 @@ static int dev_alloc(psph_handle *h, void **p, size_t bytes)
     HIPCHECK(hipMemset(*p, 0, bytes));
     h->allocs.push_back(*p);
     return 0;
+}
+
+// (Re)allocates the neighbour lists for `cap` entries per free target.  The stream must be idle.
+static int nl_alloc(psph_handle *h, int cap)
+{
+    Params &P = h->P;
+    if (h->nl) {
+        HIPCHECK(hipFree(h->nl));
+        for (auto &q : h->allocs) if (q == (void *)h->nl) q = nullptr;
+        h->nl = nullptr;
+    }
+    size_t nf = P.nfree > 0 ? (size_t)P.nfree : 1;
+    int rc = dev_alloc(h, (void **)&h->nl, sizeof(int32_t)*(size_t)cap*nf);
+    if (rc) return rc;
+    P.nl_cap = cap;
+    for (auto &d : h->D) d.nl = h->nl;
+    return 0;
+}
+
+// First list capacity: 1.5x the neighbour count a uniform spread of the free particles over
+// their bounding box would give inside the support radius, plus a margin; a multiple of 4.
+// Targets that still overflow are handled exactly by the gathers, and psph_run grows the lists.
+static int nl_initial_cap(const psph_handle *h)
+{
+    const Params &P = h->P;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    size_t nf = 0;
+    for (const auto &q : h->host) {
+        if (!q.IsFree) continue;
+        ++nf;
+        for (int c = 0; c < 3; ++c) { lo[c] = std::min(lo[c], q.x[c]); hi[c] = std::max(hi[c], q.x[c]); }
+    }
+    double R = P.cellfac*P.h_uniform;
+    double vol = 1.0;
+    for (int c = 0; c < P.dim; ++c) vol *= std::max(hi[c] - lo[c], R);
+    double est = (double)nf/vol*((P.dim == 3) ? 4.0/3.0*M_PI*R*R*R : M_PI*R*R);
+    double capd = std::min(1.5*est + 8.0, 1024.0);
+    int cap = std::max(16, (int)std::ceil(capd));
+    return (cap + 3) & ~3;
 }
 
 static void free_all(psph_handle *h)
@@ -327,13 +368,20 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.rank, sizeof(int32_t)*N);
     AL(a.key_b, sizeof(int32_t)*N);
     AL(a.src, sizeof(int32_t)*N);
+    AL(a.sorig, sizeof(uint32_t)*N);
+    AL(a.perm, sizeof(int32_t)*N);
     AL(a.cstart, sizeof(int32_t)*(P.ncells + 1));
     AL(a.ccount, sizeof(int32_t)*(P.ncells + 1));
     AL(a.cfcnt, sizeof(int32_t)*(P.ncells + 1));
     AL(a.cfree, sizeof(int32_t)*(P.ncells + 1));
-    AL(a.bsum, sizeof(int32_t)*(scan_blocks(P) + 1));
+    AL(a.fstart, sizeof(int32_t)*(P.ncells + 1));
+    AL(a.bsum, sizeof(unsigned long long)*(scan_blocks(P) + 1));
+    AL(a.fpart, sizeof(unsigned long long)*3*(N/64 + 2));
     AL(a.free_list, sizeof(int32_t)*N);
     AL(a.fixed_list, sizeof(int32_t)*N);
+    AL(a.ncnt, sizeof(int32_t)*N);
+    AL(a.ovq, sizeof(int32_t)*N);
+    AL(a.ovs, sizeof(double4)*2*N);
     AL(h->pair_cnt, sizeof(int)*N);
     AL(h->pair_off, sizeof(long long)*N);
 #undef AL
@@ -761,6 +809,15 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     }
     rc = allocate(h, n);
     if (rc) return rc;
+    // neighbour lists for the fast fluid path (entries pack the index in 27 bits)
+    P.nl_cap = 0;
+    h->nl = nullptr;
+    if ((P.fast11 || P.mixed32) && n < ((size_t)1 << 27) && !std::getenv("PSPH_NO_NLIST")) {
+        int cap0 = nl_initial_cap(h);
+        if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);
+        rc = nl_alloc(h, cap0);
+        if (rc) return rc;
+    }
     HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
     h->parity_base = 0;
     std::memset(h->ctl_host, 0, sizeof(Ctl));
@@ -826,6 +883,15 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
         rc = sync_ctl(h);
         if (rc) return rc;
         if (C.reorders > 0) h->trans_ok = true;
+        if (C.nl_over > 0 && h->P.nl_cap > 0) {
+            // some targets had more neighbours than the lists hold (their pairs were found
+            // again by the gathers): grow the lists for the steps to come
+            rc = nl_alloc(h, ((h->P.nl_cap*3/2) + 3) & ~3);
+            if (rc) return rc;
+            C.nl_over = 0;
+            rc = push_ctl(h);
+            if (rc) return rc;
+        }
         if (C.halt == kRun) {
             if (queued >= max_steps) break;   // cannot happen: the tail halts at max_steps
             continue;

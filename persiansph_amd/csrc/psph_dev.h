@@ -57,7 +57,9 @@ struct Ctl {
     unsigned long long pairs_ff, pairs_fw;  // directed free-free and free-fixed in-support pairs
     unsigned long long last_pairs;        // unordered in-support pairs of the last force pass
     int halt, error, mid_step, resume;
-    int leave, xmat, nfree_seen, nfixed_seen;
+    int leave, xmat;
+    int nl_over;                        // targets whose neighbours overflowed the lists (cumulative)
+    int nl_q;                           // overflow queue length of the current force pass
     int ct, shep_counter, first_step, pad;
     long long reorders;                 // completed reorders (buffer parity)
 };
@@ -84,6 +86,8 @@ struct Params {
     double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;
+    // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4)
+    int nl_cap;
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
@@ -130,13 +134,25 @@ struct Dev {
     int32_t *key;      // cell key of each particle in the current order
     int32_t *rank;     // arrival rank within its cell (counting sort)
     int32_t *key_b;    // key in the new order
-    int32_t *src;      // new position -> old position
+    int32_t *src;      // new position -> old position, arrival order inside a cell
+    uint32_t *sorig;   // original index of src[p] (arrival order), bit 31 set for free particles
+    int32_t *perm;     // new position -> old position, original-index order inside a cell
     int32_t *cstart;   // cell start table (ncells + 1)
+    int32_t *fstart;   // free particles before each cell (ncells + 1): free-target list offsets
     int32_t *ccount;   // per-cell counts (zeroed by the scan)
     int32_t *cfcnt;    // per-cell free-particle counts being accumulated for the next step
     int32_t *cfree;    // per-cell free-particle counts of the current step
-    int32_t *bsum;     // scan block sums
+    unsigned long long *bsum;   // scan block sums (packed all/free counts)
+    unsigned long long *fpart;  // force pass per-block partials: pairs ff, pairs fw, dt bits
     int32_t *free_list, *fixed_list;
+    // neighbour lists of the free targets (fast fluid path): entry e of target t at
+    // nl[((e/4)*nfree + t)*4 + e%4], holding j | code << 27 (code = 0..26, the neighbour cell
+    // (dx+1) + 3(dy+1) + 9(dz+1) relative to the target's cell); ncnt[t] = in-support count,
+    // which may exceed the capacity (the gathers then rescan for the rest)
+    int32_t *nl, *ncnt;
+    // overflow queue of the force pass: target t and its partial sums (k_force11_nl_over ends it)
+    int32_t *ovq;
+    double4 *ovs;
 };
 
 }  // namespace psph

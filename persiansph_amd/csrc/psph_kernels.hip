@@ -32,11 +32,24 @@ constexpr int kBlock = 256;
 constexpr int kForceBlock = 128;
 constexpr int kCap = 24;            // per-lane neighbour list slots in LDS before a flush
 constexpr int kScanTile = 2048;     // cells per scan block (256 threads x 8)
+constexpr int kOverBlocks = 32;     // workgroups of the neighbour-list overflow passes
 
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 __device__ inline int wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Workgroups go to the 8 XCDs of the device round-robin (block b to XCD b mod 8), and each XCD
+// has its own L2.  The pair gathers remap the block index so that XCD x works on the x-th
+// contiguous share of the (cell-sorted) targets: neighbouring targets read the same neighbour
+// data, which then stays in one L2 instead of being fetched into all eight.
+__device__ inline int xcd_block()
+{
+    int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    int q = nb >> 3, r = nb & 7;
+    int x = b & 7, k = b >> 3;
+    return (x < r) ? x*(q + 1) + k : r*(q + 1) + (x - r)*q + k;
+}
 
 // ------------------------------------------------------------------------------------------
 // cell keys (ListGenerate, Domain.cpp:662-784): floor((x-BLPF)/CellSize), clamped to the grid
@@ -56,6 +69,33 @@ __device__ inline int cell_key(const Params &P, double x, double y, double z, in
     return ijk[0] + P.nc[0]*(ijk[1] + P.nc[1]*ijk[2]);
 }
 
+// Arrival rank of a particle inside its next-step cell, with one atomic per distinct cell per
+// wave instead of one per particle (a wave holds consecutive, mostly same-cell particles).
+// Lanes with key < 0 take no rank.  The free-particle count of the cell is raised alongside.
+__device__ inline int cell_rank(const Dev &D, int key, bool isfree)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long fmask = __ballot(isfree && key >= 0);
+    unsigned long long todo = __ballot(key >= 0);
+    int rank = 0;
+    while (todo) {
+        int leader = __ffsll((long long)todo) - 1;
+        int lk = __builtin_amdgcn_readlane(key, leader);
+        unsigned long long same = __ballot(key == lk);
+        int base = 0;
+        if (lane == leader) {
+            base = atomicAdd(&D.ccount[lk], __popcll(same));
+            int nf = __popcll(same & fmask);
+            if (nf) atomicAdd(&D.cfcnt[lk], nf);
+        }
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (key == lk) rank = base + __popcll(same & lt);
+        todo &= ~same;
+    }
+    return rank;
+}
+
 __global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -65,45 +105,53 @@ __global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
     int kx, ky, kz;
     int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
     D.key[i] = key;
-    D.rank[i] = atomicAdd(&D.ccount[key], 1);
-    if (flag_free(D.flags[i])) atomicAdd(&D.cfcnt[key], 1);
+    D.rank[i] = cell_rank(D, key, flag_free(D.flags[i]));
 }
 
 // ------------------------------------------------------------------------------------------
 // exclusive scan of the cell counts -> cell-start table; counts are cleared for the next step
 // ------------------------------------------------------------------------------------------
-__device__ inline int block_exclusive_scan(int v, int *sh, int &total)
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T *sh, T &total)
 {
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = v;
+    T x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
+        T y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     if (lane == 63) sh[w] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int s = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { int t = sh[k]; sh[k] = s; s += t; }
+        T s = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { T t = sh[k]; sh[k] = s; s += t; }
         sh[16] = s;
     }
     __syncthreads();
     total = sh[16];
-    int r = x - v + sh[w];
+    T r = x - v + sh[w];
     __syncthreads();
     return r;
+}
+
+// The scan carries two counts per cell packed in one 64-bit word, all particles in the high
+// half and free particles in the low half; neither half can carry into the other (each total
+// is below 2^31).
+__device__ inline unsigned long long pack_counts(int all, int free_)
+{
+    return ((unsigned long long)(unsigned)all << 32) | (unsigned)free_;
 }
 
 __global__ void __launch_bounds__(256) k_scan_reduce(Dev D, Params P)
 {
     if (D.ctl->halt) return;
-    __shared__ int sh[17];
+    __shared__ unsigned long long sh[17];
     int base = blockIdx.x*kScanTile + threadIdx.x*8;
-    int s = 0;
+    unsigned long long s = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) { int c = base + k; if (c < P.ncells) s += D.ccount[c]; }
-    int total;
+    for (int k = 0; k < 8; ++k) { int c = base + k; if (c < P.ncells) s += pack_counts(D.ccount[c], D.cfcnt[c]); }
+    unsigned long long total;
     block_exclusive_scan(s, sh, total);
     if (threadIdx.x == 0) D.bsum[blockIdx.x] = total;
 }
@@ -111,46 +159,51 @@ __global__ void __launch_bounds__(256) k_scan_reduce(Dev D, Params P)
 __global__ void __launch_bounds__(1024) k_scan_top(Dev D, Params P, int nblocks)
 {
     if (D.ctl->halt) return;
-    __shared__ int sh[17];
-    __shared__ int carry;
+    __shared__ unsigned long long sh[17];
+    __shared__ unsigned long long carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int b0 = 0; b0 < nblocks; b0 += 1024) {
         int b = b0 + threadIdx.x;
-        int v = b < nblocks ? D.bsum[b] : 0;
-        int total;
-        int ex = block_exclusive_scan(v, sh, total);
+        unsigned long long v = b < nblocks ? D.bsum[b] : 0;
+        unsigned long long total;
+        unsigned long long ex = block_exclusive_scan(v, sh, total);
         if (b < nblocks) D.bsum[b] = ex + carry;
         __syncthreads();
         if (threadIdx.x == 0) carry += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        D.cstart[P.ncells] = carry;
-        D.ctl->nfree_seen = 0;
-        D.ctl->nfixed_seen = 0;
+        D.cstart[P.ncells] = (int)(carry >> 32);
+        D.fstart[P.ncells] = (int)(carry & 0xffffffffull);
     }
 }
 
 __global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
 {
     if (D.ctl->halt) return;
-    __shared__ int sh[17];
+    __shared__ unsigned long long sh[17];
     int base = blockIdx.x*kScanTile + threadIdx.x*8;
-    int v[8];
-    int s = 0;
+    unsigned long long v[8];
+    int fc[8];
+    unsigned long long s = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         int c = base + k;
-        v[k] = (c < P.ncells) ? D.ccount[c] : 0;
+        fc[k] = (c < P.ncells) ? D.cfcnt[c] : 0;
+        v[k] = (c < P.ncells) ? pack_counts(D.ccount[c], fc[k]) : 0;
         s += v[k];
     }
-    int total;
-    int ex = block_exclusive_scan(s, sh, total) + D.bsum[blockIdx.x];
+    unsigned long long total;
+    unsigned long long ex = block_exclusive_scan(s, sh, total) + D.bsum[blockIdx.x];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         int c = base + k;
-        if (c < P.ncells) { D.cstart[c] = ex; D.ccount[c] = 0; D.cfree[c] = D.cfcnt[c]; D.cfcnt[c] = 0; }
+        if (c < P.ncells) {
+            D.cstart[c] = (int)(ex >> 32);
+            D.fstart[c] = (int)(ex & 0xffffffffull);
+            D.ccount[c] = 0; D.cfree[c] = fc[k]; D.cfcnt[c] = 0;
+        }
         ex += v[k];
     }
 }
@@ -164,28 +217,37 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Dev D, Params P)
     int k = D.key[i];
     int pos = D.cstart[k] + D.rank[i];
     D.src[pos] = i;
+    D.sorig[pos] = (unsigned)D.orig[i] | (flag_free(D.flags[i]) ? 0x80000000u : 0u);
     D.key_b[pos] = k;
 }
 
-// order inside each cell by original index (insertion sort; cells hold tens of particles)
+// Order inside each cell by original index: a particle's final slot is its cell's start plus
+// the number of particles of the cell with a smaller original index (original indices are
+// distinct; cells hold tens of particles, read contiguously).  The same count over the free
+// particles of the cell places the particle in the free-target list (free particles in sorted
+// order, cell c's share starting at fstart[c]) or in the fixed-target list (fixed particles in
+// sorted order, cell c's share starting at cstart[c] - fstart[c]).
 __global__ void __launch_bounds__(kBlock) k_cellsort(Dev D, Params P)
 {
     if (D.ctl->halt) return;
-    int c = blockIdx.x*kBlock + threadIdx.x;
-    if (c >= P.ncells) return;
-    int b = D.cstart[c], e = D.cstart[c+1];
-    for (int i = b + 1; i < e; ++i) {
-        int s = D.src[i];
-        int o = D.orig[s];
-        int j = i - 1;
-        while (j >= b) {
-            int sj = D.src[j];
-            if (D.orig[sj] <= o) break;
-            D.src[j+1] = sj;
-            --j;
-        }
-        D.src[j+1] = s;
+    int p = blockIdx.x*kBlock + threadIdx.x;
+    if (p >= P.n) return;
+    int k = D.key_b[p];
+    int b = D.cstart[k], e = D.cstart[k+1];
+    unsigned so = D.sorig[p];
+    unsigned o = so & 0x7fffffffu;
+    int r = 0, rf = 0;
+    for (int q = b; q < e; ++q) {
+        unsigned x = D.sorig[q];
+        int lt = ((x & 0x7fffffffu) < o) ? 1 : 0;
+        r += lt;
+        rf += lt & (int)(x >> 31);
     }
+    int dst = b + r;
+    D.perm[dst] = D.src[p];
+    int fs = D.fstart[k];
+    if (so >> 31) D.free_list[fs + rf] = dst;
+    else D.fixed_list[(b - fs) + (r - rf)] = dst;
 }
 
 __device__ inline void decode_key(const Params &P, int key, int &kx, int &ky, int &kz)
@@ -221,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
     bool act = i < P.n;
     uint32_t f = 0;
     if (act) {
-        int s = D.src[i];
+        int s = D.perm[i];
         double4 p = D.pos[s], v = D.vel[s], vh = D.vh[s];
         double2 mm = D.mmu[s];
         f = D.flags[s];
@@ -275,21 +337,6 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl->reorders += 1;
-    // target lists (order within a list does not change any result)
-    bool isfree = act && flag_free(f);
-    bool isfix = act && !flag_free(f);
-    unsigned long long bf = __ballot(isfree), bx = __ballot(isfix);
-    int lane = threadIdx.x & 63;
-    unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int basef = 0, basex = 0;
-    if (lane == 0) {
-        if (bf) basef = atomicAdd(&D.ctl->nfree_seen, __popcll(bf));
-        if (bx) basex = atomicAdd(&D.ctl->nfixed_seen, __popcll(bx));
-    }
-    basef = __shfl(basef, 0, 64);
-    basex = __shfl(basex, 0, 64);
-    if (isfree) D.free_list[basef + __popcll(bf & lt)] = i;
-    if (isfix) D.fixed_list[basex + __popcll(bx & lt)] = i;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -348,19 +395,21 @@ __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, 
 // ------------------------------------------------------------------------------------------
 // wall pass: PrimaryComputeAcceleration (Domain.cpp:1020-1059, 1163-1206) for fixed targets
 // ------------------------------------------------------------------------------------------
+// SOLID = false compiles the fluid-only smoothing (no Material 2/3 particles in the run).
+template <bool SOLID>
 __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
 {
     if (D.ctl->halt) return;
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
     __syncthreads();
-    int t = blockIdx.x*kBlock + threadIdx.x;
+    int t = xcd_block()*kBlock + threadIdx.x;
     if (t >= P.nfixed) return;
     int i = D.fixed_list[t];
     uint32_t fi = D.flags_b[i];
     int ci = flag_cls(fi);
     const ClassParams &C = cp[ci];
-    int mat = flag_mat(fi);
+    int mat = SOLID ? flag_mat(fi) : 1;
     double4 pi = D.pos_b[i];
     int key_i = D.key[i];
     int orig_i = D.orig_b[i];
@@ -408,7 +457,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
                 double4 vj = D.vel_b[j];
                 Pw += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
             }
-            if (mat > 1)
+            if (SOLID && mat > 1)
                 for (int q = 0; q < 6; ++q) sg[q] = sg[q] + D.sig_b[6*j+q]*K;
             if (C.NoSlip) {
                 double4 vj = D.vel_b[j];
@@ -419,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     // StartAcceleration resets fixed P, NSv, Sigma (Domain.cpp:990-996); then divide by SumKernel
     if (SK != 0.0) {
         if (mat < 3) Pw = Pw/SK;
-        if (mat > 1) for (int q = 0; q < 6; ++q) sg[q] = sg[q]*(1.0/SK);
+        if (SOLID && mat > 1) for (int q = 0; q < 6; ++q) sg[q] = sg[q]*(1.0/SK);
         if (C.NoSlip) { n0 = n0/SK; n1 = n1/SK; n2 = n2/SK; }
         else { n0 = n1 = n2 = 0.0; }
     } else {
@@ -451,7 +500,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
         else D.g2[i] = make_double4(vi.x, vi.y, vi.z, Cw);
         D.g3[i] = make_double4(C.FPMassC, 1.0/dw, 0.0, (double)ci);
     }
-    if (P.any_solid) {
+    if (SOLID) {
         if (mat > 1) for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = sg[q];
         else for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = 0.0;
         for (int q = 0; q < 6; ++q) D.tau_b[6*i+q] = 0.0;
@@ -461,9 +510,168 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     }
 }
 
+// Wall pass of the fast fluid configuration (cubic spline, uniform h, fluid only): the same
+// smoothing as k_wall with the pair test decided on rij^2 outside a 1e-13 band around the
+// support radius (inside it the exact rij/h <= Cellfac test runs), neighbour rows culled
+// against the support sphere, and cells without free particles skipped.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    int t = xcd_block()*kBlock + threadIdx.x;
+    if (t >= P.nfixed) return;
+    int i = D.fixed_list[t];
+    uint32_t fi = D.flags_b[i];
+    const bool noslip = flag_noslip(fi);
+    double4 pi = D.pos_b[i];
+    const int key_i = D.key[i];
+    const int orig_i = D.orig_b[i];
+    int kx, ky, kz;
+    decode_key(P, key_i, kx, ky, kz);
+    double SK = 0.0, Pw = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    const double gx = P.gravity[0], gy = P.gravity[1], gz = P.gravity[2];
+    const double h = P.h_uniform, cellfac = P.cellfac, kw = P.kw;
+    const double lo = P.sup2_lo, hi = P.sup2_hi;
+    const double sup2m = P.sup2_hi*(1.0 + 1e-9);
+    const double ox = P.blpf[0] + kx*P.csize[0], oy = P.blpf[1] + ky*P.csize[1];
+    const double oz = (DIM == 3) ? P.blpf[2] + kz*P.csize[2] : 0.0;
+    const int ncell = (DIM == 3) ? 27 : 9;
+    for (int cc = 0; cc < ncell; ++cc) {
+        int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (DIM == 3) ? (cc / 9) - 1 : 0;
+        int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+        if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+        int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
+        if (D.cfree[cell] == 0) continue;                  // no free particle to gather from
+        double ex = (dxk < 0) ? pi.x - ox : (dxk > 0) ? ox + P.csize[0] - pi.x : 0.0;
+        double ey = (dyk < 0) ? pi.y - oy : (dyk > 0) ? oy + P.csize[1] - pi.y : 0.0;
+        double ez = (dzk < 0) ? pi.z - oz : (dzk > 0) ? oz + P.csize[2] - pi.z : 0.0;
+        ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
+        if (ex*ex + ey*ey + ez*ez > sup2m) continue;       // the cell lies outside the support
+        int beg = D.cstart[cell], end = D.cstart[cell + 1];
+        for (int j = beg; j < end; ++j) {
+            if (!flag_free(D.flags_b[j])) continue;
+            double4 pj = D.pos_b[j];
+            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+            double r2 = x0*x0 + x1*x1 + x2*x2;
+            if (r2 > hi) continue;
+            double rij = sqrt(r2);
+            double q = rij/h;
+            if (r2 > lo && !(q <= cellfac)) continue;        // outside the reference's pair set
+            // orientation of the reference's pair (P1 = smaller key / larger index), formed
+            // branch-free as in k_wall
+            int kj = D.key[j];
+            int oj = D.orig_b[j];
+            int wall_first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
+            double sgn = wall_first ? 1.0 : -1.0;
+            x0 *= sgn; x1 *= sgn; x2 *= sgn;
+            double K;
+            if (q < 1.0) K = kw*(1.0-(3.0/2.0)*q*q+(3.0/4.0)*q*q*q);
+            else if (q < 2.0) K = kw*((1.0/4.0)*(2.0-q)*(2.0-q)*(2.0-q));
+            else K = 0.0;
+            SK += K;
+            double4 vj = D.vel_b[j];
+            Pw += vj.w * K + (gx*x0 + gy*x1 + gz*x2)*pj.w*K;
+            if (noslip) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
+        }
+    }
+    if (SK != 0.0) {
+        Pw = Pw/SK;
+        if (noslip) { n0 = n0/SK; n1 = n1/SK; n2 = n2/SK; }
+        else { n0 = n1 = n2 = 0.0; }
+    } else {
+        Pw = 0.0; n0 = n1 = n2 = 0.0;
+    }
+    double4 vi = D.vel_b[i];
+    vi.w = Pw;
+    D.vel_b[i] = vi;
+    D.shep[i] = make_double2(0.0, SK);
+    double dw = density_solid(P.eos_eq, P.eos_cs, P.eos_p0, Pw, P.eos_rho0);
+    double Cw = sound_speed(P.eos_eq, P.eos_cs, dw, P.eos_rho0);
+    double2 mm = D.mmu_b[i];
+    D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
+    D.wal[i] = make_double4(n0, n1, n2, dw);
+    const double fpm = D.cls[flag_cls(fi)].FPMassC;
+    const int ci = flag_cls(fi);
+    if (P.mixed32) {
+        double o0, o1, o2;
+        cell_origin(P, key_i, o0, o1, o2);
+        D.f0[i] = make_float4((float)(pi.x - o0), (float)(pi.y - o1), (float)(pi.z - o2), (float)dw);
+        D.f1[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)(Pw/(dw*dw)));
+        if (noslip) D.f2[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), (float)Cw);
+        else D.f2[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)Cw);
+        D.f3[i] = make_float4((float)fpm, (float)(1.0/dw), 0.0f, (float)ci);
+    } else {
+        D.g0[i] = make_double4(pi.x, pi.y, pi.z, dw);
+        D.g1[i] = make_double4(vi.x, vi.y, vi.z, Pw/(dw*dw));
+        if (noslip) D.g2[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, Cw);
+        else D.g2[i] = make_double4(vi.x, vi.y, vi.z, Cw);
+        D.g3[i] = make_double4(fpm, 1.0/dw, 0.0, (double)ci);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // force pass: CalcForce11 / CalcForce2233 gathered per free particle
 // ------------------------------------------------------------------------------------------
+// Per-block partials of a force pass: in-support pair counts (free-free directed, free-fixed)
+// and the minimum dt candidate (bits of a positive double, so unsigned order is double order).
+// k_force_fold combines them; no global atomics in the pass itself.
+__device__ inline void force_partials(const Dev &D, unsigned long long dtb, unsigned nff, unsigned nfw)
+{
+    __shared__ unsigned long long red[3][16];
+    unsigned long long sff = nff, sfw = nfw, sdt = dtb;
+    for (int o = 32; o > 0; o >>= 1) {
+        sff += __shfl_down(sff, o, 64);
+        sfw += __shfl_down(sfw, o, 64);
+        unsigned long long t = __shfl_down(sdt, o, 64);
+        sdt = t < sdt ? t : sdt;
+    }
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = sff; red[1][w] = sfw; red[2][w] = sdt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            sff += red[0][k]; sfw += red[1][k];
+            sdt = red[2][k] < sdt ? red[2][k] : sdt;
+        }
+        unsigned long long *fp = D.fpart + 3*(size_t)blockIdx.x;
+        fp[0] = sff; fp[1] = sfw; fp[2] = sdt;
+    }
+}
+
+// Folds the per-block partials of the force pass into the control block (one workgroup).
+__global__ void __launch_bounds__(1024) k_force_fold(Dev D, Params P, int nb)
+{
+    Ctl *c = D.ctl;
+    if (c->halt) return;
+    __shared__ unsigned long long red[3][16];
+    unsigned long long sff = 0, sfw = 0, sdt = ~0ull;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const unsigned long long *fp = D.fpart + 3*(size_t)b;
+        sff += fp[0]; sfw += fp[1];
+        sdt = fp[2] < sdt ? fp[2] : sdt;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sff += __shfl_down(sff, o, 64);
+        sfw += __shfl_down(sfw, o, 64);
+        unsigned long long t = __shfl_down(sdt, o, 64);
+        sdt = t < sdt ? t : sdt;
+    }
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = sff; red[1][w] = sfw; red[2][w] = sdt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            sff += red[0][k]; sfw += red[1][k];
+            sdt = red[2][k] < sdt ? red[2][k] : sdt;
+        }
+        c->pairs_ff += sff;
+        c->pairs_fw += sfw;
+        if (sdt < c->dtmin_bits) c->dtmin_bits = sdt;
+        c->nl_over += c->nl_q;
+        c->nl_q = 0;
+    }
+}
+
 struct Acc {
     double a0, a1, a2, dd;
     double vx0, vx1, vx2, zw, sd;
@@ -679,7 +887,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
     for (int k = threadIdx.x; k < P.nclasses; k += kForceBlock) cp[k] = D.cls[k];
     __syncthreads();
     const int tid = threadIdx.x;
-    int t = blockIdx.x*kForceBlock + tid;
+    int t = xcd_block()*kForceBlock + tid;
+    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
     bool act = t < P.nfree;
     int i = act ? D.free_list[t] : 0;
     uint32_t fi = D.flags_b[i];
@@ -750,20 +959,12 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
         double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
         double test = sqrt(cp[ci].h/na);
         double cand = P.sqrt_h_a*test;
-        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+        dtb = dbits(cand);
     }
     // pair statistics and the cross-material check
-    unsigned long long sff = nff, sfw = nfw;
-    for (int o = 32; o > 0; o >>= 1) {
-        sff += __shfl_down(sff, o, 64);
-        sfw += __shfl_down(sfw, o, 64);
-    }
     int anyx = wave_any(xmat != 0);
-    if ((tid & 63) == 0) {
-        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
-        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
-        if (anyx) atomicOr(&D.ctl->xmat, 1);
-    }
+    if ((tid & 63) == 0 && anyx) atomicOr(&D.ctl->xmat, 1);
+    force_partials(D, dtb, nff, nfw);
 }
 
 
@@ -798,7 +999,7 @@ struct Acc11 { double a0, a1, a2, dd; };
 template <int DIM>
 __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
                                    double4 gi0, double4 gi1, double4 gi2, double4 gi3, int ci, int j,
-                                   Acc11 &A, unsigned &nff, unsigned &nfw)
+                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j], a3 = D.g3[j];
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
@@ -833,7 +1034,7 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     }
     double mi = gi3.x;
     double mj = freej ? a3.x : a3.x*mi;
-    double mu = freej ? P.mu_ff : P.mu_ff_wall;
+    double mu = freej ? mu_ff : mu_w;
     double vs = 2.0*mu*gi3.y*a3.y*GK;
     double s = -(gi1.w + a1.w + PIij)*GK;
     double b0 = gi1.x - a2.x, b1 = gi1.y - a2.y, b2 = gi1.z - a2.z;
@@ -853,7 +1054,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
     if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
     __syncthreads();
     const int tid = threadIdx.x;
-    int t = blockIdx.x*kForceBlock + tid;
+    int t = xcd_block()*kForceBlock + tid;
+    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
     bool act = t < P.nfree;
     int i = act ? D.free_list[t] : 0;
     double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
@@ -867,6 +1069,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
     const int nrows = (DIM == 3) ? 9 : 3;
     const double lo = P.sup2_lo, hi = P.sup2_hi;
     const double sup2m = P.sup2_hi*(1.0 + 1e-9);
+    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     for (int r = 0; r < nrows; ++r) {
         int beg = 0, end = 0;
         if (act) row_range_culled(D, P, kx, ky, kz, r, gi0.x, gi0.y, gi0.z, sup2m, beg, end);
@@ -886,30 +1089,179 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
             }
             if (wave_any(cnt == kCap)) {
                 for (int e = 0; e < cnt; ++e)
-                    pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], A, nff, nfw);
+                    pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], mu_ff, mu_w, A, nff, nfw);
                 cnt = 0;
             }
         }
     }
     for (int e = 0; e < cnt; ++e)
-        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], A, nff, nfw);
+        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], mu_ff, mu_w, A, nff, nfw);
     if (act) {
         D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
         double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
-        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+        dtb = dbits(cand);
     }
-    unsigned long long sff = nff, sfw = nfw;
-    for (int o = 32; o > 0; o >>= 1) {
-        sff += __shfl_down(sff, o, 64);
-        sfw += __shfl_down(sfw, o, 64);
-    }
-    if ((tid & 63) == 0) {
-        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
-        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
+    force_partials(D, dtb, nff, nfw);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// neighbour lists of the fast fluid path.  k_nlist finds each free target's in-support
+// neighbours once per step (the exact FP64 pair test of the reference); the gathers then run
+// over the lists, so every lane of a wave evaluates pairs for nearly all of the loop instead of
+// waiting on candidate scans of other lanes.
+// ------------------------------------------------------------------------------------------
+
+// Visits the in-support neighbours of free target i (at px, py, pz in cell kx, ky, kz) in the
+// order the lists store them: neighbour rows (dy, dz) in row order, each culled to the cells
+// that intersect the support sphere, candidates in sorted order.  f(j, code) receives the
+// neighbour cell code (dx+1) + 3(dy+1) + 9(dz+1).
+template <int DIM, class F>
+__device__ inline void for_each_neighbour(const Dev &D, const Params &P, int i, double px, double py,
+                                          double pz, int kx, int ky, int kz, F &&f)
+{
+    const double lo = P.sup2_lo, hi = P.sup2_hi;
+    const double sup2m = hi*(1.0 + 1e-9);
+    const int nrows = (DIM == 3) ? 9 : 3;
+    for (int r = 0; r < nrows; ++r) {
+        int beg = 0, end = 0;
+        row_range_culled(D, P, kx, ky, kz, r, px, py, pz, sup2m, beg, end);
+        if (beg >= end) continue;
+        int dy = (r % 3) - 1, dz = (DIM == 3) ? (r / 3) - 1 : 0;
+        int base = P.nc[0]*((ky + dy) + P.nc[1]*(kz + dz));
+        int m0 = D.cstart[base + kx], m1 = D.cstart[base + kx + 1];   // the middle cell
+        int crow = 3*(dy + 1) + 9*(dz + 1);
+        double4 nx = D.pos_b[beg];
+        for (int j = beg; j < end; ++j) {
+            double4 pj = nx;
+            if (j + 1 < end) nx = D.pos_b[j + 1];
+            if (j == i) continue;
+            double x0 = px - pj.x, x1 = py - pj.y, x2 = pz - pj.z;
+            double r2 = x0*x0 + x1*x1 + x2*x2;
+            bool in = r2 <= lo;
+            if (!in && r2 <= hi) in = (sqrt(r2)/P.h_uniform) <= P.cellfac;
+            if (in) f(j, crow + ((j < m0) ? 0 : (j < m1) ? 1 : 2));
+        }
     }
 }
 
+__device__ inline int nl_entry(int j, int code) { return (int)((unsigned)j | ((unsigned)code << 27)); }
+__device__ inline int nl_index(int w) { return (int)((unsigned)w & 0x7ffffffu); }
+__device__ inline int nl_code(int w) { return (int)((unsigned)w >> 27); }
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    int t = xcd_block()*kBlock + threadIdx.x;
+    if (t >= P.nfree) return;
+    int i = D.free_list[t];
+    double4 pi = D.pos_b[i];
+    int kx, ky, kz;
+    decode_key(P, D.key[i], kx, ky, kz);
+    const int cap = P.nl_cap;
+    int4 *out = reinterpret_cast<int4 *>(D.nl) + t;          // slot group g at out[g*nfree]
+    const size_t gs = (size_t)P.nfree;
+    int cnt = 0;
+    int w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    for_each_neighbour<DIM>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, [&](int j, int code) {
+        int v = nl_entry(j, code);
+        int sl = cnt & 3;
+        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
+        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+        ++cnt;
+    });
+    if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+    D.ncnt[t] = cnt;
+}
+
+// FP64 fluid force pass over the neighbour lists (same pairs, same order, same arithmetic as
+// the candidate-scanning pass k_force11_fast).
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ double pa[64], pb[64];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    int t = xcd_block()*kForceBlock + tid;
+    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
+    bool act = t < P.nfree;
+    int i = act ? D.free_list[t] : 0;
+    double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
+    int ci = (int)gi3.w;
+    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
+    Acc11 A;
+    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
+    unsigned nff = 0, nfw = 0;
+    const int cap = P.nl_cap;
+    int cnt = act ? D.ncnt[t] : 0;
+    int m = cnt < cap ? cnt : cap;
+    const int4 *lst = reinterpret_cast<const int4 *>(D.nl) + t;
+    const size_t gs = (size_t)P.nfree;
+    int4 w = make_int4(0, 0, 0, 0);
+    for (int e = 0; e < m; ++e) {
+        int sl = e & 3;
+        if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
+        int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
+        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, nl_index(v), mu_ff, mu_w, A, nff, nfw);
+    }
+    if (act && cnt > cap) {
+        // the list overflowed: k_force11_nl_over continues this target from these sums
+        int q = atomicAdd(&D.ctl->nl_q, 1);
+        D.ovq[q] = t;
+        D.ovs[2*q] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        act = false;
+    }
+    if (act) {
+        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        dtb = dbits(cand);
+    }
+    force_partials(D, dtb, nff, nfw);
+}
+
+// Ends the targets of the FP64 list pass whose neighbours overflowed the lists: the neighbours
+// past the capacity are found again in list order and added to the queued partial sums, which
+// is the same sequence of operations as one pass over a long enough list.
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    const int nq = D.ctl->nl_q;
+    if (nq == 0) return;
+    __shared__ double pa[64], pb[64];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
+    const int cap = P.nl_cap;
+    for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
+        int t = D.ovq[q];
+        int i = D.free_list[t];
+        double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
+        int ci = (int)gi3.w;
+        double4 s0 = D.ovs[2*q];
+        Acc11 A;
+        A.a0 = s0.x; A.a1 = s0.y; A.a2 = s0.z; A.dd = s0.w;
+        unsigned nff = 0, nfw = 0;
+        int kx, ky, kz;
+        decode_key(P, D.key[i], kx, ky, kz);
+        int k = 0;
+        for_each_neighbour<DIM>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, [&](int j, int) {
+            if (k >= cap) pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, mu_ff, mu_w, A, nff, nfw);
+            ++k;
+        });
+        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+        if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
+        if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
+    }
+}
 
 // ------------------------------------------------------------------------------------------
 // FP32-mixed fluid force pass: the fast-path formulas in FP32 on cell-relative coordinates
@@ -982,7 +1334,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32(Dev D, Params P)
     if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
     __syncthreads();
     const int tid = threadIdx.x;
-    int t = blockIdx.x*kForceBlock + tid;
+    int t = xcd_block()*kForceBlock + tid;
+    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
     bool act = t < P.nfree;
     int i = act ? D.free_list[t] : 0;
     float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
@@ -1049,16 +1402,111 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32(Dev D, Params P)
         D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
-        if (cand < bitsd(D.ctl->dtmin_bits)) atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+        dtb = dbits(cand);
     }
-    unsigned long long sff = nff, sfw = nfw;
-    for (int o = 32; o > 0; o >>= 1) {
-        sff += __shfl_down(sff, o, 64);
-        sfw += __shfl_down(sfw, o, 64);
+    force_partials(D, dtb, nff, nfw);
+}
+
+// FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
+// offset dk CellSize of the cell-relative FP32 positions.
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    __shared__ float pa[64], pb[64];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    int t = xcd_block()*kForceBlock + tid;
+    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
+    bool act = t < P.nfree;
+    int i = act ? D.free_list[t] : 0;
+    float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
+    int ci = (int)gi3.w;
+    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
+    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    Acc32 A;
+    A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
+    unsigned nff = 0, nfw = 0;
+    const int cap = P.nl_cap;
+    int cnt = act ? D.ncnt[t] : 0;
+    int m = cnt < cap ? cnt : cap;
+    const int4 *lst = reinterpret_cast<const int4 *>(D.nl) + t;
+    const size_t gs = (size_t)P.nfree;
+    auto pair = [&](int j, int c) {
+        float p0 = ((c % 3) - 1)*csx, p1 = (((c / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((c / 9) - 1)*csz : 0.0f;
+        pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, p0, p1, p2,
+                        A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+    };
+    int4 w = make_int4(0, 0, 0, 0);
+    for (int e = 0; e < m; ++e) {
+        int sl = e & 3;
+        if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
+        int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
+        pair(nl_index(v), nl_code(v));
     }
-    if ((tid & 63) == 0) {
-        if (sff) atomicAdd(&D.ctl->pairs_ff, sff);
-        if (sfw) atomicAdd(&D.ctl->pairs_fw, sfw);
+    if (act && cnt > cap) {
+        // the list overflowed: k_force11_f32_nl_over continues this target from these sums
+        int q = atomicAdd(&D.ctl->nl_q, 1);
+        D.ovq[q] = t;
+        D.ovs[2*q] = make_double4((double)A.a0, (double)A.a1, (double)A.a2, (double)A.dd);
+        D.ovs[2*q+1] = make_double4((double)A.ddc, 0.0, 0.0, 0.0);
+        act = false;
+    }
+    if (act) {
+        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
+        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
+        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        dtb = dbits(cand);
+    }
+    force_partials(D, dtb, nff, nfw);
+}
+
+// Ends the overflowed targets of the FP32 list pass (see k_force11_nl_over); the queued FP32
+// sums are exact in FP64 and restored as FP32.
+template <int DIM>
+__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Params P)
+{
+    if (D.ctl->halt) return;
+    const int nq = D.ctl->nl_q;
+    if (nq == 0) return;
+    __shared__ float pa[64], pb[64];
+    if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
+    __syncthreads();
+    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
+    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const int cap = P.nl_cap;
+    for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
+        int t = D.ovq[q];
+        int i = D.free_list[t];
+        float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
+        int ci = (int)gi3.w;
+        double4 s0 = D.ovs[2*q], s1 = D.ovs[2*q+1];
+        Acc32 A;
+        A.a0 = (float)s0.x; A.a1 = (float)s0.y; A.a2 = (float)s0.z; A.dd = (float)s0.w; A.ddc = (float)s1.x;
+        unsigned nff = 0, nfw = 0;
+        int kx, ky, kz;
+        decode_key(P, D.key[i], kx, ky, kz);
+        double4 pi = D.pos_b[i];
+        int k = 0;
+        for_each_neighbour<DIM>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, [&](int j, int c) {
+            if (k >= cap) {
+                float p0 = ((c % 3) - 1)*csx, p1 = (((c / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((c / 9) - 1)*csz : 0.0f;
+                pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, p0, p1, p2,
+                                A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+            }
+            ++k;
+        });
+        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
+        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
+        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
+        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+        atomicMin(&D.ctl->dtmin_bits, dbits(cand));
+        if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
+        if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
     }
 }
 
@@ -1168,6 +1616,8 @@ __device__ inline void dp_return(const ClassParams &c, double i22, double *St, c
     for (int k = 0; k < 6; ++k) tau6[k] = S[k];
 }
 
+// SOLID = false compiles the fluid-only update (no Material 2/3 particles in the run).
+template <bool SOLID>
 __global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
 {
     Ctl *ctl = D.ctl;
@@ -1230,7 +1680,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
             }
         }
         double2 mm = D.mmu_b[i];
-        if (c.Material == 1) {
+        if (!SOLID || c.Material == 1) {
             // Mat1, Particle.cpp:146-186
             P_ = eos(c.PresEq, c.Cs, c.P0, rho, c.RefDensity);
             if (P.any_les_t0) {
@@ -1352,8 +1802,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
     int kx, ky, kz;
     int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
     D.key[i] = key;
-    D.rank[i] = atomicAdd(&D.ccount[key], 1);
-    if (flag_free(f)) atomicAdd(&D.cfcnt[key], 1);
+    D.rank[i] = cell_rank(D, key, flag_free(f));
 }
 
 // Time update and stop conditions (Solve loop, Domain.cpp:1762-1793); one thread.
@@ -1471,20 +1920,61 @@ void launch_scatter(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_scatter, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
 
 void launch_cellsort(const Dev &D, const Params &P, hipStream_t s)
-{ hipLaunchKernelGGL(k_cellsort, dim3(nblk(P.ncells, kBlock)), dim3(kBlock), 0, s, D, P); }
+{ hipLaunchKernelGGL(k_cellsort, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
 
 void launch_reorder(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_reorder, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
 
 void launch_wall(const Dev &D, const Params &P, hipStream_t s)
 {
-    if (P.nfixed > 0)
-        hipLaunchKernelGGL(k_wall, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+    if (P.nfixed <= 0) return;
+    if (P.fast11 || P.mixed32) {
+        if (P.dim == 3)
+            hipLaunchKernelGGL(k_wall_fast<3>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+        else
+            hipLaunchKernelGGL(k_wall_fast<2>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+        return;
+    }
+    if (P.any_solid)
+        hipLaunchKernelGGL(k_wall<true>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+    else
+        hipLaunchKernelGGL(k_wall<false>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
 }
+
+static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s);
 
 void launch_force(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfree <= 0) return;
+    launch_force_pass(D, P, s);
+    hipLaunchKernelGGL(k_force_fold, dim3(1), dim3(1024), 0, s, D, P, nblk(P.nfree, kForceBlock));
+}
+
+static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
+{
+    if (P.nl_cap > 0) {
+        const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
+        if (P.dim == 3) {
+            hipLaunchKernelGGL(k_nlist<3>, gl, dim3(kBlock), 0, s, D, P);
+            if (P.mixed32) {
+                hipLaunchKernelGGL(k_force11_f32_nl<3>, gf, dim3(kForceBlock), 0, s, D, P);
+                hipLaunchKernelGGL(k_force11_f32_nl_over<3>, go, dim3(kForceBlock), 0, s, D, P);
+            } else {
+                hipLaunchKernelGGL(k_force11_nl<3>, gf, dim3(kForceBlock), 0, s, D, P);
+                hipLaunchKernelGGL(k_force11_nl_over<3>, go, dim3(kForceBlock), 0, s, D, P);
+            }
+        } else {
+            hipLaunchKernelGGL(k_nlist<2>, gl, dim3(kBlock), 0, s, D, P);
+            if (P.mixed32) {
+                hipLaunchKernelGGL(k_force11_f32_nl<2>, gf, dim3(kForceBlock), 0, s, D, P);
+                hipLaunchKernelGGL(k_force11_f32_nl_over<2>, go, dim3(kForceBlock), 0, s, D, P);
+            } else {
+                hipLaunchKernelGGL(k_force11_nl<2>, gf, dim3(kForceBlock), 0, s, D, P);
+                hipLaunchKernelGGL(k_force11_nl_over<2>, go, dim3(kForceBlock), 0, s, D, P);
+            }
+        }
+        return;
+    }
     if (P.mixed32) {
         if (P.dim == 3)
             hipLaunchKernelGGL(k_force11_f32<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
@@ -1506,7 +1996,12 @@ void launch_force(const Dev &D, const Params &P, hipStream_t s)
 }
 
 void launch_integrate(const Dev &D, const Params &P, hipStream_t s)
-{ hipLaunchKernelGGL(k_integrate, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
+{
+    if (P.any_solid)
+        hipLaunchKernelGGL(k_integrate<true>, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+    else
+        hipLaunchKernelGGL(k_integrate<false>, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+}
 
 void launch_tail(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, D, P); }

@@ -198,3 +198,45 @@ def test_fp32_mixed_against_fp64(dim):
     G, O_ = g.particles(), o.particles()
     np.testing.assert_allclose(G["x"], O_["x"], rtol=0, atol=1e-7)
     np.testing.assert_allclose(G["Density"], O_["Density"], rtol=1e-5)
+
+
+def _run_env(monkeypatch, env, cfg, p, dt, steps):
+    for k in ("PSPH_NO_NLIST", "PSPH_NL_CAP"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    g = Solver(cfg, p, dt)
+    st = g.run(steps)
+    q = g.particles()
+    g.close()
+    return st, q
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_neighbour_lists_match_candidate_scan(monkeypatch, dim, precision):
+    """The force pass over per-step neighbour lists evaluates the same pairs in the same order
+    as the pass that scans the candidate cells itself: bitwise identical states, also when the
+    lists overflow (capacity 8: most targets are finished by the overflow pass, and the run
+    grows the lists between batches)."""
+    from persiansph_amd import FP32_MIXED
+    if dim == 2:
+        cfg, p, dt = cases.dam_break_2d(dx=0.02, scheme=1)
+    else:
+        cfg, p, dt = _small_3d(1)
+    if precision == "fp32":
+        cfg = type(cfg).from_buffer_copy(cfg)
+        cfg.precision = FP32_MIXED
+    print(f"[nlist] dim={dim} {precision} n={len(p)}", flush=True)
+    steps = 40
+    base = _run_env(monkeypatch, {}, cfg, p, dt, steps)
+    over = _run_env(monkeypatch, {"PSPH_NL_CAP": "8"}, cfg, p, dt, steps)
+    assert base[0].steps == over[0].steps == steps
+    assert base[0].pairs == over[0].pairs
+    for f in ("x", "v", "Density", "a"):
+        np.testing.assert_array_equal(base[1][f], over[1][f], err_msg=f)
+    if precision == "fp64":
+        scan = _run_env(monkeypatch, {"PSPH_NO_NLIST": "1"}, cfg, p, dt, steps)
+        assert base[0].pairs == scan[0].pairs
+        for f in ("x", "v", "Density", "a"):
+            np.testing.assert_array_equal(base[1][f], scan[1][f], err_msg=f)

@@ -46,10 +46,14 @@ def test_soil_first_force_pass(dim, scheme):
     np.testing.assert_allclose(G["Sigma"][fx], O_["Sigma"][fx], rtol=1e-12, atol=1e-12*np.abs(O_["Sigma"]).max())
 
 
-@pytest.mark.parametrize("dim,scheme", [(2, 1), (2, 0), (3, 1)])
+# 20 steps: this column unloads to the stress-free Drucker-Prager apex around step 40, where
+# the plastic flow direction ShearStress/sqrt(J2) is set by roundoff, so even two oracle runs
+# 1 ulp apart part by O(1e-1) of |Sigma| by step 50 (tests/test_oracle_soil_sensitivity.py);
+# over 20 steps they agree to ~1e-15.
+@pytest.mark.parametrize("dim,scheme", [(2, 1), (2, 0), (3, 1), (3, 0)])
 def test_soil_trajectory(dim, scheme):
     cfg, p, dt = _soil(dim, scheme)
-    steps = 50
+    steps = 20
     print(f"[soil traj] dim={dim} scheme={scheme} n={len(p)}", flush=True)
     g = Solver(cfg, p, dt)
     o = Oracle(cfg, p, dt)
