@@ -811,8 +811,11 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     if (rc) return rc;
     // neighbour lists for the fast fluid path (entries pack the index in 27 bits)
     P.nl_cap = 0;
+    P.nl_codes = n < ((size_t)1 << 27) ? 1 : 0;
     h->nl = nullptr;
-    if ((P.fast11 || P.mixed32) && n < ((size_t)1 << 27) && !std::getenv("PSPH_NO_NLIST")) {
+    if (P.mixed32 && !P.nl_codes)
+        return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier holds at most 2^27 particles");
+    if (P.fast11 || P.mixed32) {
         int cap0 = nl_initial_cap(h);
         if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);
         rc = nl_alloc(h, cap0);

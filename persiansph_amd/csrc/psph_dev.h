@@ -86,8 +86,9 @@ struct Params {
     double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;
-    // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4)
-    int nl_cap;
+    // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4), and
+    // whether entries carry the neighbour cell code (needs fewer than 2^27 particles)
+    int nl_cap, nl_codes;
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.

@@ -1045,67 +1045,6 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     A.dd += mj*(di*a3.y)*temp1;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kForceBlock) k_force11_fast(Dev D, Params P)
-{
-    if (D.ctl->halt) return;
-    __shared__ double pa[64], pb[64];
-    __shared__ int lst[kCap*kForceBlock];
-    if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
-    __syncthreads();
-    const int tid = threadIdx.x;
-    int t = xcd_block()*kForceBlock + tid;
-    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
-    bool act = t < P.nfree;
-    int i = act ? D.free_list[t] : 0;
-    double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
-    int ci = (int)gi3.w;
-    int kx, ky, kz;
-    decode_key(P, D.key[i], kx, ky, kz);
-    Acc11 A;
-    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
-    unsigned nff = 0, nfw = 0;
-    int cnt = 0;
-    const int nrows = (DIM == 3) ? 9 : 3;
-    const double lo = P.sup2_lo, hi = P.sup2_hi;
-    const double sup2m = P.sup2_hi*(1.0 + 1e-9);
-    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
-    for (int r = 0; r < nrows; ++r) {
-        int beg = 0, end = 0;
-        if (act) row_range_culled(D, P, kx, ky, kz, r, gi0.x, gi0.y, gi0.z, sup2m, beg, end);
-        double4 nx = D.g0[beg < end ? beg : i];
-        for (int j = beg; j < end; ++j) {
-            double4 pj = nx;
-            if (j + 1 < end) nx = D.g0[j + 1];
-            if (j != i) {
-                double x0 = gi0.x - pj.x, x1 = gi0.y - pj.y, x2 = gi0.z - pj.z;
-                double r2 = x0*x0 + x1*x1 + x2*x2;
-                bool in = r2 <= lo;
-                if (!in && r2 <= hi) in = (sqrt(r2)/P.h_uniform) <= P.cellfac;
-                if (in) {
-                    lst[cnt*kForceBlock + tid] = j;
-                    ++cnt;
-                }
-            }
-            if (wave_any(cnt == kCap)) {
-                for (int e = 0; e < cnt; ++e)
-                    pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], mu_ff, mu_w, A, nff, nfw);
-                cnt = 0;
-            }
-        }
-    }
-    for (int e = 0; e < cnt; ++e)
-        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], mu_ff, mu_w, A, nff, nfw);
-    if (act) {
-        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
-        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
-        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
-        dtb = dbits(cand);
-    }
-    force_partials(D, dtb, nff, nfw);
-}
-
-
 // ------------------------------------------------------------------------------------------
 // neighbour lists of the fast fluid path.  k_nlist finds each free target's in-support
 // neighbours once per step (the exact FP64 pair test of the reference); the gathers then run
@@ -1165,8 +1104,9 @@ __global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
     const size_t gs = (size_t)P.nfree;
     int cnt = 0;
     int w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    const bool codes = P.nl_codes != 0;
     for_each_neighbour<DIM>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, [&](int j, int code) {
-        int v = nl_entry(j, code);
+        int v = codes ? nl_entry(j, code) : j;
         int sl = cnt & 3;
         w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
         if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
@@ -1176,8 +1116,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
     D.ncnt[t] = cnt;
 }
 
-// FP64 fluid force pass over the neighbour lists (same pairs, same order, same arithmetic as
-// the candidate-scanning pass k_force11_fast).
+// FP64 fluid force pass over the neighbour lists.
 template <int DIM>
 __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
 {
@@ -1197,6 +1136,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
     A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
     unsigned nff = 0, nfw = 0;
     const int cap = P.nl_cap;
+    const bool codes = P.nl_codes != 0;
     int cnt = act ? D.ncnt[t] : 0;
     int m = cnt < cap ? cnt : cap;
     const int4 *lst = reinterpret_cast<const int4 *>(D.nl) + t;
@@ -1206,7 +1146,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
         int sl = e & 3;
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
-        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, nl_index(v), mu_ff, mu_w, A, nff, nfw);
+        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, codes ? nl_index(v) : v, mu_ff, mu_w, A, nff, nfw);
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_nl_over continues this target from these sums
@@ -1322,89 +1262,6 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
     float tt = A.dd + y;
     A.ddc = (tt - A.dd) - y;
     A.dd = tt;
-}
-
-template <int DIM>
-__global__ void __launch_bounds__(kForceBlock) k_force11_f32(Dev D, Params P)
-{
-    if (D.ctl->halt) return;
-    __shared__ float pa[64], pb[64];
-    __shared__ int lst[kCap*kForceBlock];
-    __shared__ unsigned char lcell[kCap*kForceBlock];
-    if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
-    __syncthreads();
-    const int tid = threadIdx.x;
-    int t = xcd_block()*kForceBlock + tid;
-    unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
-    bool act = t < P.nfree;
-    int i = act ? D.free_list[t] : 0;
-    float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
-    int ci = (int)gi3.w;
-    int kx, ky, kz;
-    decode_key(P, D.key[i], kx, ky, kz);
-    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
-    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
-    const float sup2 = (float)(P.cellfac*P.h_uniform*P.cellfac*P.h_uniform);
-    Acc32 A;
-    A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
-    unsigned nff = 0, nfw = 0;
-    int cnt = 0;
-    const int ncell = (DIM == 3) ? 27 : 9;
-    for (int c = 0; c < ncell; ++c) {
-        int dxk = (c % 3) - 1, dyk = ((c / 3) % 3) - 1, dzk = (DIM == 3) ? (c / 9) - 1 : 0;
-        int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
-        int beg = 0, end = 0;
-        // distance from the target (cell-relative position) to the neighbour cell's box
-        float ex = (dxk < 0) ? gi0.x : (dxk > 0) ? csx - gi0.x : 0.0f;
-        float ey = (dyk < 0) ? gi0.y : (dyk > 0) ? csy - gi0.y : 0.0f;
-        float ez = (dzk < 0) ? gi0.z : (dzk > 0) ? csz - gi0.z : 0.0f;
-        ex = ex > 0.0f ? ex : 0.0f; ey = ey > 0.0f ? ey : 0.0f; ez = ez > 0.0f ? ez : 0.0f;
-        bool near = ex*ex + ey*ey + ez*ez <= sup2*1.0001f;
-        if (act && near && xx >= 0 && xx < P.nc[0] && yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
-            int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
-            beg = D.cstart[cell];
-            end = D.cstart[cell + 1];
-        }
-        float o0 = dxk*csx, o1 = dyk*csy, o2 = dzk*csz;
-        float4 nx = D.f0[beg < end ? beg : i];
-        for (int j = beg; j < end; ++j) {
-            float4 pj = nx;
-            if (j + 1 < end) nx = D.f0[j + 1];
-            if (j != i) {
-                float x0 = (gi0.x - pj.x) - o0, x1 = (gi0.y - pj.y) - o1, x2 = (gi0.z - pj.z) - o2;
-                float r2 = x0*x0 + x1*x1 + x2*x2;
-                if (r2 <= sup2) {
-                    lst[cnt*kForceBlock + tid] = j;
-                    lcell[cnt*kForceBlock + tid] = (unsigned char)c;
-                    ++cnt;
-                }
-            }
-            if (wave_any(cnt == kCap)) {
-                for (int e = 0; e < cnt; ++e) {
-                    int ce = lcell[e*kForceBlock + tid];
-                    float p0 = ((ce % 3) - 1)*csx, p1 = (((ce / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((ce / 9) - 1)*csz : 0.0f;
-                    pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], p0, p1, p2,
-                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
-                }
-                cnt = 0;
-            }
-        }
-    }
-    for (int e = 0; e < cnt; ++e) {
-        int ce = lcell[e*kForceBlock + tid];
-        float p0 = ((ce % 3) - 1)*csx, p1 = (((ce / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((ce / 9) - 1)*csz : 0.0f;
-        pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, lst[e*kForceBlock + tid], p0, p1, p2,
-                        A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
-    }
-    if (act) {
-        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
-        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
-        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
-        double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
-        dtb = dbits(cand);
-    }
-    force_partials(D, dtb, nff, nfw);
 }
 
 // FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
@@ -1973,20 +1830,6 @@ static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
                 hipLaunchKernelGGL(k_force11_nl_over<2>, go, dim3(kForceBlock), 0, s, D, P);
             }
         }
-        return;
-    }
-    if (P.mixed32) {
-        if (P.dim == 3)
-            hipLaunchKernelGGL(k_force11_f32<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
-        else
-            hipLaunchKernelGGL(k_force11_f32<2>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
-        return;
-    }
-    if (P.fast11) {
-        if (P.dim == 3)
-            hipLaunchKernelGGL(k_force11_fast<3>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
-        else
-            hipLaunchKernelGGL(k_force11_fast<2>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
         return;
     }
     if (P.any_solid)

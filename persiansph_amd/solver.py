@@ -11,7 +11,8 @@ import numpy as np
 from .abi import ERRORS, PARTICLE_DTYPE, Config, Grid, Status, grid_dict
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libpersiansph_amd.so")
+# PSPH_LIB names another build of the same library (same C ABI), e.g. to compare builds
+LIB_PATH = os.environ.get("PSPH_LIB") or os.path.join(_HERE, "lib", "libpersiansph_amd.so")
 
 EXPORTS = [
     "psph_create", "psph_upload", "psph_run", "psph_download", "psph_neighbour_pairs",
