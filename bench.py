@@ -24,6 +24,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/s: FP64 vs FP32-mixed vs surrogate, 1 GPU, ≤1% front error"
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(precision, dx):
+    """HBM bytes per force pass from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (copy-calibrated; written by python -m persiansph_amd.profiling traffic)."""
+    if dx != 0.01:
+        return None
+    path = os.path.join(PROFILES, f"traffic_force_pass_{precision}_dambreak3d_1M.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_pass")
 
 
 def log(*a):
@@ -112,9 +125,11 @@ def main():
     if not args.no_timing and stats.get("force", (0, 0))[1] > 0:
         fms = stats["force"][0]
         ach = force_b/(fms*1e-3)/1e9
+        tb = measured_traffic(args.precision, args.dx)
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach/HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_force (pair-force gather)", "kernel_ms": round(fms, 5),
+                "frac": round(ach/HBM_PEAK_GBS, 5), "traffic": int(tb) if tb else None,
+                "kernel": "force pass: k_nlist + k_force11_nl (FP64) / k_force11_f32_nl (FP32) "
+                          "+ overflow + fold", "kernel_ms": round(fms, 5),
                 "algorithmic_bytes_per_launch": int(force_b)}
         log("[bench] kernel means (ms): " + ", ".join(f"{k}={v[0]:.4f}" for k, v in stats.items() if v[1]))
     pair_rate = st.pairs*args.steps/el

@@ -53,6 +53,9 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
+// error reporting for the other entry points of the library (the surrogate's psph_gnn_*)
+int psph_internal_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
 #define HIPCHECK(expr)                                                                        \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
@@ -800,12 +803,17 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                 }
         }
     }
+    P.soil32 = 0;
     if (P.precision == PSPH_FP32_MIXED) {
-        if (!P.fast11)
-            return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier covers fluid runs with the cubic spline, Morris "
-                        "viscosity, the squared-density form, uniform h and mu, no artificial stress, "
-                        "no XSPH/Shepard terms");
-        P.mixed32 = 1;
+        if (mat0 > 1) {
+            P.soil32 = 1;            // solid/soil: FP32 pair sums, FP64 stress update
+        } else {
+            if (!P.fast11)
+                return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier covers fluid runs with the cubic spline, "
+                            "Morris viscosity, the squared-density form, uniform h and mu, no artificial stress, "
+                            "no XSPH/Shepard terms, and solid/soil runs");
+            P.mixed32 = 1;
+        }
     }
     rc = allocate(h, n);
     if (rc) return rc;

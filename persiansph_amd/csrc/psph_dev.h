@@ -89,6 +89,8 @@ struct Params {
     // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4), and
     // whether entries carry the neighbour cell code (needs fewer than 2^27 particles)
     int nl_cap, nl_codes;
+    // FP32-mixed tier of solid/soil runs: CalcForce2233 pair arithmetic and sums in FP32
+    int soil32;
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.

@@ -518,9 +518,12 @@ template <int DIM>
 __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
 {
     if (D.ctl->halt) return;
-    int t = xcd_block()*kBlock + threadIdx.x;
-    if (t >= P.nfixed) return;
-    int i = D.fixed_list[t];
+    // four lanes per fixed target, lane q taking neighbour cells q, q+4, ...; their sums meet
+    // in a fixed order ((q0 + q1) + (q2 + q3)), so results stay deterministic
+    const int gt = xcd_block()*kBlock + threadIdx.x;
+    const int t = gt >> 2, quad = gt & 3;
+    const bool act = t < P.nfixed;
+    int i = act ? D.fixed_list[t] : 0;
     uint32_t fi = D.flags_b[i];
     const bool noslip = flag_noslip(fi);
     double4 pi = D.pos_b[i];
@@ -535,8 +538,8 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
     const double sup2m = P.sup2_hi*(1.0 + 1e-9);
     const double ox = P.blpf[0] + kx*P.csize[0], oy = P.blpf[1] + ky*P.csize[1];
     const double oz = (DIM == 3) ? P.blpf[2] + kz*P.csize[2] : 0.0;
-    const int ncell = (DIM == 3) ? 27 : 9;
-    for (int cc = 0; cc < ncell; ++cc) {
+    const int ncell = act ? ((DIM == 3) ? 27 : 9) : 0;
+    for (int cc = quad; cc < ncell; cc += 4) {
         int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (DIM == 3) ? (cc / 9) - 1 : 0;
         int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
         if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
@@ -574,6 +577,11 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
             if (noslip) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
         }
     }
+    SK += __shfl_xor(SK, 1, 64); Pw += __shfl_xor(Pw, 1, 64);
+    n0 += __shfl_xor(n0, 1, 64); n1 += __shfl_xor(n1, 1, 64); n2 += __shfl_xor(n2, 1, 64);
+    SK += __shfl_xor(SK, 2, 64); Pw += __shfl_xor(Pw, 2, 64);
+    n0 += __shfl_xor(n0, 2, 64); n1 += __shfl_xor(n1, 2, 64); n2 += __shfl_xor(n2, 2, 64);
+    if (!act || quad != 0) return;
     if (SK != 0.0) {
         Pw = Pw/SK;
         if (noslip) { n0 = n0/SK; n1 = n1/SK; n2 = n2/SK; }
@@ -672,12 +680,16 @@ __global__ void __launch_bounds__(1024) k_force_fold(Dev D, Params P, int nb)
     }
 }
 
-struct Acc {
-    double a0, a1, a2, dd;
-    double vx0, vx1, vx2, zw, sd;
-    double sr[6];
-    double rr[3];
+// Per-target sums of a force pass.  T = float for the FP32-mixed soil tier, whose continuity
+// sum carries a Kahan compensation term ddc.
+template <class T>
+struct AccT {
+    T a0, a1, a2, dd, ddc;
+    T vx0, vx1, vx2, zw, sd;
+    T sr[6];
+    T rr[3];
 };
+using Acc = AccT<double>;
 
 // CalcForce11 (Interaction.cpp:25-184) contribution of neighbour j to free particle i as P1.
 __device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
@@ -779,10 +791,13 @@ __device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *
     }
 }
 
-// CalcForce2233 (Interaction.cpp:186-377) contribution of neighbour j to free particle i as P1.
+// CalcForce2233 (Interaction.cpp:186-377) contribution of neighbour j to free particle i as P1,
+// with the pair arithmetic and sums in T (double: the parity tier; float: the FP32-mixed tier,
+// positions differenced in FP64 and rounded, Kahan-compensated continuity sum).
+template <class T>
 __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                                 uint32_t fi, double4 pi, double4 vi, double4 ai, const double *sgi,
-                                const double *tri, Acc &A, int shep_on)
+                                const double *tri, AccT<T> &A, int shep_on)
 {
     uint32_t fj = D.flags_b[j];
     int ci = flag_cls(fi), cj = flag_cls(fj);
@@ -792,85 +807,94 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
     double4 vj = D.vel_b[j];
     double4 aj = D.aux[j];
     bool freej = flag_free(fj);
-    double h = pair_h(cp, P, ci, cj);
-    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
-    double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-    double Alpha = (Ci_.Alpha + Cj_.Alpha)/2.0;
-    double Beta = (Ci_.Beta + Cj_.Beta)/2.0;
-    double di = pi.w, mi = ai.x, dj, mj;
+    T h = (T)pair_h(cp, P, ci, cj);
+    T x0 = (T)(pi.x - pj.x), x1 = (T)(pi.y - pj.y), x2 = (T)(pi.z - pj.z);
+    T rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+    T Alpha = (T)((Ci_.Alpha + Cj_.Alpha)/2.0);
+    T Beta = (T)((Ci_.Beta + Cj_.Beta)/2.0);
+    T di = (T)pi.w, mi = (T)ai.x, dj, mj;
     bool soil = (Ci_.Material*Cj_.Material == 9);
     double4 wj = make_double4(0.0, 0.0, 0.0, 0.0);
     if (!freej) wj = D.wal[j];
     if (soil) {
-        if (!freej) { dj = pi.w; mj = Cj_.FPMassC * mi; } else { dj = pj.w; mj = aj.x; }
+        if (!freej) { dj = (T)pi.w; mj = (T)Cj_.FPMassC * mi; } else { dj = (T)pj.w; mj = (T)aj.x; }
     } else {
-        if (!freej) { dj = wj.w; mj = Cj_.FPMassC * mi; } else { dj = pj.w; mj = aj.x; }
+        if (!freej) { dj = (T)wj.w; mj = (T)Cj_.FPMassC * mi; } else { dj = (T)pj.w; mj = (T)aj.x; }
     }
-    double u0 = vi.x - vj.x, u1 = vi.y - vj.y, u2 = vi.z - vj.z;
-    double q = rij/h;
-    double GK = grad_kernel<double>(P.dim, P.kernel_type, q, h);
-    double K = kernel_w<double>(P.dim, P.kernel_type, q, h);
-    double PIs = 0.0;
+    T u0 = (T)(vi.x - vj.x), u1 = (T)(vi.y - vj.y), u2 = (T)(vi.z - vj.z);
+    T q = rij/h;
+    T GK = grad_kernel<T>(P.dim, P.kernel_type, q, h);
+    T K = kernel_w<T>(P.dim, P.kernel_type, q, h);
+    T PIs = 0.0;
     bool pi_on = false;
-    if (Alpha != 0.0 || Beta != 0.0) {
-        double dvx = u0*x0 + u1*x1 + u2*x2;
-        double MUij = h*dvx/(rij*rij+0.01*h*h);
-        double Cij;
-        if (soil) Cij = 0.5*(Ci_.Cs+Cj_.Cs);
-        else Cij = 0.5*(ai.z+aj.z);
-        if (dvx < 0) { PIs = (Alpha*Cij*MUij+Beta*MUij*MUij)/(0.5*(di+dj)); pi_on = true; }
+    if (Alpha != (T)0.0 || Beta != (T)0.0) {
+        T dvx = u0*x0 + u1*x1 + u2*x2;
+        T MUij = h*dvx/(rij*rij+(T)0.01*h*h);
+        T Cij;
+        if (soil) Cij = (T)(0.5*(Ci_.Cs+Cj_.Cs));
+        else Cij = (T)(0.5*(ai.z+aj.z));
+        if (dvx < (T)0) { PIs = (Alpha*Cij*MUij+Beta*MUij*MUij)/((T)0.5*(di+dj)); pi_on = true; }
     }
-    double TI6[6] = {0, 0, 0, 0, 0, 0};
+    T TI6[6] = {0, 0, 0, 0, 0, 0};
     if (Ci_.TI > 0.0 || Cj_.TI > 0.0) {
-        double sT = pow((K/kernel_w<double>(P.dim, P.kernel_type, (Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*h), h)),(Ci_.TIn+Cj_.TIn)/2.0);
-        for (int k = 0; k < 6; ++k) TI6[k] = (tri[k]+D.tir[6*j+k])*sT;
+        T sT = pow((K/kernel_w<T>(P.dim, P.kernel_type, (T)((Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*(double)h)), h)),
+                   (T)((Ci_.TIn+Cj_.TIn)/2.0));
+        for (int k = 0; k < 6; ++k) TI6[k] = (T)(tri[k]+D.tir[6*j+k])*sT;
     }
-    double b0, b1, b2;
+    T b0, b1, b2;
     if (freej) { b0 = u0; b1 = u1; b2 = u2; }
     else if (Ci_.NoSlip || Cj_.NoSlip) {
-        b0 = vi.x - (2.0*vj.x-wj.x); b1 = vi.y - (2.0*vj.y-wj.y); b2 = vi.z - (2.0*vj.z-wj.z);
+        b0 = (T)(vi.x - (2.0*vj.x-wj.x)); b1 = (T)(vi.y - (2.0*vj.y-wj.y)); b2 = (T)(vi.z - (2.0*vj.z-wj.z));
     } else {
         double4 vbj = D.vh_b[j];    // fixed particles keep vb in the half-step column
-        b0 = vi.x - vbj.x; b1 = vi.y - vbj.y; b2 = vi.z - vbj.z;
+        b0 = (T)(vi.x - vbj.x); b1 = (T)(vi.y - vbj.y); b2 = (T)(vi.z - vbj.z);
     }
-    double f05 = -0.5 * GK;
-    double s00 = (2.0*b0*x0)*f05, s01 = (b0*x1+b1*x0)*f05, s02 = (b0*x2+b2*x0)*f05;
-    double s11 = (2.0*b1*x1)*f05, s12 = (b1*x2+b2*x1)*f05, s22 = (2.0*b2*x2)*f05;
-    double r01 = (b0*x1-b1*x0)*f05, r02 = (b0*x2-b2*x0)*f05, r12 = (b1*x2-b2*x1)*f05;
+    T f05 = (T)-0.5 * GK;
+    T s00 = ((T)2.0*b0*x0)*f05, s01 = (b0*x1+b1*x0)*f05, s02 = (b0*x2+b2*x0)*f05;
+    T s11 = ((T)2.0*b1*x1)*f05, s12 = (b1*x2+b2*x1)*f05, s22 = ((T)2.0*b2*x2)*f05;
+    T r01 = (b0*x1-b1*x0)*f05, r02 = (b0*x2-b2*x0)*f05, r12 = (b1*x2-b2*x1)*f05;
     if (P.xsph != 0.0 && freej) {
-        double sx = P.xsph*mj/(0.5*(di+dj))*K;
+        T sx = (T)P.xsph*mj/((T)0.5*(di+dj))*K;
         A.vx0 += sx*-u0; A.vx1 += sx*-u1; A.vx2 += sx*-u2;
     }
     // M = Sigma_i/di^2 + Sigma_j/dj^2 + PI I + TI, then temp = (GK xij)^T M (matvec.h:806-811)
-    double Mf[9];
-    double sgj[6];
-    for (int k = 0; k < 6; ++k) sgj[k] = D.sig_b[6*j+k];
-    double Mi[9], Mj[9], T9[9];
-    sym_to_full(sgi, Mi);
+    T Mf[9];
+    T sgiT[6], sgj[6];
+    for (int k = 0; k < 6; ++k) { sgiT[k] = (T)sgi[k]; sgj[k] = (T)D.sig_b[6*j+k]; }
+    T Mi[9], Mj[9], T9[9];
+    sym_to_full(sgiT, Mi);
     sym_to_full(sgj, Mj);
     sym_to_full(TI6, T9);
     if (P.gradient_type == 0) {
-        double fa = 1.0/(di*di), fb = 1.0/(dj*dj);
+        T fa = (T)1.0/(di*di), fb = (T)1.0/(dj*dj);
         for (int k = 0; k < 9; ++k) {
-            double pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0)*PIs : 0.0;
+            T pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? (T)1.0 : (T)0.0)*PIs : (T)0.0;
             Mf[k] = Mi[k]*fa + Mj[k]*fb + pim + T9[k];
         }
     } else {
-        double fa = 1.0/(di*dj);
+        T fa = (T)1.0/(di*dj);
         for (int k = 0; k < 9; ++k) {
-            double pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0)*PIs : 0.0;
+            T pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? (T)1.0 : (T)0.0)*PIs : (T)0.0;
             Mf[k] = (Mi[k] + Mj[k])*fa + pim + T9[k];
         }
     }
-    double g0 = GK*x0, g1 = GK*x1, g2 = GK*x2;
-    double t0 = g0*Mf[0] + g1*Mf[3] + g2*Mf[6];
-    double t1 = g0*Mf[1] + g1*Mf[4] + g2*Mf[7];
-    double t2 = g0*Mf[2] + g1*Mf[5] + g2*Mf[8];
-    if (P.dim == 2) t2 = 0.0;
-    double temp1 = u0*g0 + u1*g1 + u2*g2;
+    T g0 = GK*x0, g1 = GK*x1, g2 = GK*x2;
+    T t0 = g0*Mf[0] + g1*Mf[3] + g2*Mf[6];
+    T t1 = g0*Mf[1] + g1*Mf[4] + g2*Mf[7];
+    T t2 = g0*Mf[2] + g1*Mf[5] + g2*Mf[8];
+    if (P.dim == 2) t2 = (T)0.0;
+    T temp1 = u0*g0 + u1*g1 + u2*g2;
     A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
-    A.dd += mj * (di/dj) * temp1;
-    double fq = mj/dj;
+    if constexpr (sizeof(T) == 4) {
+        T c = mj * (di/dj) * temp1;
+        T y = c - A.ddc;
+        T tt = A.dd + y;
+        A.ddc = (tt - A.dd) - y;
+        A.dd = tt;
+    } else {
+        A.dd += mj * (di/dj) * temp1;
+    }
+    T fq = mj/dj;
     A.zw += mj/dj* K;
     A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
     A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
@@ -878,7 +902,8 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
     if (shep_on) A.sd += mj* K;
 }
 
-template <bool SOLID>
+// T = float (SOLID only): the FP32-mixed soil tier; gravity is added in FP64 afterwards.
+template <bool SOLID, class T>
 __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -901,8 +926,10 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
     decode_key(P, D.key[i], kx, ky, kz);
     int shep_on = P.any_shepard && cp[ci].Shepard && (D.ctl->shep_counter == cp[ci].ShepardStep);
     int zw_on = shep_on;
-    Acc A;
-    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
+    constexpr bool F32 = sizeof(T) == 4;
+    AccT<T> A;
+    A.a0 = F32 ? (T)0 : (T)P.gravity[0]; A.a1 = F32 ? (T)0 : (T)P.gravity[1]; A.a2 = F32 ? (T)0 : (T)P.gravity[2];
+    A.dd = 0.0; A.ddc = 0.0;
     A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0;
     for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
     A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
@@ -937,7 +964,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
             if (wave_any(cnt == kCap)) {
                 for (int e = 0; e < cnt; ++e) {
                     int jj = lst[e*kForceBlock + tid];
-                    if (SOLID) pair2233(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+                    if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
                     else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
                 }
                 cnt = 0;
@@ -946,17 +973,22 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
     }
     for (int e = 0; e < cnt; ++e) {
         int jj = lst[e*kForceBlock + tid];
-        if (SOLID) pair2233(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+        if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
         else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
     }
     if (act) {
-        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        double a0 = A.a0, a1 = A.a1, a2 = A.a2, dd = A.dd;
+        if (F32) {
+            a0 = P.gravity[0] + (double)A.a0; a1 = P.gravity[1] + (double)A.a1; a2 = P.gravity[2] + (double)A.a2;
+            dd = (double)A.dd + (double)(-A.ddc);
+        }
+        D.acc[i] = make_double4(a0, a1, a2, dd);
         if (P.any_xsph || zw_on) D.xsp[i] = make_double4(A.vx0, A.vx1, A.vx2, A.zw);
         if (shep_on) D.shep[i] = make_double2(A.sd, 0.0);
         if (SOLID || P.any_sr) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
         if (SOLID) for (int k = 0; k < 3; ++k) D.rr[3*i+k] = A.rr[k];
         // dt control: min over free particles of sqrt_h_a*sqrt(h/|a|) (Domain.cpp:1281-1296)
-        double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double test = sqrt(cp[ci].h/na);
         double cand = P.sqrt_h_a*test;
         dtb = dbits(cand);
@@ -1787,9 +1819,9 @@ void launch_wall(const Dev &D, const Params &P, hipStream_t s)
     if (P.nfixed <= 0) return;
     if (P.fast11 || P.mixed32) {
         if (P.dim == 3)
-            hipLaunchKernelGGL(k_wall_fast<3>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+            hipLaunchKernelGGL(k_wall_fast<3>, dim3(nblk(4*P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
         else
-            hipLaunchKernelGGL(k_wall_fast<2>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+            hipLaunchKernelGGL(k_wall_fast<2>, dim3(nblk(4*P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
         return;
     }
     if (P.any_solid)
@@ -1832,10 +1864,12 @@ static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
         }
         return;
     }
-    if (P.any_solid)
-        hipLaunchKernelGGL(k_force<true>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+    if (P.any_solid && P.soil32)
+        hipLaunchKernelGGL((k_force<true, float>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+    else if (P.any_solid)
+        hipLaunchKernelGGL((k_force<true, double>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
     else
-        hipLaunchKernelGGL(k_force<false>, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        hipLaunchKernelGGL((k_force<false, double>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
 }
 
 void launch_integrate(const Dev &D, const Params &P, hipStream_t s)

@@ -192,7 +192,8 @@ __device__ inline void viscous_force(int viseq, T &v0, T &v1, T &v2, T Mu, T di,
 }
 
 // Symmetric 3x3 tensors stored as 6 components (xx, xy, xz, yy, yz, zz).
-__device__ inline void sym_to_full(const double s[6], double M[9])
+template <class T>
+__device__ inline void sym_to_full(const T s[6], T M[9])
 {
     M[0] = s[0]; M[1] = s[1]; M[2] = s[2];
     M[3] = s[1]; M[4] = s[3]; M[5] = s[4];

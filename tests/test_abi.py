@@ -13,11 +13,13 @@ from persiansph_amd import solver as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "persiansph_amd.h")
+GNN_HEADER = os.path.join(ROOT, "include", "psph_gnn.h")
 
 
 def _c_offsets(tmp_path, struct, fields):
     src = tmp_path / "off.c"
-    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void){']
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', f'#include "{GNN_HEADER}"',
+             'int main(void){']
     lines.append(f'printf("sizeof %zu\\n", sizeof({struct}));')
     for f in fields:
         lines.append(f'printf("{f} %zu\\n", offsetof({struct}, {f}));')
@@ -47,9 +49,9 @@ def test_small_struct_layouts(tmp_path, cls, struct):
         assert off[n] == getattr(cls, n).offset, n
 
 
-def _declared():
-    txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^int (psph_\w+)\(", txt, re.M)))
+def _declared(header=HEADER):
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^(?:int|int64_t) (psph_\w+)\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -89,3 +91,33 @@ def test_particle_defaults_match_constructor():
     assert np.all(p["TIn"] == 4.0) and np.all(p["FPMassC"] == 1.0) and np.all(p["m"] == 300.0)
     assert np.all(p["ShepardStep"] == 40) and np.all(p["FirstStep"] == 1) and np.all(p["IsFree"] == 1)
     assert np.all(p["CSmag"] == 0.17) and np.all(p["Material"] == 0)
+
+
+def test_surrogate_config_layout(tmp_path):
+    from persiansph_amd.gnn import GnnConfig
+    names = [f[0] for f in GnnConfig._fields_]
+    off = _c_offsets(tmp_path, "psph_gnn_config", names)
+    assert off["sizeof"] == ctypes.sizeof(GnnConfig)
+    for n in names:
+        assert off[n] == getattr(GnnConfig, n).offset, n
+
+
+def test_library_exports_every_surrogate_symbol():
+    from persiansph_amd.gnn import GNN_EXPORTS
+    L = ctypes.CDLL(S.LIB_PATH)
+    decl = _declared(GNN_HEADER)
+    assert len(decl) == 14
+    for name in decl:
+        assert hasattr(L, name), name
+    assert sorted(GNN_EXPORTS) == decl
+
+
+def test_surrogate_reports_missing_device():
+    """Without a visible GPU the surrogate's create fails with the no-device code."""
+    from persiansph_amd.gnn import Network
+    try:
+        Network(2, 1, 0.1).close()
+    except S.SolverError as e:
+        assert e.code == -6, e
+    else:
+        pytest.skip("a GPU is visible")
