@@ -8,11 +8,12 @@
  * trajectories of the plain solver by hand-written HIP forward, backward and Adam kernels
  * (BF16 weights and activations, FP32 accumulation on the matrix cores, FP32 master weights).
  *
- * Network: encoder MLPs on node inputs (velocity, density deviation, free flag) and edge
- * inputs (relative position and distance over the radius); `layers` message-passing layers
- * (edge update e += MLP([e, h_recv, h_send]), sum of incoming edges, node update
- * h += MLP([h, sum])); decoder MLP to the normalised change of position, velocity and density
- * (2*dim + 1 outputs).  Every MLP is Linear(k, 128) -> ReLU -> Linear(128, 128).
+ * Network: encoder MLPs on node inputs (velocity, density deviation, free flag, neighbour
+ * count) and edge inputs (relative position and distance over the radius); `layers`
+ * message-passing layers (edge update e += MLP([e, h_recv, h_send]), mean of incoming edges,
+ * node update h += MLP([h, mean])); decoder MLP to the normalised change of position (net of
+ * the drift), velocity and density (2*dim + 1 outputs).  Every MLP is
+ * Linear(k, 128) -> ReLU -> Linear(128, 128).
  *
  * Status codes are those of persiansph_amd.h (message via psph_last_error).
  */
@@ -35,9 +36,11 @@ typedef struct psph_gnn_config {
     double rho0;            /* reference density of the inputs */
     double vel_scale;       /* node input v / vel_scale */
     double rho_scale;       /* node input (rho/rho0 - 1) / rho_scale */
-    double out_dx;          /* output scales: dx = y[0:dim] * out_dx */
+    double out_dx;          /* output scales: dx = drift_t * v + y[0:dim] * out_dx */
     double out_dv;          /*                dv = y[dim:2dim] * out_dv */
     double out_drho;        /*                drho = y[2dim] * out_drho */
+    double drift_t;         /* time the network advances (k solver steps); 0: no drift term */
+    double deg_scale;       /* node input: neighbour count / deg_scale (0: 1) */
     uint64_t seed;          /* weight initialisation */
 } psph_gnn_config;
 
@@ -75,8 +78,8 @@ int psph_gnn_predict(psph_gnn *g, float *y);
    state is written to x, v, rho (n x 3, n x 3, n) when they are not NULL. */
 int psph_gnn_advance(psph_gnn *g, double *x, double *v, double *rho);
 
-/* Training targets: the normalised change from the loaded state to (x1, v1, rho1), free
-   particles weighted 1 and fixed ones 0 ... */
+/* Training targets: the normalised change from the loaded state to (x1, v1, rho1), position
+   net of the drift drift_t * v, free particles weighted 1 and fixed ones 0 ... */
 int psph_gnn_set_target_state(psph_gnn *g, const double *x1, const double *v1, const double *rho1);
 /* ... or given directly (tgt: n x (2*dim+1), wgt: n). */
 int psph_gnn_set_targets(psph_gnn *g, const float *tgt, const float *wgt);

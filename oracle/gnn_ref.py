@@ -42,10 +42,14 @@ def num_params(layers):
     return sum(k*H + H for k in layer_inputs(layers))
 
 
-def segment_sum(E, recv, n):
+def degrees(recv, n):
+    return np.maximum(np.bincount(recv, minlength=n), 1).astype(np.float64)
+
+
+def segment_mean(E, recv, n):
     A = np.zeros((n, E.shape[1]))
     np.add.at(A, recv, E)
-    return A
+    return A/degrees(recv, n)[:, None]
 
 
 def forward(p, layers, X, U, recv, send, dout, keep=False):
@@ -65,7 +69,7 @@ def forward(p, layers, X, U, recv, send, dout, keep=False):
         ce = np.concatenate([e, h[recv], h[send]], axis=1)
         ze = relu(ce @ We1 + be1)
         e = e + ze @ We2 + be2
-        a = segment_sum(e, recv, n)
+        a = segment_mean(e, recv, n)
         cn = np.concatenate([h, a], axis=1)
         zn = relu(cn @ Wn1 + bn1)
         h = h + zn @ Wn2 + bn2
@@ -113,7 +117,7 @@ def gradient(p, layers, X, U, recv, send, tgt, wgt):
         dzn = lin_back(in2, c["ZN"][l], dh)*(c["ZN"][l] > 0)
         dcn = lin_back(in1, c["CN"][l], dzn)
         dh_next = dh + dcn[:, :H]
-        da = dcn[:, H:]
+        da = dcn[:, H:]/degrees(recv, n)[:, None]
         det = de + da[recv]
         dze = lin_back(ie2, c["ZE"][l], det)*(c["ZE"][l] > 0)
         dce = lin_back(ie1, c["CE"][l], dze)

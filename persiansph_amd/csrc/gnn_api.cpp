@@ -200,6 +200,8 @@ int psph_gnn_create(const psph_gnn_config *cfg, psph_gnn **out)
     g->fs.out_dx = cfg->out_dx;
     g->fs.out_dv = cfg->out_dv;
     g->fs.out_drho = cfg->out_drho;
+    g->fs.drift_t = cfg->drift_t;
+    g->fs.inv_deg = cfg->deg_scale > 0.0 ? 1.0/cfg->deg_scale : 1.0;
     auto add = [&](int kin) {
         Lin l;
         l.kin = kin;
@@ -212,16 +214,18 @@ int psph_gnn_create(const psph_gnn_config *cfg, psph_gnn **out)
     add(32); add(128); add(32); add(128);
     for (int l = 0; l < g->L; ++l) { add(384); add(128); add(256); add(128); }
     add(128); add(128);
-    // Glorot-uniform weights over the layer's real fan-in, zero biases; the decoder's output
-    // layer starts at 1/10 of that scale; weights of padded inputs are zero
+    // Glorot-uniform weights over the layer's real fan-in, zero biases; the output layers of
+    // the residual updates start at 0.3 and the decoder's output layer at 0.1 of that scale,
+    // which keeps the latents' magnitude flat through the layers; padded inputs get zeros
     std::vector<float> p(g->nparam, 0.0f);
     Rng rng{cfg->seed ? cfg->seed : 1};
     for (size_t k = 0; k < g->lin.size(); ++k) {
         const Lin &l = g->lin[k];
         int fan_in = l.kin;
-        if (k == 0) fan_in = cfg->dim + 2;
+        if (k == 0) fan_in = cfg->dim + 3;
         if (k == 2) fan_in = cfg->dim + 1;
         double a = std::sqrt(6.0/(fan_in + 128.0));
+        if (k >= 4 && k < 4 + 4*(size_t)g->L && (k - 4) % 2 == 1) a *= 0.3;
         if (k + 1 == g->lin.size()) a *= 0.1;
         for (int i = 0; i < fan_in; ++i)
             for (int j = 0; j < 128; ++j) {
@@ -367,7 +371,7 @@ int psph_gnn_load_state(psph_gnn *g, const double *x, const double *v, const dou
     for (int64_t i = 0; i < n; ++i) g->wsum += isfree[i] ? 1.0 : 0.0;
     rc = build_graph(g, x);
     if (rc) return rc;
-    node_in(g->s, (int)n, g->dim, g->v, g->rho, g->isfree, g->fs, g->Xn);
+    node_in(g->s, (int)n, g->dim, g->v, g->rho, g->isfree, g->rp, g->fs, g->Xn);
     if (g->ne > 0) edge_in(g->s, (int)g->ne, g->dim, g->recv, g->send, g->x, 1.0/g->cfg.radius, g->Ue);
     HC(hipGetLastError());
     g->have_state = true;
@@ -539,7 +543,7 @@ static int backward(psph_gnn *g)
         gemm_bwd_input(s, n, 256, g->dZ, g->W(g->pn1(l)), g->dInN);
         if (ne > 0) {
             // aggregation: the gradient of each e' gains that of its receiver's sum
-            edge_grad(s, ne, g->recv, de_next, g->dInN + 128, 256, g->dEt, g->dEtb);
+            edge_grad(s, ne, g->rp, g->recv, de_next, g->dInN + 128, 256, g->dEt, g->dEtb);
             // edge update e' = e + MLP([e, h_recv, h_send])
             gemm_wgrad_rows(s, ne, 128, g->ZE[l], 128, g->dEtb, g->dW(g->pe2(l)), g->db(g->pe2(l)));
             gemm_bwd_mask(s, ne, g->dEtb, g->W(g->pe2(l)), g->ZE[l], g->dZ);

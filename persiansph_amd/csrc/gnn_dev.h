@@ -18,7 +18,9 @@ struct GraphGrid {
 // input normalisation and output scales of the network
 struct FeatScale {
     double inv_v, rho0, inv_rho;       // node inputs: v * inv_v, (rho/rho0 - 1) * inv_rho
-    double out_dx, out_dv, out_drho;   // outputs: change = y * out_*
+    double out_dx, out_dv, out_drho;   // outputs: change = y * out_* (position: plus drift_t v)
+    double drift_t;
+    double inv_deg;                    // node input: neighbour count * inv_deg
 };
 
 void gemm_fwd_rows(hipStream_t s, int M, int K, const uint16_t *A, int lda, const uint16_t *Wt, const float *b,
@@ -38,8 +40,8 @@ void gemm_wgrad_edgecat(hipStream_t s, int M, const uint16_t *E, const uint16_t 
 void gemm_wgrad_nodecat(hipStream_t s, int M, const uint16_t *H, const uint16_t *A, const uint16_t *dY, float *dW,
                         float *db);
 void segsum(hipStream_t s, int n, const int32_t *rp, const uint16_t *E, uint16_t *a);
-void edge_grad(hipStream_t s, int ne, const int32_t *recv, const float *gnext, const float *da, int lda, float *g,
-               uint16_t *gb);
+void edge_grad(hipStream_t s, int ne, const int32_t *rp, const int32_t *recv, const float *gnext, const float *da,
+               int lda, float *g, uint16_t *gb);
 void node_grad(hipStream_t s, int n, const int32_t *rp, const int32_t *rev, const float *dh_next, const float *dNode,
                const float *dEdge, float *dh, uint16_t *dhb);
 void add_cols(hipStream_t s, int M, int ld, int c0, const float *src, const float *base, float *g, uint16_t *gb);
@@ -56,7 +58,7 @@ void graph_edges(hipStream_t s, int pass, int n, const GraphGrid &g, const doubl
 void graph_finish(hipStream_t s, int n, const int32_t *rp, const int32_t *cols, int32_t *recv, int32_t *send, int32_t *rev);
 void scan1(hipStream_t s, int n, const int32_t *in, int32_t *out);
 void node_in(hipStream_t s, int n, int dim, const double *v, const double *rho, const int32_t *isfree,
-             const FeatScale &fs, uint16_t *X);
+             const int32_t *rp, const FeatScale &fs, uint16_t *X);
 void edge_in(hipStream_t s, int ne, int dim, const int32_t *recv, const int32_t *send, const double *x, double inv_r,
              uint16_t *U);
 void apply_out(hipStream_t s, int n, int dim, const float *y, const int32_t *isfree, const FeatScale &fs,

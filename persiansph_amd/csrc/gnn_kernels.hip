@@ -5,7 +5,7 @@
 // Network (hidden width 128, L message-passing layers):
 //   encoder   h0 = MLP_n(x_i)                e0 = MLP_e(u_ij)
 //   layer l   e' = e + MLP([e, h_recv, h_send])      (edge update, residual)
-//             a_i = sum over edges into i of e'       (aggregation, CSR by receiver)
+//             a_i = mean over edges into i of e'      (aggregation, CSR by receiver)
 //             h' = h + MLP([h, a])                    (node update, residual)
 //   decoder   y = MLP_d(h_L)                          (per-particle state change)
 // every MLP is Linear -> ReLU -> Linear.  The graph is symmetric, so the edges leaving node j
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) k_colsum(int M, const uint16_t *dY, float
     if (half == 0) atomicAdd(&db[c], sh[c] + sh[c + 128]);
 }
 
-// a[i] = sum of E over the CSR segment of i (bf16 out, FP32 sums); one wave per node
+// a[i] = mean of E over the CSR segment of i (bf16 out, FP32 sums); one wave per node
 __global__ void __launch_bounds__(256) k_segsum(int n, const int32_t *rp, const uint16_t *E, uint16_t *a)
 {
     int i = blockIdx.x*4 + (threadIdx.x >> 6);
@@ -256,20 +256,27 @@ __global__ void __launch_bounds__(256) k_segsum(int n, const int32_t *rp, const 
         s0 += bf2f((uint16_t)(v & 0xffffu));
         s1 += bf2f((uint16_t)(v >> 16));
     }
+    int deg = rp[i + 1] - rp[i];
+    float inv = 1.0f/(float)(deg > 0 ? deg : 1);
+    s0 *= inv;
+    s1 *= inv;
     uint32_t o = (uint32_t)f2bf(s0) | ((uint32_t)f2bf(s1) << 16);
     *(uint32_t *)(a + (size_t)i*128 + 2*lane) = o;
 }
 
-// gradient into an edge latent: g[e] = gnext[e] + da[recv[e]] (FP32, da with leading
-// dimension lda), also as bf16
-__global__ void __launch_bounds__(256) k_edge_grad(int ne, const int32_t *recv, const float *gnext,
-                                                   const float *da, int lda, float *g, uint16_t *gb)
+// gradient into an edge latent: g[e] = gnext[e] + da[recv[e]] / deg(recv[e]) (the mean's
+// share; FP32, da with leading dimension lda), also as bf16
+__global__ void __launch_bounds__(256) k_edge_grad(int ne, const int32_t *rp, const int32_t *recv,
+                                                   const float *gnext, const float *da, int lda, float *g,
+                                                   uint16_t *gb)
 {
     size_t t = (size_t)blockIdx.x*256 + threadIdx.x;
     if (t >= (size_t)ne*128) return;
     size_t e = t >> 7;
     int c = (int)(t & 127);
-    float v = (gnext ? gnext[t] : 0.0f) + da[(size_t)recv[e]*lda + c];
+    int r = recv[e];
+    int deg = rp[r + 1] - rp[r];
+    float v = (gnext ? gnext[t] : 0.0f) + da[(size_t)r*lda + c]/(float)(deg > 0 ? deg : 1);
     g[t] = v;
     gb[t] = f2bf(v);
 }
@@ -503,9 +510,10 @@ __global__ void __launch_bounds__(1024) k_scan1(int n, const int32_t *in, int32_
 
 // ---- features and state update ------------------------------------------------------------
 // node inputs (padded to 32 columns): velocity / vs (dim), (rho/rho0 - 1) / ds, 1 for free
-// particles, then zeros
+// particles, neighbour count / expected count, then zeros
 __global__ void __launch_bounds__(256) k_node_in(int n, int dim, const double *v, const double *rho,
-                                                 const int32_t *isfree, FeatScale fs, uint16_t *X)
+                                                 const int32_t *isfree, const int32_t *rp, FeatScale fs,
+                                                 uint16_t *X)
 {
     int t = blockIdx.x*256 + threadIdx.x;
     if (t >= n*32) return;
@@ -514,6 +522,7 @@ __global__ void __launch_bounds__(256) k_node_in(int n, int dim, const double *v
     if (c < dim) f = (float)(v[3*i + c]*fs.inv_v);
     else if (c == dim) f = (float)((rho[i]/fs.rho0 - 1.0)*fs.inv_rho);
     else if (c == dim + 1) f = isfree[i] ? 1.0f : 0.0f;
+    else if (c == dim + 2) f = (float)((rp[i + 1] - rp[i])*fs.inv_deg);
     X[t] = f2bf(f);
 }
 
@@ -537,7 +546,8 @@ __global__ void __launch_bounds__(256) k_edge_in(int ne, int dim, const int32_t 
 }
 
 // state after k solver steps: x += dx, v += dv, rho += drho for free particles, from the
-// network output y (normalised: dx = y[0:dim] sx, dv = y[dim:2dim] sv, drho = y[2dim] sr)
+// network output y (normalised: dx = drift_t v + y[0:dim] sx, dv = y[dim:2dim] sv,
+// drho = y[2dim] sr)
 __global__ void __launch_bounds__(256) k_apply(int n, int dim, const float *y, const int32_t *isfree,
                                                FeatScale fs, double *x, double *v, double *rho)
 {
@@ -545,13 +555,15 @@ __global__ void __launch_bounds__(256) k_apply(int n, int dim, const float *y, c
     if (i >= n || !isfree[i]) return;
     const float *yi = y + (size_t)i*(2*dim + 1);
     for (int d = 0; d < dim; ++d) {
-        x[3*i + d] += (double)yi[d]*fs.out_dx;
-        v[3*i + d] += (double)yi[dim + d]*fs.out_dv;
+        double v0 = v[3*i + d];
+        x[3*i + d] += fs.drift_t*v0 + (double)yi[d]*fs.out_dx;
+        v[3*i + d] = v0 + (double)yi[dim + d]*fs.out_dv;
     }
     rho[i] += (double)yi[2*dim]*fs.out_drho;
 }
 
-// targets for training: the normalised change between two states (free particles weighted 1)
+// targets for training: the normalised change between two states, position net of the drift
+// drift_t v0 (free particles weighted 1)
 __global__ void __launch_bounds__(256) k_targets(int n, int dim, const double *x0, const double *v0,
                                                  const double *r0, const double *x1, const double *v1,
                                                  const double *r1, const int32_t *isfree, FeatScale fs,
@@ -561,7 +573,7 @@ __global__ void __launch_bounds__(256) k_targets(int n, int dim, const double *x
     if (i >= n) return;
     float *t = tgt + (size_t)i*(2*dim + 1);
     for (int d = 0; d < dim; ++d) {
-        t[d] = (float)((x1[3*i + d] - x0[3*i + d])/fs.out_dx);
+        t[d] = (float)((x1[3*i + d] - x0[3*i + d] - fs.drift_t*v0[3*i + d])/fs.out_dx);
         t[dim + d] = (float)((v1[3*i + d] - v0[3*i + d])/fs.out_dv);
     }
     t[2*dim] = (float)((r1[i] - r0[i])/fs.out_drho);
@@ -637,9 +649,9 @@ void gemm_wgrad_nodecat(hipStream_t s, int M, const uint16_t *H, const uint16_t 
 
 void segsum(hipStream_t s, int n, const int32_t *rp, const uint16_t *E, uint16_t *a)
 { hipLaunchKernelGGL(k_segsum, dim3(nb(n, 4)), dim3(256), 0, s, n, rp, E, a); }
-void edge_grad(hipStream_t s, int ne, const int32_t *recv, const float *gnext, const float *da, int lda, float *g,
-               uint16_t *gb)
-{ hipLaunchKernelGGL(k_edge_grad, dim3(nb((size_t)ne*128, 256)), dim3(256), 0, s, ne, recv, gnext, da, lda, g, gb); }
+void edge_grad(hipStream_t s, int ne, const int32_t *rp, const int32_t *recv, const float *gnext, const float *da,
+               int lda, float *g, uint16_t *gb)
+{ hipLaunchKernelGGL(k_edge_grad, dim3(nb((size_t)ne*128, 256)), dim3(256), 0, s, ne, rp, recv, gnext, da, lda, g, gb); }
 void node_grad(hipStream_t s, int n, const int32_t *rp, const int32_t *rev, const float *dh_next, const float *dNode,
                const float *dEdge, float *dh, uint16_t *dhb)
 { hipLaunchKernelGGL(k_node_grad, dim3(nb(n, 4)), dim3(256), 0, s, n, rp, rev, dh_next, dNode, dEdge, dh, dhb); }
@@ -674,8 +686,8 @@ void scan1(hipStream_t s, int n, const int32_t *in, int32_t *out)
 { hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, n, in, out); }
 
 void node_in(hipStream_t s, int n, int dim, const double *v, const double *rho, const int32_t *isfree,
-             const FeatScale &fs, uint16_t *X)
-{ hipLaunchKernelGGL(k_node_in, dim3(nb((size_t)n*32, 256)), dim3(256), 0, s, n, dim, v, rho, isfree, fs, X); }
+             const int32_t *rp, const FeatScale &fs, uint16_t *X)
+{ hipLaunchKernelGGL(k_node_in, dim3(nb((size_t)n*32, 256)), dim3(256), 0, s, n, dim, v, rho, isfree, rp, fs, X); }
 void edge_in(hipStream_t s, int ne, int dim, const int32_t *recv, const int32_t *send, const double *x, double inv_r,
              uint16_t *U)
 { hipLaunchKernelGGL(k_edge_in, dim3(nb((size_t)ne*32, 256)), dim3(256), 0, s, ne, dim, recv, send, x, inv_r, U); }

@@ -27,7 +27,8 @@ class GnnConfig(ctypes.Structure):
         ("dim", ctypes.c_int32), ("layers", ctypes.c_int32), ("hidden", ctypes.c_int32),
         ("train", ctypes.c_int32), ("radius", ctypes.c_double), ("rho0", ctypes.c_double),
         ("vel_scale", ctypes.c_double), ("rho_scale", ctypes.c_double), ("out_dx", ctypes.c_double),
-        ("out_dv", ctypes.c_double), ("out_drho", ctypes.c_double), ("seed", ctypes.c_uint64),
+        ("out_dv", ctypes.c_double), ("out_drho", ctypes.c_double), ("drift_t", ctypes.c_double),
+        ("deg_scale", ctypes.c_double), ("seed", ctypes.c_uint64),
     ]
 
 
@@ -76,10 +77,10 @@ def layer_shapes(layers):
 
 class Network:
     def __init__(self, dim, layers, radius, rho0=1000.0, vel_scale=1.0, rho_scale=1e-3,
-                 out_dx=1e-3, out_dv=1e-2, out_drho=1.0, train=True, seed=1):
+                 out_dx=1e-3, out_dv=1e-2, out_drho=1.0, drift_t=0.0, deg_scale=1.0, train=True, seed=1):
         cfg = GnnConfig(dim=dim, layers=layers, hidden=HIDDEN, train=1 if train else 0, radius=radius,
                         rho0=rho0, vel_scale=vel_scale, rho_scale=rho_scale, out_dx=out_dx,
-                        out_dv=out_dv, out_drho=out_drho, seed=seed)
+                        out_dv=out_dv, out_drho=out_drho, drift_t=drift_t, deg_scale=deg_scale, seed=seed)
         h = ctypes.c_void_p()
         _check(_lib().psph_gnn_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -4,16 +4,17 @@ plain solver, with the plain solver as its fallback and judge.
 
 Pieces:
   dam_break_variant     seeded 2-D dam-break inputs with a given column aspect ratio
-  trajectory            plain-solver frames every k steps
+  trajectory            plain-solver frames every T = k dt0 of simulated time
   scales_from           input / output normalisation from trajectories
   train                 Adam training on (frame, frame + k steps) pairs
   predictors            `net_predictor` and the two predictors without a network: ballistic
                         (x += k dt v + g (k dt)^2 / 2, v += g k dt) and drift (x += k dt v)
   prediction_error      k-step error of a predictor on held-out frames
-  HybridEngine          windows of M steps advanced by the network (M/k calls), each judged:
-                        the plain solver re-runs k steps from the window's start and the
-                        network's first call must agree with it (front position within
-                        `front_tol` of the travelled distance), and the window's end state
+  HybridEngine          windows of M dt0 advanced by the network (M/k calls of T = k dt0),
+                        each judged: the plain solver re-runs T from the window's start and
+                        the network's first call must agree with it (front position within
+                        `front_tol` of the travelled distance, and position error at most
+                        `gain_tol` times that of drift stepping), and the window's end state
                         must keep the mean density drift and the gain of kinetic +
                         gravitational energy below thresholds; otherwise the window is
                         reverted and advanced by the plain solver.
@@ -25,7 +26,7 @@ import numpy as np
 
 from . import analysis, cases
 from .gnn import Network
-from .solver import Solver
+from .solver import Solver, SolverError
 
 G = 9.81
 C0 = 10.0*math.sqrt(2.0*G*2.0)
@@ -51,12 +52,15 @@ class Frame:
                    np.array(q["IsFree"], dtype=np.int32), t)
 
 
-def trajectory(cfg, p, dt, steps, k):
-    """Frames of the plain solver every k steps (steps // k + 1 frames)."""
+def trajectory(cfg, p, dt, t_end, T):
+    """Frames of the plain solver at times T, 2T, ... <= t_end (each taken at the first force
+    pass with Time >= the sample time, as an output time of Solve; the solver's adaptive dt
+    makes the step count per frame vary)."""
     s = Solver(cfg, p, dt)
     frames = [Frame.from_particles(s.particles(), 0.0)]
-    for _ in range(steps//k):
-        st = s.run(k)
+    n_frames = int(round(t_end/T))
+    for m in range(1, n_frames + 1):
+        st = s.run(10**9, tout=m*T)
         q = s.particles()
         if len(q) != len(frames[0].x):
             break                          # particles left the domain: end the trajectory
@@ -65,29 +69,25 @@ def trajectory(cfg, p, dt, steps, k):
     return frames
 
 
-def scales_from(trajs, k, radius, rho0=1000.0):
-    """Normalisation: velocity scale and density-deviation scale of the inputs, and the
-    standard deviation of each output (change over k steps) of the free particles."""
-    dxs, dvs, drs, vs, rs = [], [], [], [], []
-    for fr in trajs:
-        for a, b in zip(fr[:-1], fr[1:]):
-            f = a.free == 1
-            dxs.append((b.x - a.x)[f, :2].ravel())
-            dvs.append((b.v - a.v)[f, :2].ravel())
-            drs.append((b.rho - a.rho)[f])
-            vs.append(a.v[f, :2].ravel())
-            rs.append((a.rho[f]/rho0 - 1.0))
-    cat = np.concatenate
-    pos = lambda x, floor: max(float(np.std(x)), floor)
-    return dict(radius=radius, rho0=rho0, vel_scale=pos(cat(vs), 1e-3), rho_scale=pos(cat(rs), 1e-6),
-                out_dx=pos(cat(dxs), 1e-9), out_dv=pos(cat(dvs), 1e-9), out_drho=pos(cat(drs), 1e-9))
+def scales_from(trajs, k, dt, radius, rho0=1000.0, g=G, height=2.0):
+    """Normalisation.  Inputs: velocity over the free-fall speed sqrt(2 g height), density
+    deviation over 1 %.  Outputs over T = k dt: position change net of the drift T v over
+    g T^2 / 2, velocity change over g T, density change over its RMS in the data (free
+    particles).  Physical scales keep fast and resting particles on one footing."""
+    T = k*dt
+    drs = [(b.rho - a.rho)[a.free == 1] for fr in trajs for a, b in zip(fr[:-1], fr[1:])]
+    rms = float(np.sqrt(np.mean(np.concatenate(drs)**2))) if drs else 0.0
+    return dict(radius=radius, rho0=rho0, vel_scale=math.sqrt(2.0*g*height), rho_scale=0.01,
+                out_dx=0.5*g*T*T, out_dv=g*T, out_drho=max(rms, 1e-6*rho0), drift_t=T,
+                deg_scale=float(analysis.bulk_neighbours(2)))
 
 
 def make_network(dim, layers, scales, train=True, seed=1):
     s = scales
     return Network(dim, layers, s["radius"], rho0=s["rho0"], vel_scale=s["vel_scale"],
                    rho_scale=s["rho_scale"], out_dx=s["out_dx"], out_dv=s["out_dv"],
-                   out_drho=s["out_drho"], train=train, seed=seed)
+                   out_drho=s["out_drho"], drift_t=s.get("drift_t", 0.0), deg_scale=s.get("deg_scale", 1.0),
+                   train=train, seed=seed)
 
 
 def train(net, trajs, steps, lr=1e-4, seed=0, log=None, log_every=50):
@@ -146,7 +146,7 @@ def prediction_error(trajs, predict, scales):
         for a, b in zip(fr[:-1], fr[1:]):
             x, v, r = predict(a)
             f = a.free == 1
-            ex = (x - b.x)[f, :2]/scales["out_dx"]
+            ex = (x - b.x)[f, :2]/scales["out_dx"]     # the drift is exact, so this is net of it
             ev = (v - b.v)[f, :2]/scales["out_dv"]
             er = (r - b.rho)[f]/scales["out_drho"]
             errs.append(math.sqrt((np.sum(ex*ex) + np.sum(ev*ev) + np.sum(er*er))/(5.0*f.sum())))
@@ -165,11 +165,13 @@ class HybridEngine:
     """Advances a run in windows of M steps; see the module docstring for the judge."""
 
     def __init__(self, cfg, p, dt, predictor, k=10, M=50, front_tol=0.01, rho_tol=0.02,
-                 energy_tol=0.01, dim=2):
+                 energy_tol=0.01, gain_tol=0.8, dim=2):
         self.cfg, self.p0, self.dt = cfg, p, dt
         self.predict, self.k, self.M = predictor, k, M
         self.front_tol, self.rho_tol, self.energy_tol = front_tol, rho_tol, energy_tol
+        self.gain_tol = gain_tol
         self.dim = dim
+        self.T = k*dt                      # simulated time of one network call
         self.mass = np.array(p["Mass"])
         self.h = float(p["h"][0])
         self.stats = dict(windows=0, accepted=0, reverted=0, net_calls=0, solver_steps=0,
@@ -181,22 +183,29 @@ class HybridEngine:
         q["va"], q["vb"], q["Densitya"], q["Densityb"] = fr.v, fr.v, fr.rho, fr.rho
         return Solver(self.cfg, q, self.dt, time=fr.t)
 
-    def _plain(self, fr, steps):
+    def _plain(self, fr, duration):
+        """The plain solver from fr for `duration` seconds of simulated time."""
         t0 = time.perf_counter()
         s = self._solver(fr)
-        st = s.run(steps)
+        st = s.run(10**9, tf=fr.t + duration)
         out = Frame.from_particles(s.particles(), st.time)
         s.close()
-        self.stats["solver_steps"] += steps
+        self.stats["solver_steps"] += st.steps
         self.stats["t_solver"] += time.perf_counter() - t0
         return out
 
     def _net(self, fr):
+        """One network call, or None when the prediction fails or is not finite."""
         t0 = time.perf_counter()
-        x, v, r = self.predict(fr)
+        try:
+            x, v, r = self.predict(fr)
+        except SolverError:
+            x = None
         self.stats["net_calls"] += 1
         self.stats["t_net"] += time.perf_counter() - t0
-        return Frame(x, v, r, fr.free, fr.t + self.k*self.dt)
+        if x is None or not (np.all(np.isfinite(x)) and np.all(np.isfinite(v)) and np.all(np.isfinite(r))):
+            return None
+        return Frame(x, v, r, fr.free, fr.t + self.T)
 
     def _front(self, fr):
         q = np.zeros(len(fr.x), dtype=[("x", "f8", 3), ("IsFree", "i4"), ("h", "f8")])
@@ -214,21 +223,29 @@ class HybridEngine:
             calls = []
             for _ in range(self.M//self.k):
                 s = self._net(s)
+                if s is None:
+                    break
                 calls.append(s)
-            judge = self._plain(fr, self.k)
-            fj, fn = self._front(judge), self._front(calls[0])
-            travel = max(fj - x0, 2.0*self.h)
-            f = s.free == 1
-            drift = float(np.mean(np.abs(s.rho[f]/1000.0 - 1.0)))
-            gain = (energy(s, self.mass) - e0)/max(abs(e0), 1e-30)
-            ok = (abs(fn - fj) <= self.front_tol*travel and drift <= self.rho_tol and gain <= self.energy_tol
-                  and np.all(np.isfinite(s.x)))
+            ok = s is not None
+            if ok:
+                judge = self._plain(fr, self.T)
+                fj, fn = self._front(judge), self._front(calls[0])
+                travel = max(fj - x0, 2.0*self.h)
+                f = s.free == 1
+                drift = float(np.mean(np.abs(s.rho[f]/1000.0 - 1.0)))
+                gain = (energy(s, self.mass) - e0)/max(abs(e0), 1e-30)
+                # the first call must also beat plain drift stepping against the solver
+                f0 = fr.free == 1
+                e_call = np.sqrt(np.mean(np.sum((calls[0].x - judge.x)[f0]**2, axis=1)))
+                e_drift = np.sqrt(np.mean(np.sum((fr.x + self.T*fr.v - judge.x)[f0]**2, axis=1)))
+                ok = (abs(fn - fj) <= self.front_tol*travel and drift <= self.rho_tol and gain <= self.energy_tol
+                      and e_call <= self.gain_tol*e_drift)
             if ok:
                 self.stats["accepted"] += 1
                 fr = s
             else:
                 self.stats["reverted"] += 1
-                fr = self._plain(fr, self.M)
+                fr = self._plain(fr, self.M*self.dt)
         return fr
 
 
@@ -248,11 +265,12 @@ def cmd_train(a):
     trajs = []
     for i, asp in enumerate(_variants(a.variants, a.seed)):
         cfg, p, dt = dam_break_variant(float(asp), dx=a.dx, area=a.area)
-        trajs.append(trajectory(cfg, p, dt, a.steps, a.k))
+        trajs.append(trajectory(cfg, p, dt, a.steps*dt, a.k*dt))
         _log(f"[data] variant {i} aspect {asp:.3f}: {len(p)} particles, {len(trajs[-1])} frames "
              f"({time.perf_counter() - t0:.1f}s)")
     radius = 2.0*1.3*a.dx
-    sc = scales_from(trajs, a.k, radius)
+    dt = dam_break_variant(1.0, dx=a.dx, area=a.area)[2]
+    sc = scales_from(trajs, a.k, dt, radius)
     net = make_network(2, a.layers, sc, seed=a.seed)
     t1 = time.perf_counter()
     losses = train(net, trajs, a.iters, lr=a.lr, seed=a.seed, log=_log, log_every=max(1, a.iters//20))
@@ -282,11 +300,12 @@ def cmd_eval(a):
     res = []
     for asp in _variants(a.variants, a.seed + 1000):
         cfg, p, dt = dam_break_variant(float(asp), dx=dx, area=meta["area"])
-        held = [trajectory(cfg, p, dt, a.steps, k)]
+        held = [trajectory(cfg, p, dt, a.steps*dt, k*dt)]
         row = dict(aspect=float(asp), particles=len(p),
                    err_network=prediction_error(held, net_predictor(net), meta["scales"]),
                    err_ballistic=prediction_error(held, ballistic_predictor(k, dt), meta["scales"]),
                    err_drift=prediction_error(held, drift_predictor(k, dt), meta["scales"]))
+        span = a.windows*a.M*dt              # simulated time of the comparison
         for name, pred in (("network", net_predictor(net)), ("ballistic", ballistic_predictor(k, dt))):
             eng = HybridEngine(cfg, p, dt, pred, k=k, M=a.M)
             t0 = time.perf_counter()
@@ -294,16 +313,16 @@ def cmd_eval(a):
             el = time.perf_counter() - t0
             st = dict(eng.stats)
             st["wall_s"] = el
-            st["particle_updates_per_s"] = len(p)*a.windows*a.M/el
+            st["simulated_s_per_wall_s"] = span/el
             row["engine_" + name] = st
-        # plain solver over the same steps
+        # plain solver over the same simulated time (adaptive dt: its own step count)
         s = Solver(cfg, p, dt)
-        s.run(1)
         t0 = time.perf_counter()
-        s.run(a.windows*a.M)
+        st = s.run(10**9, tf=span)
         el = time.perf_counter() - t0
         s.close()
-        row["plain_particle_updates_per_s"] = len(p)*a.windows*a.M/el
+        row["plain"] = dict(steps=st.steps, wall_s=el, simulated_s_per_wall_s=span/el,
+                            particle_updates_per_s=len(p)*st.steps/el)
         _log(json.dumps(row))
         res.append(row)
     out = dict(model=a.model, variants=res,
@@ -325,7 +344,9 @@ def cmd_bench3d(a):
     n = len(p)
     radius = 2.0*1.3*a.dx
     k = a.k
-    sc = dict(radius=radius, rho0=1000.0, vel_scale=1.0, rho_scale=1e-3, out_dx=1e-4, out_dv=1e-2, out_drho=1.0)
+    T = k*dt
+    sc = dict(radius=radius, rho0=1000.0, vel_scale=math.sqrt(2.0*G*2.0), rho_scale=0.01, out_dx=0.5*G*T*T,
+              out_dv=G*T, out_drho=1.0, drift_t=T, deg_scale=float(analysis.bulk_neighbours(3)))
     net = make_network(3, a.layers, sc, train=False, seed=a.seed)
     fr = Frame.from_particles(p, 0.0)
     t0 = time.perf_counter()

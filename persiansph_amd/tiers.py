@@ -3,9 +3,13 @@
     python -m persiansph_amd.tiers --dx 0.01 --t-end 1.0 --samples 20 --out tiers.json
 
 Both solvers stop at each sample time t_k (the first force pass with Time >= t_k, as an output
-time of Solve would); at each stop the script compares the FP32-mixed state with the FP64 state
-(density and acceleration relative errors over the particles both still hold) and the surge
-front positions, and records the wall time per step of each tier.
+time of Solve would).  At each stop the script records:
+  - the per-step error: one force pass of each tier on the same state (the FP64 run's), with
+    the relative L2 error of the FP32-mixed accelerations and continuity rates;
+  - the trajectory difference: density excess, acceleration and position of the two runs
+    (these grow with the chaotic divergence of the two trajectories, not with a per-step error);
+  - the surge front of each run and its relative difference;
+  - the wall time per step of each tier.
 """
 import argparse
 import json
@@ -57,11 +61,22 @@ def main():
         rho_err = analysis.relative_error(q32["Density"][i32][fr] - 1000.0, q64["Density"][i64][fr] - 1000.0)
         a_err = analysis.relative_error(q32["a"][i32][fr], q64["a"][i64][fr])
         x_err = float(np.abs(q32["x"][i32][fr] - q64["x"][i64][fr]).max())
+        # per-step error: both tiers' force pass on the FP64 run's state
+        one = {}
+        for name, c in (("fp64", cfg), ("fp32", cfg32)):
+            s1 = Solver(c, q64, s64.deltat, time=s64.time)
+            s1.run(1, tout=0.0)
+            one[name] = s1.particles()
+            s1.close()
+        f1 = one["fp64"]["IsFree"] == 1
+        step_a = analysis.relative_error(one["fp32"]["a"][f1], one["fp64"]["a"][f1])
+        step_dd = analysis.relative_error(one["fp32"]["dDensity"][f1], one["fp64"]["dDensity"][f1])
         f64 = analysis.front_position(q64, 3)
         f32 = analysis.front_position(q32, 3)
         row = {
             "t_fp64": s64.time, "t_fp32": s32.time, "steps_fp64": steps["fp64"], "steps_fp32": steps["fp32"],
             "n_fp64": int(len(q64)), "n_fp32": int(len(q32)),
+            "step_rel_err_accel": step_a, "step_rel_err_ddensity": step_dd,
             "rel_err_density_excess": rho_err, "rel_err_accel": a_err, "max_abs_dx": x_err,
             "front_fp64": f64, "front_fp32": f32,
             "front_rel_err": abs(f32 - f64)/f64, "front_rel_err_travel": abs(f32 - f64)/max(f64 - L0, 1e-9),
@@ -75,6 +90,8 @@ def main():
         "samples": rows,
         "speedup_fp32_over_fp64": (wall["fp64"]/steps["fp64"])/(wall["fp32"]/steps["fp32"]),
         "max_front_rel_err": max(r["front_rel_err"] for r in rows),
+        "max_step_rel_err_accel": max(r["step_rel_err_accel"] for r in rows),
+        "max_step_rel_err_ddensity": max(r["step_rel_err_ddensity"] for r in rows),
     }
     if args.out:
         with open(args.out, "w") as f:

@@ -14,11 +14,11 @@ def test_trained_network_beats_stepping_without_it():
     trajs = []
     for aspect in (0.6, 1.0, 1.6, 2.0):
         cfg, p, dt = S.dam_break_variant(aspect, dx=dx, area=0.5)
-        trajs.append(S.trajectory(cfg, p, dt, steps, k))
+        trajs.append(S.trajectory(cfg, p, dt, steps*dt, k*dt))
     cfg_h, p_h, dt_h = S.dam_break_variant(1.3, dx=dx, area=0.5)
-    held = [S.trajectory(cfg_h, p_h, dt_h, steps, k)]
+    held = [S.trajectory(cfg_h, p_h, dt_h, steps*dt_h, k*dt_h)]
     radius = 2.0*1.3*dx
-    sc = S.scales_from(trajs, k, radius)
+    sc = S.scales_from(trajs, k, dt_h, radius)
     print(f"[surrogate] {len(p_h)} particles, {sum(len(t) for t in trajs)} frames, scales {sc}", flush=True)
     net = S.make_network(2, 4, sc, seed=3)
     untrained = S.prediction_error(held, S.net_predictor(net), sc)
