@@ -105,6 +105,8 @@ typedef struct psph_status {
     size_t  n;               /* particle count after the call */
     int32_t stop_reason;     /* PSPH_STOP_* */
     int32_t mid_step;        /* 1 if stopped after the force pass with Move pending */
+    int64_t list_builds;     /* steps of this call that sorted the particles and built the
+                                neighbour lists (all of them unless the lists are reused) */
 } psph_status;
 
 /* The cell grid built by CellInitiate (Domain.cpp:549-660). */

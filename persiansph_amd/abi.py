@@ -73,6 +73,7 @@ class Status(ctypes.Structure):
         ("time", ctypes.c_double), ("deltat", ctypes.c_double), ("deltatmin", ctypes.c_double),
         ("steps", ctypes.c_int64), ("pairs", ctypes.c_int64), ("deleted", ctypes.c_int64),
         ("n", ctypes.c_size_t), ("stop_reason", ctypes.c_int32), ("mid_step", ctypes.c_int32),
+        ("list_builds", ctypes.c_int64),
     ]
 
 

@@ -100,8 +100,10 @@ struct psph_handle {
     // scratch for the pair query
     int *pair_cnt = nullptr;
     long long *pair_off = nullptr;
-    // neighbour lists of the fast fluid path (capacity in Params::nl_cap)
+    // neighbour lists of the fast fluid path (capacity in Params::nl_cap; skin lists with list
+    // reuse); skin sub-lists of the wall targets (capacity Params::wl_cap)
     int32_t *nl = nullptr;
+    int32_t *wl = nullptr;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -116,27 +118,50 @@ static int dev_alloc(psph_handle *h, void **p, size_t bytes)
     return 0;
 }
 
-// (Re)allocates the neighbour lists for `cap` entries per free target.  The stream must be idle.
+static int dev_free(psph_handle *h, void **p)
+{
+    if (*p) {
+        HIPCHECK(hipFree(*p));
+        for (auto &q : h->allocs) if (q == *p) q = nullptr;
+        *p = nullptr;
+    }
+    return 0;
+}
+
+// (Re)allocates the neighbour lists for `cap` entries per free target.  The stream must be
+// idle; lists reallocated under list reuse are rebuilt by the next step (the caller requests
+// it).
 static int nl_alloc(psph_handle *h, int cap)
 {
     Params &P = h->P;
-    if (h->nl) {
-        HIPCHECK(hipFree(h->nl));
-        for (auto &q : h->allocs) if (q == (void *)h->nl) q = nullptr;
-        h->nl = nullptr;
-    }
+    int rc = dev_free(h, (void **)&h->nl);
+    if (rc) return rc;
     size_t nf = P.nfree > 0 ? (size_t)P.nfree : 1;
-    int rc = dev_alloc(h, (void **)&h->nl, sizeof(int32_t)*(size_t)cap*nf);
+    rc = dev_alloc(h, (void **)&h->nl, sizeof(int32_t)*(size_t)cap*nf);
     if (rc) return rc;
     P.nl_cap = cap;
     for (auto &d : h->D) d.nl = h->nl;
     return 0;
 }
 
+// (Re)allocates the wall targets' skin sub-lists for `cap` entries per (target, lane).
+static int wl_alloc(psph_handle *h, int cap)
+{
+    Params &P = h->P;
+    int rc = dev_free(h, (void **)&h->wl);
+    if (rc) return rc;
+    size_t nw = P.nfixed > 0 ? 4*(size_t)P.nfixed : 4;
+    rc = dev_alloc(h, (void **)&h->wl, sizeof(int32_t)*(size_t)cap*nw);
+    if (rc) return rc;
+    P.wl_cap = cap;
+    for (auto &d : h->D) d.wl = h->wl;
+    return 0;
+}
+
 // First list capacity: 1.5x the neighbour count a uniform spread of the free particles over
 // their bounding box would give inside the support radius, plus a margin; a multiple of 4.
 // Targets that still overflow are handled exactly by the gathers, and psph_run grows the lists.
-static int nl_initial_cap(const psph_handle *h)
+static int nl_initial_cap(const psph_handle *h, double radius)
 {
     const Params &P = h->P;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -146,12 +171,22 @@ static int nl_initial_cap(const psph_handle *h)
         ++nf;
         for (int c = 0; c < 3; ++c) { lo[c] = std::min(lo[c], q.x[c]); hi[c] = std::max(hi[c], q.x[c]); }
     }
-    double R = P.cellfac*P.h_uniform;
+    double R = radius;
     double vol = 1.0;
     for (int c = 0; c < P.dim; ++c) vol *= std::max(hi[c] - lo[c], R);
     double est = (double)nf/vol*((P.dim == 3) ? 4.0/3.0*M_PI*R*R*R : M_PI*R*R);
     double capd = std::min(1.5*est + 8.0, 1024.0);
     int cap = std::max(16, (int)std::ceil(capd));
+    return (cap + 3) & ~3;
+}
+
+// First wall sub-list capacity: a wall next to the fluid sees about half the sphere, and one of
+// its four lanes up to ~35 % of that; 1.5x margin, a multiple of 4.  Overflows make every step
+// rebuild until psph_run has grown the sub-lists.
+static int wl_initial_cap(const psph_handle *h, double radius)
+{
+    double est = (double)nl_initial_cap(h, radius);       // ~1.5x the full-sphere count + 8
+    int cap = std::max(8, (int)std::ceil(0.5*0.35*est + 8.0));
     return (cap + 3) & ~3;
 }
 
@@ -385,6 +420,10 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.ncnt, sizeof(int32_t)*N);
     AL(a.ovq, sizeof(int32_t)*N);
     AL(a.ovs, sizeof(double4)*2*N);
+    if (P.verlet) {
+        AL(a.wcnt, sizeof(int32_t)*4*N);
+        AL(a.xref, sizeof(double4)*N);
+    }
     AL(h->pair_cnt, sizeof(int)*N);
     AL(h->pair_off, sizeof(long long)*N);
 #undef AL
@@ -567,7 +606,10 @@ static void enqueue_step(psph_handle *h, int d, bool finish_only)
     const Params &P = h->P;
     const Dev &D = h->D[d];
     if (!finish_only) {
-        timed(h, KScan, [&] { launch_scan(D, P, h->stream); });
+        timed(h, KScan, [&] {
+            if (P.verlet) launch_keys(D, P, h->stream);      // only on steps that rebuild
+            launch_scan(D, P, h->stream);
+        });
         timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
         timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
         timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
@@ -590,6 +632,28 @@ static int push_ctl(psph_handle *h)
 {
     HIPCHECK(hipMemcpyAsync(h->ctl_dev, h->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Readies the next step after the host has written or re-sorted the state held by variant d:
+// pushes the control block and computes the cell keys and arrival ranks for the next sort; with
+// list reuse the next step rebuilds and its k_keys computes them.
+// With list reuse: makes the next step sort and rebuild the lists.  A step pending its Move
+// ends with the tail, which turns need_rebuild into rebuild for the step after it.
+static void request_rebuild(psph_handle *h)
+{
+    if (!h->P.verlet) return;
+    if (h->ctl_host->mid_step) h->ctl_host->need_rebuild = 1;
+    else h->ctl_host->rebuild = 1;
+}
+
+static int prime_first_step(psph_handle *h, int d)
+{
+    request_rebuild(h);
+    int rc = push_ctl(h);
+    if (rc) return rc;
+    if (!h->P.verlet) launch_keys(h->D[d], h->P, h->stream);
+    HIPCHECK(hipGetLastError());
     return 0;
 }
 
@@ -628,9 +692,8 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     h->ctl_host->leave = 0;
     h->ctl_host->halt = kRun;
     h->trans_ok = false;
-    rc = push_ctl(h);
+    rc = prime_first_step(h, 0);
     if (rc) return rc;
-    launch_keys(h->D[0], P, h->stream);
     *ndel = del;
     return 0;
 }
@@ -815,19 +878,48 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.mixed32 = 1;
         }
     }
+    // neighbour-list reuse on the fast fluid path: skin = PSPH_SKIN x the support radius
+    // (default 0.03; 0 turns reuse off), at most what keeps the reach inside the 5^DIM cells
+    // around a particle's cell; entries then pack the index in 25 bits
+    P.verlet = 0;
+    P.verlet_force = 0;
+    P.wl_cap = 0;
+    const double support = P.cellfac*P.h_uniform;
+    if (P.fast11 || P.mixed32) {
+        double skin = 0.03;
+        if (const char *e = std::getenv("PSPH_SKIN")) skin = std::atof(e);
+        double csmin = std::min(P.csize[0], P.csize[1]);
+        if (P.dim == 3) csmin = std::min(csmin, P.csize[2]);
+        skin = std::min(skin, 2.0*csmin*(1.0 - 1e-9)/support - 1.0);
+        if (skin > 0.0 && n < ((size_t)1 << 25)) {
+            P.verlet = 1;
+            double reach = support*(1.0 + skin), half = 0.5*skin*support;
+            P.reach2 = reach*reach*(1.0 + 1e-9);
+            P.skin_trig2 = half*half*(1.0 - 1e-6);
+            P.adj2 = csmin*csmin*(1.0 - 1e-12);
+            P.verlet_force = std::getenv("PSPH_REBUILD_EVERY_STEP") ? 1 : 0;
+        }
+    }
     rc = allocate(h, n);
     if (rc) return rc;
-    // neighbour lists for the fast fluid path (entries pack the index in 27 bits)
+    // neighbour lists for the fast fluid path (entries pack the index in 27 bits, 25 with reuse)
     P.nl_cap = 0;
     P.nl_codes = n < ((size_t)1 << 27) ? 1 : 0;
-    h->nl = nullptr;
+    h->nl = h->wl = nullptr;
     if (P.mixed32 && !P.nl_codes)
         return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier holds at most 2^27 particles");
     if (P.fast11 || P.mixed32) {
-        int cap0 = nl_initial_cap(h);
+        double radius = P.verlet ? std::sqrt(P.reach2) : support;
+        int cap0 = nl_initial_cap(h, radius);
         if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);
         rc = nl_alloc(h, cap0);
         if (rc) return rc;
+        if (P.verlet) {
+            int wcap = wl_initial_cap(h, radius);
+            if (const char *e = std::getenv("PSPH_WL_CAP")) wcap = std::max(4, (std::atoi(e) + 3) & ~3);
+            rc = wl_alloc(h, wcap);
+            if (rc) return rc;
+        }
     }
     HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
     h->parity_base = 0;
@@ -844,10 +936,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     else C.first_step = 1;
     rc = write_state(h, h->host.data(), n, 0);
     if (rc) return rc;
-    rc = push_ctl(h);
+    rc = prime_first_step(h, 0);
     if (rc) return rc;
-    launch_keys(h->D[0], P, h->stream);
-    HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->uploaded = true;
     return 0;
@@ -864,6 +954,7 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     int rc = sync_ctl(h);
     if (rc) return rc;
     Ctl &C = *h->ctl_host;
+    const long long builds0 = C.rebuilds;
     C.tf = tf;
     C.tout = tout;
     C.max_steps = max_steps;
@@ -894,12 +985,18 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
         rc = sync_ctl(h);
         if (rc) return rc;
         if (C.reorders > 0) h->trans_ok = true;
-        if (C.nl_over > 0 && h->P.nl_cap > 0) {
+        const bool grow_nl = (C.nl_over > 0 || (C.list_over & 1)) && h->P.nl_cap > 0;
+        const bool grow_wl = (C.list_over & 2) && h->P.wl_cap > 0;
+        if (grow_nl || grow_wl) {
             // some targets had more neighbours than the lists hold (their pairs were found
-            // again by the gathers): grow the lists for the steps to come
-            rc = nl_alloc(h, ((h->P.nl_cap*3/2) + 3) & ~3);
+            // again by the gathers, or the steps rebuilt every time): grow the lists for the
+            // steps to come; reused lists are rebuilt by the next step
+            if (grow_nl) rc = nl_alloc(h, ((h->P.nl_cap*3/2) + 3) & ~3);
+            if (!rc && grow_wl) rc = wl_alloc(h, ((h->P.wl_cap*3/2) + 3) & ~3);
             if (rc) return rc;
             C.nl_over = 0;
+            C.list_over = 0;
+            request_rebuild(h);
             rc = push_ctl(h);
             if (rc) return rc;
         }
@@ -928,6 +1025,7 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     s.pairs = (int64_t)C.last_pairs;
     s.n = (size_t)h->P.n;
     s.mid_step = C.mid_step;
+    s.list_builds = C.rebuilds - builds0;
     s.stop_reason = (C.halt == kHaltOutput) ? PSPH_STOP_OUTPUT : (C.halt == kHaltTend) ? PSPH_STOP_TEND : PSPH_STOP_STEPS;
     if (st) *st = s;
     if (C.halt == kHaltError) {
@@ -972,8 +1070,10 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     // an extra sort changes no later result)
     Ctl save = *h->ctl_host;
     h->ctl_host->halt = kRun;
+    if (P.verlet) h->ctl_host->rebuild = 1;
     rc = push_ctl(h);
     if (rc) return rc;
+    if (P.verlet) launch_keys(D, P, h->stream);          // ranks are kept only for rebuilds
     launch_scan(D, P, h->stream);
     launch_scatter(D, P, h->stream);
     launch_cellsort(D, P, h->stream);
@@ -999,12 +1099,12 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     // restore the control block (the reorder counter keeps the new parity) and the next keys
     rc = sync_ctl(h);
     if (rc) return rc;
-    long long reorders = h->ctl_host->reorders;
+    long long reorders = h->ctl_host->reorders, rebuilds = h->ctl_host->rebuilds;
     *h->ctl_host = save;
     h->ctl_host->reorders = reorders;
-    rc = push_ctl(h);
+    h->ctl_host->rebuilds = rebuilds;
+    rc = prime_first_step(h, cur_src(h));               // the order changed: lists are stale
     if (rc) return rc;
-    launch_keys(h->D[cur_src(h)], P, h->stream);
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->trans_ok = false;
     if (pairs && (size_t)tot > cap) return fail(PSPH_ERR_CAPACITY, "need room for %lld pairs", tot);

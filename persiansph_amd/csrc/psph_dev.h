@@ -2,9 +2,10 @@
 //
 // Particles live in HBM as structure-of-arrays, sorted by the reference's cell key
 // kx + nx*(ky + ny*kz) (the CellInitiate grid, Source/Domain.cpp:549-660), and inside a
-// cell by original (reference) index.  Every step re-sorts: counting sort on the cell key
-// (a one-digit radix sort whose digit is the whole key), cell-start table, per-cell order by
-// original index, then a gather of the persistent columns into the new order.
+// cell by original (reference) index.  A sort is a counting sort on the cell key (a one-digit
+// radix sort whose digit is the whole key), cell-start table, per-cell order by original index,
+// then a gather of the persistent columns into the new order.  It runs every step, or, with
+// neighbour-list reuse, at every list build; steps between builds keep the order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,6 +61,13 @@ struct Ctl {
     int leave, xmat;
     int nl_over;                        // targets whose neighbours overflowed the lists (cumulative)
     int nl_q;                           // overflow queue length of the current force pass
+    // neighbour-list reuse (Params::verlet): rebuild = this step sorts and builds the lists;
+    // need_rebuild = some free particle moved more than half the skin since the last build
+    // (set by integrate, turned into rebuild by the tail); list_over = bit 0: a free target's
+    // skin list overflowed, bit 1: a wall target's did (every step rebuilds until the host
+    // has grown the lists)
+    int rebuild, need_rebuild, list_over, pad2;
+    long long rebuilds;                 // list builds (sorts) since upload
     int ct, shep_counter, first_step, pad;
     long long reorders;                 // completed reorders (buffer parity)
 };
@@ -91,6 +99,16 @@ struct Params {
     int nl_cap, nl_codes;
     // FP32-mixed tier of solid/soil runs: CalcForce2233 pair arithmetic and sums in FP32
     int soil32;
+    // neighbour-list reuse of the fast fluid path: the particle order, the cell table and the
+    // lists are rebuilt only when some free particle has moved more than half the skin since
+    // the last build.  Lists then hold every candidate within the support radius plus the skin
+    // (reach); each step keeps exactly the reference's pairs from them.
+    int verlet, verlet_force;          // reuse on; rebuild every step anyway (test switch)
+    int wl_cap;                        // wall sub-list capacity per (wall target, lane), multiple of 4
+    double reach2;                     // (support + skin)^2 with a small margin
+    double skin_trig2;                 // (skin/2)^2 less a small margin: the rebuild trigger
+    double adj2;                       // squared smallest cell size (less a margin): pairs closer
+                                       // than this are in neighbouring cells
 };
 
 // Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
@@ -151,11 +169,18 @@ struct Dev {
     // neighbour lists of the free targets (fast fluid path): entry e of target t at
     // nl[((e/4)*nfree + t)*4 + e%4], holding j | code << 27 (code = 0..26, the neighbour cell
     // (dx+1) + 3(dy+1) + 9(dz+1) relative to the target's cell); ncnt[t] = in-support count,
-    // which may exceed the capacity (the gathers then rescan for the rest)
+    // which may exceed the capacity (the gathers then rescan for the rest).  With list reuse
+    // the lists are skin lists: every candidate within the support radius plus the skin at the
+    // last build, j | code << 25 with the code over the 5^DIM cells around the target's cell
     int32_t *nl, *ncnt;
     // overflow queue of the force pass: target t and its partial sums (k_force11_nl_over ends it)
     int32_t *ovq;
     double4 *ovs;
+    // list reuse: skin sub-lists of the wall targets, one per (target, lane) gt = 4 t + lane,
+    // entry e at wl[((e/4)*4 nfixed + gt)*4 + e%4], and their counts; positions at the last
+    // build.  (The free targets' skin lists are nl/ncnt themselves.)
+    int32_t *wl, *wcnt;
+    double4 *xref;
 };
 
 }  // namespace psph

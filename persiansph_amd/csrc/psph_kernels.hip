@@ -12,6 +12,13 @@
 // Every kernel returns at once when the device control block says the run has halted, so the
 // host may queue many steps ahead without round trips.
 //
+// Neighbour-list reuse (fast fluid path): the sort (keys, scan, scatter, cellsort, permuting
+// reorder) and the list builds run only on steps that rebuild.  Lists then carry every candidate
+// within the support radius plus a skin; integrate flags a rebuild once some free particle has
+// moved half the skin since the build.  The other steps copy the columns in place order, walk
+// the skin lists and keep exactly the pairs the reference's cell search and rij/h <= Cellfac
+// test would form.
+//
 // Pair sets: a free particle gathers every candidate in the 27 (3-D) or 9 (2-D) cells of the
 // reference grid around its own and keeps those with rij/h <= Cellfac, exactly the pairs
 // MainNeighbourSearch + CalcForce* evaluate.  Each pair contribution is computed with the
@@ -96,9 +103,14 @@ __device__ inline int cell_rank(const Dev &D, int key, bool isfree)
     return rank;
 }
 
+// With neighbour-list reuse the sort and the list builds run only on steps that rebuild.
+__device__ inline bool sort_skipped(const Dev &D, const Params &P) { return P.verlet && !D.ctl->rebuild; }
+
+// Cell keys and arrival ranks of the current positions: at upload (and after host-side changes)
+// for the first step, and with list reuse at the head of every step that rebuilds.
 __global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     int i = blockIdx.x*kBlock + threadIdx.x;
     if (i >= P.n) return;
     double4 p = D.pos[i];
@@ -145,7 +157,7 @@ __device__ inline unsigned long long pack_counts(int all, int free_)
 
 __global__ void __launch_bounds__(256) k_scan_reduce(Dev D, Params P)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     __shared__ unsigned long long sh[17];
     int base = blockIdx.x*kScanTile + threadIdx.x*8;
     unsigned long long s = 0;
@@ -158,7 +170,7 @@ __global__ void __launch_bounds__(256) k_scan_reduce(Dev D, Params P)
 
 __global__ void __launch_bounds__(1024) k_scan_top(Dev D, Params P, int nblocks)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     __shared__ unsigned long long sh[17];
     __shared__ unsigned long long carry;
     if (threadIdx.x == 0) carry = 0;
@@ -181,7 +193,7 @@ __global__ void __launch_bounds__(1024) k_scan_top(Dev D, Params P, int nblocks)
 
 __global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     __shared__ unsigned long long sh[17];
     int base = blockIdx.x*kScanTile + threadIdx.x*8;
     unsigned long long v[8];
@@ -211,7 +223,7 @@ __global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
 // counting-sort scatter: new position = cell start + arrival rank
 __global__ void __launch_bounds__(kBlock) k_scatter(Dev D, Params P)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     int i = blockIdx.x*kBlock + threadIdx.x;
     if (i >= P.n) return;
     int k = D.key[i];
@@ -229,7 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Dev D, Params P)
 // sorted order, cell c's share starting at cstart[c] - fstart[c]).
 __global__ void __launch_bounds__(kBlock) k_cellsort(Dev D, Params P)
 {
-    if (D.ctl->halt) return;
+    if (D.ctl->halt || sort_skipped(D, P)) return;
     int p = blockIdx.x*kBlock + threadIdx.x;
     if (p >= P.n) return;
     int k = D.key_b[p];
@@ -271,7 +283,8 @@ __device__ inline void cell_origin(const Params &P, int key, double &o0, double 
 // ------------------------------------------------------------------------------------------
 // reorder: gather persistent columns into the new order; build target lists; per-particle
 // derived values used by the pair gathers; TIR of free soil/solid (StartAcceleration
-// Domain.cpp:951-988)
+// Domain.cpp:951-988).  With list reuse, a step that does not rebuild keeps the order: the
+// columns are copied as they stand (the double buffer flips every step either way).
 // ------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
 {
@@ -279,11 +292,12 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
     __syncthreads();
+    const bool sorted = !sort_skipped(D, P);
     int i = blockIdx.x*kBlock + threadIdx.x;
     bool act = i < P.n;
     uint32_t f = 0;
     if (act) {
-        int s = D.perm[i];
+        int s = sorted ? D.perm[i] : i;
         double4 p = D.pos[s], v = D.vel[s], vh = D.vh[s];
         double2 mm = D.mmu[s];
         f = D.flags[s];
@@ -294,7 +308,11 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
         D.mmu_b[i] = mm;
         D.flags_b[i] = f;
         D.orig_b[i] = D.orig[s];
-        D.key[i] = D.key_b[i];
+        if (sorted) {
+            D.key[i] = D.key_b[i];
+            if (P.verlet) D.xref[i] = p;      // positions the lists are built from
+        }
+        // key_b[i] stays the cell of slot i at the last sort, the origin of the FP32 positions
         const ClassParams &c = cp[flag_cls(f)];
         if (flag_free(f)) {
             // sound speed and P/rho^2 of a free particle (CalcForce11 :67-72, :139)
@@ -336,7 +354,10 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
             }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl->reorders += 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        D.ctl->reorders += 1;
+        if (sorted) D.ctl->rebuilds += 1;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -356,36 +377,6 @@ __device__ inline void row_range(const Dev &D, const Params &P, int kx, int ky, 
     end = D.cstart[base + x1 + 1];
 }
 
-
-// Neighbour row r of the target's cell restricted to the cells that intersect the support
-// sphere around the target at (px, py, pz): cells whose box lies farther than the support
-// radius cannot hold a pair.  The kept cells of a row are contiguous (the middle cell is the
-// nearest in x).  sup2m is the squared support radius with a small safety margin.
-__device__ inline void row_range_culled(const Dev &D, const Params &P, int kx, int ky, int kz, int r,
-                                        double px, double py, double pz, double sup2m,
-                                        int &beg, int &end)
-{
-    int dy = (r % 3) - 1, dz = (P.dim == 3) ? (r / 3) - 1 : 0;
-    int yy = ky + dy, zz = kz + dz;
-    beg = end = 0;
-    if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) return;
-    double oy = P.blpf[1] + ky*P.csize[1], oz = P.blpf[2] + kz*P.csize[2];
-    double ey = (dy < 0) ? py - oy : (dy > 0) ? oy + P.csize[1] - py : 0.0;
-    double ez = (dz < 0) ? pz - oz : (dz > 0) ? oz + P.csize[2] - pz : 0.0;
-    ey = ey > 0.0 ? ey : 0.0;
-    ez = ez > 0.0 ? ez : 0.0;
-    double dyz = ey*ey + ez*ez;
-    if (dyz > sup2m) return;
-    double ox = P.blpf[0] + kx*P.csize[0];
-    double el = px - ox, er = ox + P.csize[0] - px;
-    el = el > 0.0 ? el : 0.0;
-    er = er > 0.0 ? er : 0.0;
-    int x0 = (kx > 0 && dyz + el*el <= sup2m) ? kx - 1 : kx;
-    int x1 = (kx + 1 < P.nc[0] && dyz + er*er <= sup2m) ? kx + 1 : kx;
-    int base = P.nc[0]*(yy + P.nc[1]*zz);
-    beg = D.cstart[base + x0];
-    end = D.cstart[base + x1 + 1];
-}
 
 __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, int cj)
 {
@@ -510,16 +501,69 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// pair membership and cell codes of the fast fluid path
+// ------------------------------------------------------------------------------------------
+
+// The reference's pair test rij/h <= Cellfac (CalcForce11, Interaction.cpp:32) on a squared
+// distance: an r^2 band decides all but a 1e-13-wide shell around the support radius, where
+// the exact test runs on sqrt(r^2) as the reference forms rij.
+__device__ inline bool in_support(const Params &P, double r2)
+{
+    if (r2 <= P.sup2_lo) return true;
+    if (r2 > P.sup2_hi) return false;
+    return (sqrt(r2)/P.h_uniform) <= P.cellfac;
+}
+
+// Whether the cells of two keys are the same or neighbours: the reference pairs particles of
+// neighbouring cells only (MainNeighbourSearch, Domain.cpp:808-945).  Cells can be slightly
+// smaller than the support (the CellNo rounding of CellInitiate), so a pair closer than the
+// support may still be two cells apart and outside the reference's pair set.
+__device__ __noinline__ bool keys_adjacent(const Params &P, int a, int b)
+{
+    int ax, ay, az, bx, by, bz;
+    decode_key(P, a, ax, ay, az);
+    decode_key(P, b, bx, by, bz);
+    return abs(ax - bx) <= 1 && abs(ay - by) <= 1 && abs(az - bz) <= 1;
+}
+
+// Lists cover the W^DIM cells around the target's cell, W = 2 REACH + 1: REACH = 1 for lists
+// of one step (support radius), REACH = 2 for reused lists (support radius plus a skin of at
+// most one cell).  A list entry is j | code << shift with the cell code
+// (dx+REACH) + W (dy+REACH) + W^2 (dz+REACH) of the neighbour's cell (dz = 0 in 2-D).
+template <int REACH>
+struct Codes {
+    static constexpr int W = 2*REACH + 1;
+    static constexpr int shift = (REACH == 1) ? 27 : 25;
+    static constexpr unsigned mask = (1u << shift) - 1u;
+};
+template <int REACH>
+__device__ inline int nl_entry(int j, int code) { return (int)((unsigned)j | ((unsigned)code << Codes<REACH>::shift)); }
+template <int REACH>
+__device__ inline int nl_index(int w) { return (int)((unsigned)w & Codes<REACH>::mask); }
+template <int REACH>
+__device__ inline int nl_code(int w) { return (int)((unsigned)w >> Codes<REACH>::shift); }
+template <int REACH>
+__device__ inline void code_offset(int c, int &dx, int &dy, int &dz)
+{
+    constexpr int W = Codes<REACH>::W;
+    dx = c % W - REACH;
+    dy = (c / W) % W - REACH;
+    dz = c / (W*W) - REACH;
+}
 // Wall pass of the fast fluid configuration (cubic spline, uniform h, fluid only): the same
-// smoothing as k_wall with the pair test decided on rij^2 outside a 1e-13 band around the
-// support radius (inside it the exact rij/h <= Cellfac test runs), neighbour rows culled
-// against the support sphere, and cells without free particles skipped.
-template <int DIM>
+// smoothing as k_wall with the pair test of in_support, cells culled against the support
+// sphere, and cells without free particles skipped.  Four lanes per fixed target, lane q
+// taking neighbour cells q, q+4, ...; their sums meet in a fixed order ((q0 + q1) + (q2 + q3)),
+// so results stay deterministic.
+//
+// REACH = 2 (list reuse): a step that rebuilds walks the 5^DIM cells around the wall's cell and
+// records, per lane, every free candidate within the support radius plus the skin; the steps
+// until the next build walk these sub-lists instead of the cells.
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
 {
     if (D.ctl->halt) return;
-    // four lanes per fixed target, lane q taking neighbour cells q, q+4, ...; their sums meet
-    // in a fixed order ((q0 + q1) + (q2 + q3)), so results stay deterministic
     const int gt = xcd_block()*kBlock + threadIdx.x;
     const int t = gt >> 2, quad = gt & 3;
     const bool act = t < P.nfixed;
@@ -533,48 +577,87 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
     decode_key(P, key_i, kx, ky, kz);
     double SK = 0.0, Pw = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
     const double gx = P.gravity[0], gy = P.gravity[1], gz = P.gravity[2];
-    const double h = P.h_uniform, cellfac = P.cellfac, kw = P.kw;
-    const double lo = P.sup2_lo, hi = P.sup2_hi;
-    const double sup2m = P.sup2_hi*(1.0 + 1e-9);
-    const double ox = P.blpf[0] + kx*P.csize[0], oy = P.blpf[1] + ky*P.csize[1];
-    const double oz = (DIM == 3) ? P.blpf[2] + kz*P.csize[2] : 0.0;
-    const int ncell = act ? ((DIM == 3) ? 27 : 9) : 0;
-    for (int cc = quad; cc < ncell; cc += 4) {
-        int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (DIM == 3) ? (cc / 9) - 1 : 0;
-        int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
-        if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
-        int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
-        if (D.cfree[cell] == 0) continue;                  // no free particle to gather from
-        double ex = (dxk < 0) ? pi.x - ox : (dxk > 0) ? ox + P.csize[0] - pi.x : 0.0;
-        double ey = (dyk < 0) ? pi.y - oy : (dyk > 0) ? oy + P.csize[1] - pi.y : 0.0;
-        double ez = (dzk < 0) ? pi.z - oz : (dzk > 0) ? oz + P.csize[2] - pi.z : 0.0;
-        ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
-        if (ex*ex + ey*ey + ez*ez > sup2m) continue;       // the cell lies outside the support
-        int beg = D.cstart[cell], end = D.cstart[cell + 1];
-        for (int j = beg; j < end; ++j) {
-            if (!flag_free(D.flags_b[j])) continue;
+    const double h = P.h_uniform, kw = P.kw;
+    // contribution of free neighbour j at squared distance r2 (a pair of the reference's set)
+    auto add = [&](int j, double4 pj, double x0, double x1, double x2, double r2) {
+        double q = sqrt(r2)/h;
+        // orientation of the reference's pair (P1 = smaller key / larger index), formed
+        // branch-free as in k_wall
+        int kj = D.key[j];
+        int oj = D.orig_b[j];
+        int wall_first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
+        double sgn = wall_first ? 1.0 : -1.0;
+        x0 *= sgn; x1 *= sgn; x2 *= sgn;
+        double K;
+        if (q < 1.0) K = kw*(1.0-(3.0/2.0)*q*q+(3.0/4.0)*q*q*q);
+        else if (q < 2.0) K = kw*((1.0/4.0)*(2.0-q)*(2.0-q)*(2.0-q));
+        else K = 0.0;
+        SK += K;
+        double4 vj = D.vel_b[j];
+        Pw += vj.w * K + (gx*x0 + gy*x1 + gz*x2)*pj.w*K;
+        if (noslip) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
+    };
+    const size_t gs = 4*(size_t)P.nfixed;                   // sub-list slot group stride
+    if (REACH == 2 && !D.ctl->rebuild) {
+        // between builds: the lane's sub-list, reduced to the reference's pairs
+        const int cap = P.wl_cap;
+        int c = act ? D.wcnt[gt] : 0;
+        int m = c < cap ? c : cap;
+        const int4 *lst = reinterpret_cast<const int4 *>(D.wl) + gt;
+        int4 w = make_int4(0, 0, 0, 0);
+        for (int e = 0; e < m; ++e) {
+            int sl = e & 3;
+            if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
+            int j = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
             double4 pj = D.pos_b[j];
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
             double r2 = x0*x0 + x1*x1 + x2*x2;
-            if (r2 > hi) continue;
-            double rij = sqrt(r2);
-            double q = rij/h;
-            if (r2 > lo && !(q <= cellfac)) continue;        // outside the reference's pair set
-            // orientation of the reference's pair (P1 = smaller key / larger index), formed
-            // branch-free as in k_wall
-            int kj = D.key[j];
-            int oj = D.orig_b[j];
-            int wall_first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
-            double sgn = wall_first ? 1.0 : -1.0;
-            x0 *= sgn; x1 *= sgn; x2 *= sgn;
-            double K;
-            if (q < 1.0) K = kw*(1.0-(3.0/2.0)*q*q+(3.0/4.0)*q*q*q);
-            else if (q < 2.0) K = kw*((1.0/4.0)*(2.0-q)*(2.0-q)*(2.0-q));
-            else K = 0.0;
-            SK += K;
-            double4 vj = D.vel_b[j];
-            Pw += vj.w * K + (gx*x0 + gy*x1 + gz*x2)*pj.w*K;
-            if (noslip) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
+            bool in = in_support(P, r2);
+            if (in && r2 > P.adj2) in = keys_adjacent(P, key_i, D.key[j]);
+            if (in) add(j, pj, x0, x1, x2, r2);
+        }
+    } else {
+        constexpr int W = Codes<REACH>::W;
+        const double cx = P.csize[0], cy = P.csize[1], cz = P.csize[2];
+        const double cull2 = ((REACH == 2) ? P.reach2 : P.sup2_hi)*(1.0 + 1e-9);
+        const double ox = P.blpf[0] + kx*cx, oy = P.blpf[1] + ky*cy;
+        const double oz = (DIM == 3) ? P.blpf[2] + kz*cz : 0.0;
+        const int ncell = act ? ((DIM == 3) ? W*W*W : W*W) : 0;
+        const int cap = P.wl_cap;
+        int4 *out = reinterpret_cast<int4 *>(D.wl) + gt;
+        int wc = 0, w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+        for (int cc = quad; cc < ncell; cc += 4) {
+            int dxk = (cc % W) - REACH, dyk = ((cc / W) % W) - REACH, dzk = (DIM == 3) ? (cc / (W*W)) - REACH : 0;
+            int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+            if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+            int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
+            if (D.cfree[cell] == 0) continue;                  // no free particle to gather from
+            // distance from the wall to the cell's box
+            double ex = (dxk < 0) ? pi.x - (ox + (dxk + 1)*cx) : (dxk > 0) ? (ox + dxk*cx) - pi.x : 0.0;
+            double ey = (dyk < 0) ? pi.y - (oy + (dyk + 1)*cy) : (dyk > 0) ? (oy + dyk*cy) - pi.y : 0.0;
+            double ez = (dzk < 0) ? pi.z - (oz + (dzk + 1)*cz) : (dzk > 0) ? (oz + dzk*cz) - pi.z : 0.0;
+            ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
+            if (ex*ex + ey*ey + ez*ez > cull2) continue;       // the cell lies outside the reach
+            const bool adj = dxk >= -1 && dxk <= 1 && dyk >= -1 && dyk <= 1 && dzk >= -1 && dzk <= 1;
+            int beg = D.cstart[cell], end = D.cstart[cell + 1];
+            for (int j = beg; j < end; ++j) {
+                if (!flag_free(D.flags_b[j])) continue;
+                double4 pj = D.pos_b[j];
+                double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                double r2 = x0*x0 + x1*x1 + x2*x2;
+                if (REACH == 2 && r2 <= P.reach2) {
+                    int sl = wc & 3;
+                    w0 = (sl == 0) ? j : w0; w1 = (sl == 1) ? j : w1; w2 = (sl == 2) ? j : w2; w3 = (sl == 3) ? j : w3;
+                    if (sl == 3 && wc < cap) out[(size_t)(wc >> 2)*gs] = make_int4(w0, w1, w2, w3);
+                    ++wc;
+                }
+                if (adj && in_support(P, r2)) add(j, pj, x0, x1, x2, r2);
+            }
+        }
+        if (REACH == 2 && act) {
+            if ((wc & 3) != 0 && wc < cap) out[(size_t)(wc >> 2)*gs] = make_int4(w0, w1, w2, w3);
+            D.wcnt[gt] = wc;
+            if (wc > cap) atomicOr(&D.ctl->list_over, 2);
         }
     }
     SK += __shfl_xor(SK, 1, 64); Pw += __shfl_xor(Pw, 1, 64);
@@ -602,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
     const int ci = flag_cls(fi);
     if (P.mixed32) {
         double o0, o1, o2;
-        cell_origin(P, key_i, o0, o1, o2);
+        cell_origin(P, D.key_b[i], o0, o1, o2);      // the cell of slot i at the last sort
         D.f0[i] = make_float4((float)(pi.x - o0), (float)(pi.y - o1), (float)(pi.z - o2), (float)dw);
         D.f1[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)(Pw/(dw*dw)));
         if (noslip) D.f2[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), (float)Cw);
@@ -1026,16 +1109,29 @@ __device__ inline double rsq_nr(double x)
     return fma(y, e, y);
 }
 
+// Whether list entry j of free target i is a pair of the reference's set at the current
+// positions, given their squared distance: inside the support (in_support) and in neighbouring
+// cells, which needs the current cell keys only beyond the smallest cell size.
+__device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, double r2)
+{
+    bool in = in_support(P, r2);
+    if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[j]);
+    return in;
+}
+
 struct Acc11 { double a0, a1, a2, dd; };
 
-template <int DIM>
+// SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
+// current positions (skin_member); i is the target (for its cell key).
+template <int DIM, bool SKIN>
 __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
-                                   double4 gi0, double4 gi1, double4 gi2, double4 gi3, int ci, int j,
+                                   double4 gi0, double4 gi1, double4 gi2, double4 gi3, int ci, int i, int j,
                                    double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j], a3 = D.g3[j];
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
     double r2 = x0*x0 + x1*x1 + x2*x2;
+    if (SKIN && !skin_member(D, P, i, j, r2)) return;
     double invr = rsq_nr(r2);
     double rij = r2 > 0.0 ? r2*invr : 0.0;
     double q = rij*P.invh;
@@ -1079,77 +1175,125 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
 
 // ------------------------------------------------------------------------------------------
 // neighbour lists of the fast fluid path.  k_nlist finds each free target's in-support
-// neighbours once per step (the exact FP64 pair test of the reference); the gathers then run
-// over the lists, so every lane of a wave evaluates pairs for nearly all of the loop instead of
-// waiting on candidate scans of other lanes.
+// neighbours (the exact FP64 pair test of the reference); the gathers then run over the lists,
+// so every lane of a wave evaluates pairs for nearly all of the loop instead of waiting on
+// candidate scans of other lanes.
+//
+// List reuse (REACH = 2): a step that rebuilds records every candidate within the support
+// radius plus a skin (the skin list); until some free particle has moved half the skin, no
+// pair can form outside those candidates.  The gathers walk the skin list every step and keep
+// the entries that are pairs of the reference's set at the current positions, so every step's
+// pair set is the reference's; the sums run in the list's order, the walk order of the build.
 // ------------------------------------------------------------------------------------------
 
-// Visits the in-support neighbours of free target i (at px, py, pz in cell kx, ky, kz) in the
-// order the lists store them: neighbour rows (dy, dz) in row order, each culled to the cells
-// that intersect the support sphere, candidates in sorted order.  f(j, code) receives the
-// neighbour cell code (dx+1) + 3(dy+1) + 9(dz+1).
-template <int DIM, class F>
-__device__ inline void for_each_neighbour(const Dev &D, const Params &P, int i, double px, double py,
-                                          double pz, int kx, int ky, int kz, F &&f)
+// Visits the candidates of free target i (at px, py, pz in cell kx, ky, kz) in list order:
+// neighbour rows (dy, dz) in row order over the W^DIM cells around the target's cell, each row
+// culled to the cells that intersect the sphere of squared radius cull2 around the target,
+// candidates in sorted order.  f(j, code, r2) receives every candidate j != i of the kept cells
+// with its cell code and its squared distance, computed as the reference forms xij.
+template <int DIM, int REACH, class F>
+__device__ inline void walk_rows(const Dev &D, const Params &P, int i, double px, double py, double pz,
+                                 int kx, int ky, int kz, double cull2, F &&f)
 {
-    const double lo = P.sup2_lo, hi = P.sup2_hi;
-    const double sup2m = hi*(1.0 + 1e-9);
-    const int nrows = (DIM == 3) ? 9 : 3;
+    constexpr int W = Codes<REACH>::W;
+    const int nrows = (DIM == 3) ? W*W : W;
+    const double cx = P.csize[0], cy = P.csize[1], cz = P.csize[2];
+    const double ox = P.blpf[0] + kx*cx, oy = P.blpf[1] + ky*cy, oz = P.blpf[2] + kz*cz;
+    double el = px - ox, er = ox + cx - px;
+    el = el > 0.0 ? el : 0.0;
+    er = er > 0.0 ? er : 0.0;
     for (int r = 0; r < nrows; ++r) {
-        int beg = 0, end = 0;
-        row_range_culled(D, P, kx, ky, kz, r, px, py, pz, sup2m, beg, end);
+        int dy = (r % W) - REACH, dz = (DIM == 3) ? (r / W) - REACH : 0;
+        int yy = ky + dy, zz = kz + dz;
+        if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+        double ey = (dy < 0) ? py - (oy + (dy + 1)*cy) : (dy > 0) ? (oy + dy*cy) - py : 0.0;
+        double ez = (dz < 0) ? pz - (oz + (dz + 1)*cz) : (dz > 0) ? (oz + dz*cz) - pz : 0.0;
+        ey = ey > 0.0 ? ey : 0.0;
+        ez = ez > 0.0 ? ez : 0.0;
+        double dyz = ey*ey + ez*ez;
+        if (dyz > cull2) continue;
+        int x0 = kx, x1 = kx;
+        if (kx > 0 && dyz + el*el <= cull2) {
+            x0 = kx - 1;
+            if (REACH == 2 && kx > 1 && dyz + (el + cx)*(el + cx) <= cull2) x0 = kx - 2;
+        }
+        if (kx + 1 < P.nc[0] && dyz + er*er <= cull2) {
+            x1 = kx + 1;
+            if (REACH == 2 && kx + 2 < P.nc[0] && dyz + (er + cx)*(er + cx) <= cull2) x1 = kx + 2;
+        }
+        int base = P.nc[0]*(yy + P.nc[1]*zz);
+        int beg = D.cstart[base + x0], end = D.cstart[base + x1 + 1];
         if (beg >= end) continue;
-        int dy = (r % 3) - 1, dz = (DIM == 3) ? (r / 3) - 1 : 0;
-        int base = P.nc[0]*((ky + dy) + P.nc[1]*(kz + dz));
-        int m0 = D.cstart[base + kx], m1 = D.cstart[base + kx + 1];   // the middle cell
-        int crow = 3*(dy + 1) + 9*(dz + 1);
+        // the x offset of a candidate's cell from the cell starts of the row (indices clamped to
+        // the row, so cells outside the grid take no candidates)
+        int b0 = D.cstart[base + kx], b1 = D.cstart[base + kx + 1];
+        int bm = (REACH == 2) ? D.cstart[base + (kx > 0 ? kx - 1 : 0)] : 0;
+        int bp = (REACH == 2) ? D.cstart[base + (kx + 2 < P.nc[0] ? kx + 2 : P.nc[0])] : 0;
+        int crow = W*(dy + REACH) + W*W*(dz + REACH);
         double4 nx = D.pos_b[beg];
         for (int j = beg; j < end; ++j) {
             double4 pj = nx;
             if (j + 1 < end) nx = D.pos_b[j + 1];
             if (j == i) continue;
-            double x0 = px - pj.x, x1 = py - pj.y, x2 = pz - pj.z;
-            double r2 = x0*x0 + x1*x1 + x2*x2;
-            bool in = r2 <= lo;
-            if (!in && r2 <= hi) in = (sqrt(r2)/P.h_uniform) <= P.cellfac;
-            if (in) f(j, crow + ((j < m0) ? 0 : (j < m1) ? 1 : 2));
+            double x0d = px - pj.x, x1d = py - pj.y, x2d = pz - pj.z;
+            double r2 = x0d*x0d + x1d*x1d + x2d*x2d;
+            int cxo = (REACH == 1) ? ((j < b0) ? 0 : (j < b1) ? 1 : 2)
+                                   : (int)(j >= bm) + (int)(j >= b0) + (int)(j >= b1) + (int)(j >= bp);
+            f(j, crow + cxo, r2);
         }
     }
 }
 
-__device__ inline int nl_entry(int j, int code) { return (int)((unsigned)j | ((unsigned)code << 27)); }
-__device__ inline int nl_index(int w) { return (int)((unsigned)w & 0x7ffffffu); }
-__device__ inline int nl_code(int w) { return (int)((unsigned)w >> 27); }
+// A lane's list in slot groups of four (one 16-byte store per group): entry e of the lane at
+// out[(e/4)*gs] component e%4.  Entries past the capacity are counted, not stored.
+struct ListWriter {
+    int4 *out;
+    size_t gs;
+    int cap, cnt;
+    int w0, w1, w2, w3;
+    __device__ ListWriter(int4 *o, size_t g, int c) : out(o), gs(g), cap(c), cnt(0), w0(0), w1(0), w2(0), w3(0) {}
+    __device__ void push(int v)
+    {
+        int sl = cnt & 3;
+        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
+        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+        ++cnt;
+    }
+    __device__ void flush()
+    {
+        if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+    }
+};
 
-template <int DIM>
+// List build.  REACH = 1: the pairs of this step.  REACH = 2 (list reuse, steps that rebuild
+// only): the skin list, every candidate within the support radius plus the skin; one that
+// overflows makes every step rebuild until the host has grown the lists.
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
 {
     if (D.ctl->halt) return;
+    if (REACH == 2 && !D.ctl->rebuild) return;
     int t = xcd_block()*kBlock + threadIdx.x;
     if (t >= P.nfree) return;
     int i = D.free_list[t];
     double4 pi = D.pos_b[i];
     int kx, ky, kz;
     decode_key(P, D.key[i], kx, ky, kz);
-    const int cap = P.nl_cap;
-    int4 *out = reinterpret_cast<int4 *>(D.nl) + t;          // slot group g at out[g*nfree]
-    const size_t gs = (size_t)P.nfree;
-    int cnt = 0;
-    int w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    ListWriter out(reinterpret_cast<int4 *>(D.nl) + t, (size_t)P.nfree, P.nl_cap);
     const bool codes = P.nl_codes != 0;
-    for_each_neighbour<DIM>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, [&](int j, int code) {
-        int v = codes ? nl_entry(j, code) : j;
-        int sl = cnt & 3;
-        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
-        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-        ++cnt;
+    const double reach2 = P.reach2;
+    const double cull2 = ((REACH == 2) ? reach2 : P.sup2_hi)*(1.0 + 1e-9);
+    walk_rows<DIM, REACH>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, cull2, [&](int j, int code, double r2) {
+        bool keep = (REACH == 2) ? (r2 <= reach2) : in_support(P, r2);
+        if (keep) out.push(codes ? nl_entry<REACH>(j, code) : j);
     });
-    if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-    D.ncnt[t] = cnt;
+    out.flush();
+    D.ncnt[t] = out.cnt;
+    if (REACH == 2 && out.cnt > P.nl_cap) atomicOr(&D.ctl->list_over, 1);
 }
 
 // FP64 fluid force pass over the neighbour lists.
-template <int DIM>
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -1178,7 +1322,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
         int sl = e & 3;
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
-        pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, codes ? nl_index(v) : v, mu_ff, mu_w, A, nff, nfw);
+        pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, codes ? nl_index<REACH>(v) : v,
+                                     mu_ff, mu_w, A, nff, nfw);
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_nl_over continues this target from these sums
@@ -1196,10 +1341,12 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
     force_partials(D, dtb, nff, nfw);
 }
 
-// Ends the targets of the FP64 list pass whose neighbours overflowed the lists: the neighbours
+// Ends the targets of the FP64 list pass whose neighbours overflowed the lists: the entries
 // past the capacity are found again in list order and added to the queued partial sums, which
-// is the same sequence of operations as one pass over a long enough list.
-template <int DIM>
+// is the same sequence of operations as one pass over a long enough list.  Lists overflow only
+// on steps that build them (a reused list that overflowed makes every step rebuild), so the
+// cell table is current.
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -1210,6 +1357,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P
     __syncthreads();
     const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     const int cap = P.nl_cap;
+    const double reach2 = P.reach2;
+    const double cull2 = ((REACH == 2) ? reach2 : P.sup2_hi)*(1.0 + 1e-9);
     for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
         int t = D.ovq[q];
         int i = D.free_list[t];
@@ -1222,8 +1371,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P
         int kx, ky, kz;
         decode_key(P, D.key[i], kx, ky, kz);
         int k = 0;
-        for_each_neighbour<DIM>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, [&](int j, int) {
-            if (k >= cap) pair11_fast<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, mu_ff, mu_w, A, nff, nfw);
+        walk_rows<DIM, REACH>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, cull2, [&](int j, int, double r2) {
+            if (!((REACH == 2) ? (r2 <= reach2) : in_support(P, r2))) return;      // a list entry
+            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, mu_ff, mu_w, A, nff, nfw);
             ++k;
         });
         D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
@@ -1243,16 +1393,44 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P
 // ------------------------------------------------------------------------------------------
 struct Acc32 { float a0, a1, a2, dd, ddc; };
 
-template <int DIM>
+// Skin-list membership on the FP32 squared distance: its error is below 1e-6 of the support
+// radius squared (cell-relative positions), so outside a 1e-4 band around the support radius
+// and beyond 1e-4 below the smallest cell size it decides as the FP64 test would; inside the
+// band the FP64 test runs on the FP64 positions.
+struct Band32 {
+    float lo, hi, adj;
+    __device__ explicit Band32(const Params &P)
+        : lo((float)(P.sup2_lo*(1.0 - 1e-4))), hi((float)(P.sup2_hi*(1.0 + 1e-4))),
+          adj((float)(P.adj2*(1.0 - 1e-4))) {}
+};
+
+__device__ inline bool skin_member64(const Dev &D, const Params &P, int i, int j)
+{
+    double4 pi = D.pos_b[i], pj = D.pos_b[j];
+    double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+    return skin_member(D, P, i, j, x0*x0 + x1*x1 + x2*x2);
+}
+
+__device__ inline bool skin_member32(const Dev &D, const Params &P, const Band32 &B, int i, int j, float r2)
+{
+    if (r2 > B.hi) return false;
+    if (r2 >= B.lo) return skin_member64(D, P, i, j);
+    return r2 <= B.adj || keys_adjacent(P, D.key[i], D.key[j]);
+}
+
+// SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
+// current positions (skin_member32); i is the target.
+template <int DIM, bool SKIN>
 __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
-                                  float4 gi0, float4 gi1, float4 gi2, float4 gi3, int ci, int j,
+                                  float4 gi0, float4 gi1, float4 gi2, float4 gi3, int ci, int i, int j,
                                   float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
                                   float kw, float kg, float kgh, float invh, float h, float h2c,
-                                  float mu_ff, float mu_w)
+                                  float mu_ff, float mu_w, const Band32 &band)
 {
     float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j], a3 = D.f3[j];
     float x0 = (gi0.x - a0.x) - o0, x1 = (gi0.y - a0.y) - o1, x2 = (gi0.z - a0.z) - o2;
     float r2 = x0*x0 + x1*x1 + x2*x2;
+    if (SKIN && !skin_member32(D, P, band, i, j, r2)) return;
     float invr = __builtin_amdgcn_rsqf(r2);
     float rij = r2 > 0.0f ? r2*invr : 0.0f;
     float q = rij*invh;
@@ -1297,8 +1475,9 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
 }
 
 // FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
-// offset dk CellSize of the cell-relative FP32 positions.
-template <int DIM>
+// offset dk CellSize of the cell-relative FP32 positions (each relative to the particle's cell
+// at the last sort).
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -1315,6 +1494,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
     const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
     const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
     unsigned nff = 0, nfw = 0;
@@ -1324,16 +1504,18 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
     const int4 *lst = reinterpret_cast<const int4 *>(D.nl) + t;
     const size_t gs = (size_t)P.nfree;
     auto pair = [&](int j, int c) {
-        float p0 = ((c % 3) - 1)*csx, p1 = (((c / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((c / 9) - 1)*csz : 0.0f;
-        pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, p0, p1, p2,
-                        A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+        int dx, dy, dz;
+        code_offset<REACH>(c, dx, dy, dz);
+        float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
+        pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, p0, p1, p2,
+                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
     };
     int4 w = make_int4(0, 0, 0, 0);
     for (int e = 0; e < m; ++e) {
         int sl = e & 3;
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
-        pair(nl_index(v), nl_code(v));
+        pair(nl_index<REACH>(v), nl_code<REACH>(v));
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_f32_nl_over continues this target from these sums
@@ -1355,7 +1537,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
 
 // Ends the overflowed targets of the FP32 list pass (see k_force11_nl_over); the queued FP32
 // sums are exact in FP64 and restored as FP32.
-template <int DIM>
+template <int DIM, int REACH>
 __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Params P)
 {
     if (D.ctl->halt) return;
@@ -1368,6 +1550,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Para
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
     const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
     const int cap = P.nl_cap;
+    const Band32 band(P);
+    const double reach2 = P.reach2;
+    const double cull2 = ((REACH == 2) ? reach2 : P.sup2_hi)*(1.0 + 1e-9);
     for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
         int t = D.ovq[q];
         int i = D.free_list[t];
@@ -1381,11 +1566,14 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Para
         decode_key(P, D.key[i], kx, ky, kz);
         double4 pi = D.pos_b[i];
         int k = 0;
-        for_each_neighbour<DIM>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, [&](int j, int c) {
+        walk_rows<DIM, REACH>(D, P, i, pi.x, pi.y, pi.z, kx, ky, kz, cull2, [&](int j, int c, double r2) {
+            if (!((REACH == 2) ? (r2 <= reach2) : in_support(P, r2))) return;      // a list entry
             if (k >= cap) {
-                float p0 = ((c % 3) - 1)*csx, p1 = (((c / 3) % 3) - 1)*csy, p2 = (DIM == 3) ? ((c / 9) - 1)*csz : 0.0f;
-                pair11_f32<DIM>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, j, p0, p1, p2,
-                                A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w);
+                int dx, dy, dz;
+                code_offset<REACH>(c, dx, dy, dz);
+                float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
+                pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, p0, p1, p2,
+                                            A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
             }
             ++k;
         });
@@ -1681,17 +1869,25 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
         D.vel_b[i] = v;
         D.vh_b[i] = vh;
     }
+    if (P.verlet && flag_free(f)) {
+        // list reuse: a particle more than half the skin away from where the lists were built
+        // makes the next step rebuild
+        double4 r = D.xref[i];
+        double d0 = p.x - r.x, d1 = p.y - r.y, d2 = p.z - r.z;
+        if (d0*d0 + d1*d1 + d2*d2 > P.skin_trig2) ctl->need_rebuild = 1;
+    }
     // CheckParticleLeave test against [BLPF, TRPR]
     if ((p.x > P.trpr[0]) || (p.y > P.trpr[1]) || (p.z > P.trpr[2]) ||
         (p.x < P.blpf[0]) || (p.y < P.blpf[1]) || (p.z < P.blpf[2])) {
         atomicAdd(&ctl->leave, 1);
         return;
     }
-    // next step's cell key (ListGenerate after Move)
+    // next step's cell key (ListGenerate after Move); the arrival ranks of the next sort come
+    // with it, or, with list reuse, from k_keys on the steps that rebuild
     int kx, ky, kz;
     int key = cell_key(P, p.x, p.y, p.z, kx, ky, kz);
     D.key[i] = key;
-    D.rank[i] = cell_rank(D, key, flag_free(f));
+    if (!P.verlet) D.rank[i] = cell_rank(D, key, flag_free(f));
 }
 
 // Time update and stop conditions (Solve loop, Domain.cpp:1762-1793); one thread.
@@ -1711,6 +1907,10 @@ __global__ void k_tail(Dev D, Params P)
     c->steps_done += 1;
     c->resume = 0;
     c->mid_step = 0;
+    if (P.verlet) {
+        c->rebuild = (c->need_rebuild || c->list_over || P.verlet_force) ? 1 : 0;
+        c->need_rebuild = 0;
+    }
     if (P.scheme == 0) {
         // Move_MVerlet's per-particle counters, uniform over free particles (Particle.cpp:191-195, 272-273)
         if (c->first_step) c->ct = 30;
@@ -1818,10 +2018,14 @@ void launch_wall(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfixed <= 0) return;
     if (P.fast11 || P.mixed32) {
-        if (P.dim == 3)
-            hipLaunchKernelGGL(k_wall_fast<3>, dim3(nblk(4*P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
-        else
-            hipLaunchKernelGGL(k_wall_fast<2>, dim3(nblk(4*P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+        const dim3 g(nblk(4*P.nfixed, kBlock)), b(kBlock);
+        if (P.dim == 3) {
+            if (P.verlet) hipLaunchKernelGGL((k_wall_fast<3, 2>), g, b, 0, s, D, P);
+            else hipLaunchKernelGGL((k_wall_fast<3, 1>), g, b, 0, s, D, P);
+        } else {
+            if (P.verlet) hipLaunchKernelGGL((k_wall_fast<2, 2>), g, b, 0, s, D, P);
+            else hipLaunchKernelGGL((k_wall_fast<2, 1>), g, b, 0, s, D, P);
+        }
         return;
     }
     if (P.any_solid)
@@ -1839,28 +2043,31 @@ void launch_force(const Dev &D, const Params &P, hipStream_t s)
     hipLaunchKernelGGL(k_force_fold, dim3(1), dim3(1024), 0, s, D, P, nblk(P.nfree, kForceBlock));
 }
 
+// The list passes of the fast fluid path: build (every step, or with list reuse on the steps
+// that rebuild), the gather and its overflow end.
+template <int DIM, int REACH>
+static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
+{
+    const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
+    hipLaunchKernelGGL((k_nlist<DIM, REACH>), gl, dim3(kBlock), 0, s, D, P);
+    if (P.mixed32) {
+        hipLaunchKernelGGL((k_force11_f32_nl<DIM, REACH>), gf, dim3(kForceBlock), 0, s, D, P);
+        hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+    } else {
+        hipLaunchKernelGGL((k_force11_nl<DIM, REACH>), gf, dim3(kForceBlock), 0, s, D, P);
+        hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+    }
+}
+
 static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nl_cap > 0) {
-        const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
         if (P.dim == 3) {
-            hipLaunchKernelGGL(k_nlist<3>, gl, dim3(kBlock), 0, s, D, P);
-            if (P.mixed32) {
-                hipLaunchKernelGGL(k_force11_f32_nl<3>, gf, dim3(kForceBlock), 0, s, D, P);
-                hipLaunchKernelGGL(k_force11_f32_nl_over<3>, go, dim3(kForceBlock), 0, s, D, P);
-            } else {
-                hipLaunchKernelGGL(k_force11_nl<3>, gf, dim3(kForceBlock), 0, s, D, P);
-                hipLaunchKernelGGL(k_force11_nl_over<3>, go, dim3(kForceBlock), 0, s, D, P);
-            }
+            if (P.verlet) launch_list_pass<3, 2>(D, P, s);
+            else launch_list_pass<3, 1>(D, P, s);
         } else {
-            hipLaunchKernelGGL(k_nlist<2>, gl, dim3(kBlock), 0, s, D, P);
-            if (P.mixed32) {
-                hipLaunchKernelGGL(k_force11_f32_nl<2>, gf, dim3(kForceBlock), 0, s, D, P);
-                hipLaunchKernelGGL(k_force11_f32_nl_over<2>, go, dim3(kForceBlock), 0, s, D, P);
-            } else {
-                hipLaunchKernelGGL(k_force11_nl<2>, gf, dim3(kForceBlock), 0, s, D, P);
-                hipLaunchKernelGGL(k_force11_nl_over<2>, go, dim3(kForceBlock), 0, s, D, P);
-            }
+            if (P.verlet) launch_list_pass<2, 2>(D, P, s);
+            else launch_list_pass<2, 1>(D, P, s);
         }
         return;
     }

@@ -201,7 +201,8 @@ def test_fp32_mixed_against_fp64(dim):
 
 
 def _run_env(monkeypatch, env, cfg, p, dt, steps):
-    monkeypatch.delenv("PSPH_NL_CAP", raising=False)
+    for k in ("PSPH_NL_CAP", "PSPH_WL_CAP", "PSPH_SKIN", "PSPH_REBUILD_EVERY_STEP"):
+        monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     g = Solver(cfg, p, dt)
@@ -211,13 +212,16 @@ def _run_env(monkeypatch, env, cfg, p, dt, steps):
     return st, q
 
 
+@pytest.mark.parametrize("lists", ["per_step", "skin"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_neighbour_list_overflow_is_exact(monkeypatch, dim, precision):
-    """Targets with more neighbours than the per-step lists hold are finished by the overflow
-    pass, which continues the same sums in list order: with capacity 8 (most targets overflow
-    on the first batch, then the run grows the lists) the states are bitwise those of the
-    default capacity."""
+def test_neighbour_list_overflow_is_exact(monkeypatch, dim, precision, lists):
+    """Targets with more neighbours than the lists hold are finished by the overflow pass,
+    which continues the same sums in list order: with capacity 8 (most targets overflow on the
+    first batch, then the run grows the lists) the states are bitwise those of the default
+    capacity.  `per_step`: lists of the support radius built every step; `skin`: lists with the
+    skin, walked over the 5^DIM cells, rebuilt every step so that both runs sum in the same
+    order (wall sub-lists at capacity 4 as well)."""
     from persiansph_amd import FP32_MIXED
     if dim == 2:
         cfg, p, dt = cases.dam_break_2d(dx=0.02, scheme=1)
@@ -228,8 +232,10 @@ def test_neighbour_list_overflow_is_exact(monkeypatch, dim, precision):
         cfg.precision = FP32_MIXED
     print(f"[nlist] dim={dim} {precision} n={len(p)}", flush=True)
     steps = 40
-    base = _run_env(monkeypatch, {}, cfg, p, dt, steps)
-    over = _run_env(monkeypatch, {"PSPH_NL_CAP": "8"}, cfg, p, dt, steps)
+    env = {"PSPH_SKIN": "0"} if lists == "per_step" else {"PSPH_REBUILD_EVERY_STEP": "1"}
+    small = {"PSPH_NL_CAP": "8"} if lists == "per_step" else {"PSPH_NL_CAP": "8", "PSPH_WL_CAP": "4"}
+    base = _run_env(monkeypatch, env, cfg, p, dt, steps)
+    over = _run_env(monkeypatch, {**env, **small}, cfg, p, dt, steps)
     assert base[0].steps == over[0].steps == steps
     assert base[0].pairs == over[0].pairs
     for f in ("x", "v", "Density", "a"):
