@@ -77,7 +77,7 @@ struct psph_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     Params P{};
-    Dev D[2]{};                 // D[0]: persistent A -> B, D[1]: B -> A
+    Dev D{};                    // device buffers; buf[Ctl::side] holds the state between steps
     std::vector<void *> allocs;
     Ctl *ctl_dev = nullptr;
     Ctl *ctl_host = nullptr;    // pinned
@@ -89,7 +89,6 @@ struct psph_handle {
     bool uploaded = false;
     bool broken = false;
     psph_grid grid{};
-    long long parity_base = 0;         // reorders already reflected in D[] selection
     bool trans_ok = false;             // transient columns match the current order
     // timing
     bool timing = false;
@@ -140,7 +139,7 @@ static int nl_alloc(psph_handle *h, int cap)
     rc = dev_alloc(h, (void **)&h->nl, sizeof(int32_t)*(size_t)cap*nf);
     if (rc) return rc;
     P.nl_cap = cap;
-    for (auto &d : h->D) d.nl = h->nl;
+    h->D.nl = h->nl;
     return 0;
 }
 
@@ -154,7 +153,7 @@ static int wl_alloc(psph_handle *h, int cap)
     rc = dev_alloc(h, (void **)&h->wl, sizeof(int32_t)*(size_t)cap*nw);
     if (rc) return rc;
     P.wl_cap = cap;
-    for (auto &d : h->D) d.wl = h->wl;
+    h->D.wl = h->wl;
     return 0;
 }
 
@@ -388,12 +387,12 @@ static int allocate(psph_handle *h, size_t n)
         AL(a.f0, sizeof(float4)*N);
         AL(a.f1, sizeof(float4)*N);
         AL(a.f2, sizeof(float4)*N);
-        AL(a.f3, sizeof(float4)*N);
+        AL(a.fw, sizeof(float4)*N);
     } else if (P.fast11) {
         AL(a.g0, sizeof(double4)*N);
         AL(a.g1, sizeof(double4)*N);
         AL(a.g2, sizeof(double4)*N);
-        AL(a.g3, sizeof(double4)*N);
+        AL(a.gw, sizeof(double4)*N);
     }
     AL(a.wal, sizeof(double4)*N);
     AL(a.acc, sizeof(double4)*N);
@@ -414,7 +413,6 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.cfree, sizeof(int32_t)*(P.ncells + 1));
     AL(a.fstart, sizeof(int32_t)*(P.ncells + 1));
     AL(a.bsum, sizeof(unsigned long long)*(scan_blocks(P) + 1));
-    AL(a.fpart, sizeof(unsigned long long)*3*(N/64 + 2));
     AL(a.free_list, sizeof(int32_t)*N);
     AL(a.fixed_list, sizeof(int32_t)*N);
     AL(a.ncnt, sizeof(int32_t)*N);
@@ -422,6 +420,7 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.ovs, sizeof(double4)*2*N);
     if (P.verlet) {
         AL(a.wcnt, sizeof(int32_t)*4*N);
+        AL(a.wact, sizeof(int32_t)*N);
         AL(a.xref, sizeof(double4)*N);
     }
     AL(h->pair_cnt, sizeof(int)*N);
@@ -429,28 +428,34 @@ static int allocate(psph_handle *h, size_t n)
 #undef AL
     a.ctl = h->ctl_dev;
     a.cls = h->cls_dev;
-    for (int d = 0; d < 2; ++d) {
-        Dev x = a;
-        int s = d, t = 1 - d;
-        x.pos = pos[s]; x.vel = vel[s]; x.vh = vh[s]; x.mmu = mmu[s]; x.flags = flags[s]; x.orig = orig[s];
-        x.sig = sig[s]; x.sigh = sigh[s]; x.tau = tau[s]; x.tauh = tauh[s]; x.eps = eps[s]; x.epsh = epsh[s];
-        x.pos_b = pos[t]; x.vel_b = vel[t]; x.vh_b = vh[t]; x.mmu_b = mmu[t]; x.flags_b = flags[t]; x.orig_b = orig[t];
-        x.sig_b = sig[t]; x.sigh_b = sigh[t]; x.tau_b = tau[t]; x.tauh_b = tauh[t]; x.eps_b = eps[t]; x.epsh_b = epsh[t];
-        h->D[d] = x;
-    }
+    for (int b = 0; b < 2; ++b)
+        a.buf[b] = Cols{pos[b], vel[b], vh[b], mmu[b], flags[b], orig[b],
+                        sig[b], sigh[b], tau[b], tauh[b], eps[b], epsh[b]};
+    h->D = a;
     if (!h->ctl_host) HIPCHECK(hipHostMalloc((void **)&h->ctl_host, sizeof(Ctl), hipHostMallocDefault));
     h->cap = N;
     return 0;
 }
 
-// Index of the Dev variant whose *source* buffers hold the current state.
-static int cur_src(psph_handle *h)
+// Buffer of the persistent columns that holds the current state (control block read back):
+// buf[side] between steps; during a step halted after its force pass, the step's working copy.
+static int state_buffer(const psph_handle *h)
 {
-    long long r = h->ctl_host->reorders;
-    return (int)((r - h->parity_base) & 1);
+    const Ctl &C = *h->ctl_host;
+    return C.mid_step ? (C.side ^ C.rebuild) : C.side;
 }
 
-// Writes particles (reference order) into the current buffers; classes must be built.
+// The device buffers with the persistent columns of buffer b as the current ones (host reads).
+static Dev with_state(const psph_handle *h, int b)
+{
+    Dev D = h->D;
+    const Cols &S = h->D.buf[b];
+    D.pos = S.pos; D.vel = S.vel; D.vh = S.vh; D.mmu = S.mmu; D.flags = S.flags; D.orig = S.orig;
+    D.sig = S.sig; D.sigh = S.sigh; D.tau = S.tau; D.tauh = S.tauh; D.eps = S.eps; D.epsh = S.epsh;
+    return D;
+}
+
+// Writes particles (reference order) into persistent buffer `buf`; classes must be built.
 static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf)
 {
     Params &P = h->P;
@@ -486,7 +491,7 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
             to_sym(P.scheme == 1 ? q.Straina : q.Strainb, &epsh[6*i]);
         }
     }
-    Dev &D = h->D[buf];
+    const Cols &D = h->D.buf[buf];
     HIPCHECK(hipMemcpy(D.pos, pos.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.vel, vel.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.vh, vh.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
@@ -512,9 +517,7 @@ static int read_state(psph_handle *h)
     size_t n = (size_t)P.n;
     HIPCHECK(hipStreamSynchronize(h->stream));
     HIPCHECK(hipMemcpy(h->ctl_host, h->ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost));
-    int b = cur_src(h);
-    // after a reorder the state sits in the reorder destination of the variant that ran it
-    const Dev &D = h->D[b];
+    const Dev D = with_state(h, state_buffer(h));
     std::vector<double4> pos(n), vel(n), vh(n), aux(n), wal(n), acc(n), xsp(n);
     std::vector<double2> mmu(n), shep(n);
     std::vector<uint32_t> flags(n);
@@ -601,10 +604,10 @@ static int read_state(psph_handle *h)
 // ------------------------------------------------------------------------------------------
 // step queueing
 // ------------------------------------------------------------------------------------------
-static void enqueue_step(psph_handle *h, int d, bool finish_only)
+static void enqueue_step(psph_handle *h, bool finish_only)
 {
     const Params &P = h->P;
-    const Dev &D = h->D[d];
+    const Dev &D = h->D;
     if (!finish_only) {
         timed(h, KScan, [&] {
             if (P.verlet) launch_keys(D, P, h->stream);      // only on steps that rebuild
@@ -647,12 +650,12 @@ static void request_rebuild(psph_handle *h)
     else h->ctl_host->rebuild = 1;
 }
 
-static int prime_first_step(psph_handle *h, int d)
+static int prime_first_step(psph_handle *h)
 {
     request_rebuild(h);
     int rc = push_ctl(h);
     if (rc) return rc;
-    if (!h->P.verlet) launch_keys(h->D[d], h->P, h->stream);
+    if (!h->P.verlet) launch_keys(h->D, h->P, h->stream);
     HIPCHECK(hipGetLastError());
     return 0;
 }
@@ -683,16 +686,15 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     for (auto &q : h->host) nf += q.IsFree ? 1 : 0;
     P.nfree = nf;
     P.nfixed = P.n - nf;
-    // the state now lives in the source buffers of variant 0
-    h->parity_base = h->ctl_host->reorders;
-    rc = write_state(h, h->host.data(), h->host.size(), 0);
+    // the compacted state replaces the current one
+    rc = write_state(h, h->host.data(), h->host.size(), h->ctl_host->side);
     if (rc) return rc;
-    HIPCHECK(hipMemsetAsync(h->D[0].ccount, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
-    HIPCHECK(hipMemsetAsync(h->D[0].cfcnt, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
+    HIPCHECK(hipMemsetAsync(h->D.ccount, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
+    HIPCHECK(hipMemsetAsync(h->D.cfcnt, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
     h->ctl_host->leave = 0;
     h->ctl_host->halt = kRun;
     h->trans_ok = false;
-    rc = prime_first_step(h, 0);
+    rc = prime_first_step(h);
     if (rc) return rc;
     *ndel = del;
     return 0;
@@ -922,9 +924,10 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         }
     }
     HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
-    h->parity_base = 0;
     std::memset(h->ctl_host, 0, sizeof(Ctl));
     Ctl &C = *h->ctl_host;
+    C.side = 0;
+    C.rebuild = 1;                  // the first step sorts (every step does without list reuse)
     C.time = time;
     C.deltat = C.deltatint = C.deltatmin = dt;
     C.tf = INFINITY;
@@ -936,7 +939,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     else C.first_step = 1;
     rc = write_state(h, h->host.data(), n, 0);
     if (rc) return rc;
-    rc = prime_first_step(h, 0);
+    rc = prime_first_step(h);
     if (rc) return rc;
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->uploaded = true;
@@ -961,24 +964,17 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     C.halt = kRun;
     rc = push_ctl(h);
     if (rc) return rc;
-    launch_head(h->D[0], h->P, h->stream);
+    launch_head(h->D, h->P, h->stream);
     long long queued = 0;
     bool pending_finish = C.mid_step != 0 && max_steps > 0;
     const long long batch = 16;
     for (;;) {
         // queue a batch; every kernel is a no-op once the device has halted
-        int d = cur_src(h);
         long long nq = std::min<long long>(batch, max_steps - queued);
         if (pending_finish && nq < 1) nq = 1;
         for (long long k = 0; k < nq; ++k) {
-            if (pending_finish) {
-                // the pending step's Move runs on the variant whose reorder produced the state
-                enqueue_step(h, d ^ 1, true);
-                pending_finish = false;
-            } else {
-                enqueue_step(h, d, false);
-                d ^= 1;
-            }
+            enqueue_step(h, pending_finish);     // a pending step is finished by its Move alone
+            pending_finish = false;
             ++queued;
         }
         HIPCHECK(hipGetLastError());
@@ -1038,10 +1034,6 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     return 0;
 }
 
-// The pending-finish variant: a step halted for output ran its reorder on variant d, leaving
-// the state in that variant's destination buffers; cur_src() counts that reorder, so the
-// finishing integrate must run on the variant whose *destination* is current, i.e. d = cur^1.
-
 int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n)
 {
     if (!h) return fail(PSPH_ERR_INVALID, "null handle");
@@ -1064,13 +1056,12 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     if (rc) return rc;
     if (h->ctl_host->mid_step) return fail(PSPH_ERR_STATE, "pair query while a step is pending");
     const Params &P = h->P;
-    int d = cur_src(h);
-    const Dev &D = h->D[d];
+    const Dev &D = h->D;
     // a sort of the current positions (the cell structure after Move; the order is canonical, so
     // an extra sort changes no later result)
     Ctl save = *h->ctl_host;
     h->ctl_host->halt = kRun;
-    if (P.verlet) h->ctl_host->rebuild = 1;
+    h->ctl_host->rebuild = 1;
     rc = push_ctl(h);
     if (rc) return rc;
     if (P.verlet) launch_keys(D, P, h->stream);          // ranks are kept only for rebuilds
@@ -1078,8 +1069,7 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     launch_scatter(D, P, h->stream);
     launch_cellsort(D, P, h->stream);
     launch_reorder(D, P, h->stream);
-    const Dev &E = h->D[d];   // reorder wrote into D's destination buffers
-    launch_pairs(E, P, h->stream, 0, nullptr, h->pair_cnt, nullptr);
+    launch_pairs(D, P, h->stream, 0, nullptr, h->pair_cnt, nullptr);   // on the sorted copy
     std::vector<int> cnt(P.n);
     HIPCHECK(hipMemcpyAsync(cnt.data(), h->pair_cnt, sizeof(int)*P.n, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1091,19 +1081,20 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     if (pairs && (size_t)tot <= cap && tot > 0) {
         HIPCHECK(hipMemcpy(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice));
         HIPCHECK(hipMalloc((void **)&out_dev, sizeof(long long)*2*tot));
-        launch_pairs(E, P, h->stream, 1, h->pair_off, nullptr, out_dev);
+        launch_pairs(D, P, h->stream, 1, h->pair_off, nullptr, out_dev);
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipMemcpy(pairs, out_dev, sizeof(long long)*2*tot, hipMemcpyDeviceToHost));
         hipFree(out_dev);
     }
-    // restore the control block (the reorder counter keeps the new parity) and the next keys
+    // restore the control block; the sorted copy is now the state, and the lists are stale
     rc = sync_ctl(h);
     if (rc) return rc;
     long long reorders = h->ctl_host->reorders, rebuilds = h->ctl_host->rebuilds;
     *h->ctl_host = save;
     h->ctl_host->reorders = reorders;
     h->ctl_host->rebuilds = rebuilds;
-    rc = prime_first_step(h, cur_src(h));               // the order changed: lists are stale
+    h->ctl_host->side = save.side ^ 1;
+    rc = prime_first_step(h);
     if (rc) return rc;
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->trans_ok = false;

@@ -61,12 +61,14 @@ struct Ctl {
     int leave, xmat;
     int nl_over;                        // targets whose neighbours overflowed the lists (cumulative)
     int nl_q;                           // overflow queue length of the current force pass
-    // neighbour-list reuse (Params::verlet): rebuild = this step sorts and builds the lists;
+    // rebuild = this step sorts (always 1 without list reuse, see Params::verlet);
     // need_rebuild = some free particle moved more than half the skin since the last build
     // (set by integrate, turned into rebuild by the tail); list_over = bit 0: a free target's
     // skin list overflowed, bit 1: a wall target's did (every step rebuilds until the host
     // has grown the lists)
-    int rebuild, need_rebuild, list_over, pad2;
+    int rebuild, need_rebuild, list_over;
+    int side;                           // buffer of the persistent columns at the step's start
+    int nwact;                          // wall targets with skin neighbours at the last build
     long long rebuilds;                 // list builds (sorts) since upload
     int ct, shep_counter, first_step, pad;
     long long reorders;                 // completed reorders (buffer parity)
@@ -111,10 +113,24 @@ struct Params {
                                        // than this are in neighbouring cells
 };
 
-// Device buffers.  "A"/"B" columns are persistent and double-buffered across the reorder.
+// One set of the persistent columns.
+struct Cols {
+    double4 *pos, *vel, *vh;
+    double2 *mmu;
+    uint32_t *flags;
+    int32_t *orig;
+    double *sig, *sigh, *tau, *tauh, *eps, *epsh;
+};
+
+// Device buffers.  The persistent columns are double-buffered (buf[0], buf[1]).  Kernels see
+// them through pos..epsh, the state at the step's start (buf[Ctl::side]), and pos_b..epsh_b,
+// the step's working copy (buf[side ^ rebuild]): the reorder writes it, permuted, on steps that
+// sort, and it is the same buffer on steps that keep the order.  The kernels bind these views
+// from the control block at their start (bind() in the kernels).
 struct Dev {
     Ctl *ctl;
     ClassParams *cls;
+    Cols buf[2];
     // persistent (current order)
     double4 *pos;      // x, y, z, Density
     double4 *vel;      // vx, vy, vz, Pressure
@@ -137,13 +153,15 @@ struct Dev {
     // transient (current order)
     double4 *aux;      // Mass, Mu, sound speed, P/d^2  (fixed: wall density-based values)
     // gather views for the fast fluid path, written per step by reorder (free) and wall (fixed):
-    // g0 = x, y, z, d   g1 = v, P/d^2   g2 = v_visc, c   g3 = m (free) or FPMassC (fixed), 1/d, free, class
+    // g0 = x, y, z, d   g1 = v, P/d^2   g2 = c, meta, m (free) or FPMassC (fixed), 1/d
+    // gw = v_visc (fixed particles only)
     // (d, P, c: the density, pressure and sound speed the pair formulas use for that particle;
-    //  v_visc = v for free particles and 2 v - NSv for no-slip walls)
-    double4 *g0, *g1, *g2, *g3;
+    //  meta = class for free particles, -1 - class for fixed ones; v_visc = 2 v - NSv for no-slip
+    //  walls, v otherwise, and a free neighbour's v_visc is its v)
+    double4 *g0, *g1, *g2, *gw;
     // FP32-mixed gather views: f0 = position relative to the particle's cell origin, d;
-    // f1 = v, P/d^2; f2 = v_visc, c; f3 = m or FPMassC, 1/d, free, class
-    float4 *f0, *f1, *f2, *f3;
+    // f1 = v, P/d^2; f2 = c, meta, m or FPMassC, 1/d; fw = v_visc (fixed particles only)
+    float4 *f0, *f1, *f2, *fw;
     double4 *wal;      // fixed: NSv x,y,z, wall density d_w
     double4 *acc;      // a x,y,z, dDensity
     double4 *xsp;      // VXSPH x,y,z, ZWab
@@ -164,7 +182,6 @@ struct Dev {
     int32_t *cfcnt;    // per-cell free-particle counts being accumulated for the next step
     int32_t *cfree;    // per-cell free-particle counts of the current step
     unsigned long long *bsum;   // scan block sums (packed all/free counts)
-    unsigned long long *fpart;  // force pass per-block partials: pairs ff, pairs fw, dt bits
     int32_t *free_list, *fixed_list;
     // neighbour lists of the free targets (fast fluid path): entry e of target t at
     // nl[((e/4)*nfree + t)*4 + e%4], holding j | code << 27 (code = 0..26, the neighbour cell
@@ -180,6 +197,7 @@ struct Dev {
     // entry e at wl[((e/4)*4 nfixed + gt)*4 + e%4], and their counts; positions at the last
     // build.  (The free targets' skin lists are nl/ncnt themselves.)
     int32_t *wl, *wcnt;
+    int32_t *wact;     // wall targets with skin neighbours at the last build (Ctl::nwact of them)
     double4 *xref;
 };
 

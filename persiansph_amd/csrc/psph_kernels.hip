@@ -58,6 +58,21 @@ __device__ inline int xcd_block()
     return (x < r) ? x*(q + 1) + k : r*(q + 1) + (x - r)*q + k;
 }
 
+// The persistent-column views of this step (see Dev): the state at the step's start is in
+// buf[side], the working copy in buf[side ^ rebuild].  Only the tail changes side and rebuild,
+// at the end of a step, so every kernel of a step binds the same buffers.
+__device__ inline Dev bind(const Dev &D0)
+{
+    Dev D = D0;
+    const int s = D0.ctl->side, b = s ^ D0.ctl->rebuild;
+    const Cols &S = D0.buf[s], &B = D0.buf[b];
+    D.pos = S.pos; D.vel = S.vel; D.vh = S.vh; D.mmu = S.mmu; D.flags = S.flags; D.orig = S.orig;
+    D.sig = S.sig; D.sigh = S.sigh; D.tau = S.tau; D.tauh = S.tauh; D.eps = S.eps; D.epsh = S.epsh;
+    D.pos_b = B.pos; D.vel_b = B.vel; D.vh_b = B.vh; D.mmu_b = B.mmu; D.flags_b = B.flags; D.orig_b = B.orig;
+    D.sig_b = B.sig; D.sigh_b = B.sigh; D.tau_b = B.tau; D.tauh_b = B.tauh; D.eps_b = B.eps; D.epsh_b = B.epsh;
+    return D;
+}
+
 // ------------------------------------------------------------------------------------------
 // cell keys (ListGenerate, Domain.cpp:662-784): floor((x-BLPF)/CellSize), clamped to the grid
 // for particles within hmax outside it
@@ -108,8 +123,9 @@ __device__ inline bool sort_skipped(const Dev &D, const Params &P) { return P.ve
 
 // Cell keys and arrival ranks of the current positions: at upload (and after host-side changes)
 // for the first step, and with list reuse at the head of every step that rebuilds.
-__global__ void __launch_bounds__(kBlock) k_keys(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_keys(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt || sort_skipped(D, P)) return;
     int i = blockIdx.x*kBlock + threadIdx.x;
     if (i >= P.n) return;
@@ -221,8 +237,9 @@ __global__ void __launch_bounds__(256) k_scan_down(Dev D, Params P)
 }
 
 // counting-sort scatter: new position = cell start + arrival rank
-__global__ void __launch_bounds__(kBlock) k_scatter(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_scatter(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt || sort_skipped(D, P)) return;
     int i = blockIdx.x*kBlock + threadIdx.x;
     if (i >= P.n) return;
@@ -286,77 +303,82 @@ __device__ inline void cell_origin(const Params &P, int key, double &o0, double 
 // Domain.cpp:951-988).  With list reuse, a step that does not rebuild keeps the order: the
 // columns are copied as they stand (the double buffer flips every step either way).
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_reorder(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
     __syncthreads();
     const bool sorted = !sort_skipped(D, P);
     int i = blockIdx.x*kBlock + threadIdx.x;
-    bool act = i < P.n;
-    uint32_t f = 0;
-    if (act) {
-        int s = sorted ? D.perm[i] : i;
-        double4 p = D.pos[s], v = D.vel[s], vh = D.vh[s];
-        double2 mm = D.mmu[s];
-        f = D.flags[s];
-        if (P.dim == 2) v.z = 0.0;          // StartAcceleration zeroes v(2) in 2-D (Domain.cpp:1011)
-        D.pos_b[i] = p;
-        D.vel_b[i] = v;
-        D.vh_b[i] = vh;
-        D.mmu_b[i] = mm;
-        D.flags_b[i] = f;
-        D.orig_b[i] = D.orig[s];
-        if (sorted) {
-            D.key[i] = D.key_b[i];
-            if (P.verlet) D.xref[i] = p;      // positions the lists are built from
-        }
-        // key_b[i] stays the cell of slot i at the last sort, the origin of the FP32 positions
-        const ClassParams &c = cp[flag_cls(f)];
-        if (flag_free(f)) {
-            // sound speed and P/rho^2 of a free particle (CalcForce11 :67-72, :139)
-            double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
-            double pr = v.w/(p.w*p.w);
-            D.aux[i] = make_double4(mm.x, mm.y, C, pr);
-            if (P.mixed32) {
-                double o0, o1, o2;
-                cell_origin(P, D.key_b[i], o0, o1, o2);
-                D.f0[i] = make_float4((float)(p.x - o0), (float)(p.y - o1), (float)(p.z - o2), (float)p.w);
-                D.f1[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)pr);
-                D.f2[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)C);
-                D.f3[i] = make_float4((float)mm.x, (float)(1.0/p.w), 1.0f, (float)flag_cls(f));
-            } else if (P.fast11) {
-                D.g0[i] = p;
-                D.g1[i] = make_double4(v.x, v.y, v.z, pr);
-                D.g2[i] = make_double4(v.x, v.y, v.z, C);
-                D.g3[i] = make_double4(mm.x, 1.0/p.w, 1.0, (double)flag_cls(f));
+    if (i < P.n) {
+        const int s = sorted ? D.perm[i] : i;
+        const uint32_t f = D.flags[s];
+        // a step that keeps the order works in place (pos_b is pos): fixed particles of the
+        // fluid path then need nothing here (the wall pass writes their views)
+        if (sorted || flag_free(f) || P.any_solid) {
+            double4 p = D.pos[s], v = D.vel[s];
+            double2 mm = D.mmu[s];
+            if (P.dim == 2) v.z = 0.0;          // StartAcceleration zeroes v(2) in 2-D (Domain.cpp:1011)
+            if (sorted) {
+                D.pos_b[i] = p;
+                D.vel_b[i] = v;
+                D.vh_b[i] = D.vh[s];
+                D.mmu_b[i] = mm;
+                D.flags_b[i] = f;
+                D.orig_b[i] = D.orig[s];
+                D.key[i] = D.key_b[i];
+                if (P.verlet) D.xref[i] = p;      // positions the lists are built from
+            } else if (P.dim == 2) {
+                D.vel_b[i] = v;
             }
-        } else {
-            D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
-        }
-        if (P.any_solid) {
-            double sg[6];
-            for (int q = 0; q < 6; ++q) {
-                sg[q] = D.sig[6*s+q];
-                D.sig_b[6*i+q] = sg[q];
-                D.sigh_b[6*i+q] = D.sigh[6*s+q];
-                D.tau_b[6*i+q] = D.tau[6*s+q];
-                D.tauh_b[6*i+q] = D.tauh[6*s+q];
-                D.eps_b[6*i+q] = D.eps[6*s+q];
-                D.epsh_b[6*i+q] = D.epsh[6*s+q];
-            }
+            // key_b[i] stays the cell of slot i at the last sort, the origin of the FP32 positions
+            const ClassParams &c = cp[flag_cls(f)];
             if (flag_free(f)) {
-                // TIR is rebuilt every step; entries the 2-D formula leaves alone are zero
-                double t[6] = {0, 0, 0, 0, 0, 0};
-                if (c.Material > 1 && c.TI > 0.0) tir_tensor(P.dim, sg, c.TI, p.w, t);
-                for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t[q];
+                // sound speed and P/rho^2 of a free particle (CalcForce11 :67-72, :139)
+                double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
+                double pr = v.w/(p.w*p.w);
+                D.aux[i] = make_double4(mm.x, mm.y, C, pr);
+                if (P.mixed32) {
+                    double o0, o1, o2;
+                    cell_origin(P, D.key_b[i], o0, o1, o2);
+                    D.f0[i] = make_float4((float)(p.x - o0), (float)(p.y - o1), (float)(p.z - o2), (float)p.w);
+                    D.f1[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)pr);
+                    D.f2[i] = make_float4((float)C, (float)flag_cls(f), (float)mm.x, (float)(1.0/p.w));
+                } else if (P.fast11) {
+                    D.g0[i] = p;
+                    D.g1[i] = make_double4(v.x, v.y, v.z, pr);
+                    D.g2[i] = make_double4(C, (double)flag_cls(f), mm.x, 1.0/p.w);
+                }
+            } else {
+                D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
+            }
+            if (P.any_solid) {
+                double sg[6];
+                for (int q = 0; q < 6; ++q) {
+                    sg[q] = D.sig[6*s+q];
+                    if (sorted) {
+                        D.sig_b[6*i+q] = sg[q];
+                        D.sigh_b[6*i+q] = D.sigh[6*s+q];
+                        D.tau_b[6*i+q] = D.tau[6*s+q];
+                        D.tauh_b[6*i+q] = D.tauh[6*s+q];
+                        D.eps_b[6*i+q] = D.eps[6*s+q];
+                        D.epsh_b[6*i+q] = D.epsh[6*s+q];
+                    }
+                }
+                if (flag_free(f)) {
+                    // TIR is rebuilt every step; entries the 2-D formula leaves alone are zero
+                    double t[6] = {0, 0, 0, 0, 0, 0};
+                    if (c.Material > 1 && c.TI > 0.0) tir_tensor(P.dim, sg, c.TI, p.w, t);
+                    for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t[q];
+                }
             }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.ctl->reorders += 1;
-        if (sorted) D.ctl->rebuilds += 1;
+        if (sorted) { D.ctl->rebuilds += 1; D.ctl->nwact = 0; }
     }
 }
 
@@ -388,8 +410,9 @@ __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, 
 // ------------------------------------------------------------------------------------------
 // SOLID = false compiles the fluid-only smoothing (no Material 2/3 particles in the run).
 template <bool SOLID>
-__global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
@@ -476,21 +499,6 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D, Params P)
     double2 mm = D.mmu_b[i];
     D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
     D.wal[i] = make_double4(n0, n1, n2, dw);
-    if (P.mixed32) {
-        double o0, o1, o2;
-        cell_origin(P, key_i, o0, o1, o2);
-        D.f0[i] = make_float4((float)(pi.x - o0), (float)(pi.y - o1), (float)(pi.z - o2), (float)dw);
-        D.f1[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)(Pw/(dw*dw)));
-        if (C.NoSlip) D.f2[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), (float)Cw);
-        else D.f2[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)Cw);
-        D.f3[i] = make_float4((float)C.FPMassC, (float)(1.0/dw), 0.0f, (float)ci);
-    } else if (P.fast11) {
-        D.g0[i] = make_double4(pi.x, pi.y, pi.z, dw);
-        D.g1[i] = make_double4(vi.x, vi.y, vi.z, Pw/(dw*dw));
-        if (C.NoSlip) D.g2[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, Cw);
-        else D.g2[i] = make_double4(vi.x, vi.y, vi.z, Cw);
-        D.g3[i] = make_double4(C.FPMassC, 1.0/dw, 0.0, (double)ci);
-    }
     if (SOLID) {
         if (mat > 1) for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = sg[q];
         else for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = 0.0;
@@ -561,12 +569,26 @@ __device__ inline void code_offset(int c, int &dx, int &dy, int &dz)
 // records, per lane, every free candidate within the support radius plus the skin; the steps
 // until the next build walk these sub-lists instead of the cells.
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
-    const int gt = xcd_block()*kBlock + threadIdx.x;
-    const int t = gt >> 2, quad = gt & 3;
-    const bool act = t < P.nfixed;
+    // Between builds only the walls with skin neighbours at the build (the active walls) can
+    // have pairs; the others keep the values the build step wrote.  Active walls are taken in
+    // plain block order, so that their blocks spread over all XCDs.
+    const bool reuse = REACH == 2 && !D.ctl->rebuild;
+    const int gl = (reuse ? (int)blockIdx.x : xcd_block())*kBlock + threadIdx.x;
+    const int quad = gl & 3;
+    bool act;
+    int t;
+    if (reuse) {
+        act = (gl >> 2) < D.ctl->nwact;
+        t = act ? D.wact[gl >> 2] : 0;
+    } else {
+        t = gl >> 2;
+        act = t < P.nfixed;
+    }
+    const int gt = 4*t + quad;                               // the lane's sub-list
     int i = act ? D.fixed_list[t] : 0;
     uint32_t fi = D.flags_b[i];
     const bool noslip = flag_noslip(fi);
@@ -598,7 +620,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
         if (noslip) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
     };
     const size_t gs = 4*(size_t)P.nfixed;                   // sub-list slot group stride
-    if (REACH == 2 && !D.ctl->rebuild) {
+    if (reuse) {
         // between builds: the lane's sub-list, reduced to the reference's pairs
         const int cap = P.wl_cap;
         int c = act ? D.wcnt[gt] : 0;
@@ -654,10 +676,27 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
                 if (adj && in_support(P, r2)) add(j, pj, x0, x1, x2, r2);
             }
         }
-        if (REACH == 2 && act) {
-            if ((wc & 3) != 0 && wc < cap) out[(size_t)(wc >> 2)*gs] = make_int4(w0, w1, w2, w3);
-            D.wcnt[gt] = wc;
-            if (wc > cap) atomicOr(&D.ctl->list_over, 2);
+        if (REACH == 2) {
+            if (act) {
+                if ((wc & 3) != 0 && wc < cap) out[(size_t)(wc >> 2)*gs] = make_int4(w0, w1, w2, w3);
+                D.wcnt[gt] = wc;
+                if (wc > cap) atomicOr(&D.ctl->list_over, 2);
+            }
+            // record the wall as active if any of its four lanes has skin neighbours (one
+            // atomic per wave; lane 4k stands for wall k of the wave)
+            int any = act ? wc : 0;
+            any |= __shfl_xor(any, 1, 64);
+            any |= __shfl_xor(any, 2, 64);
+            const bool rec = act && quad == 0 && any != 0;
+            const unsigned long long m = __ballot(rec);
+            const int lane = threadIdx.x & 63;
+            int base = 0;
+            if (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                if (lane == leader) base = atomicAdd(&D.ctl->nwact, __popcll(m));
+                base = __builtin_amdgcn_readlane(base, leader);
+            }
+            if (rec) D.wact[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
         }
     }
     SK += __shfl_xor(SK, 1, 64); Pw += __shfl_xor(Pw, 1, 64);
@@ -688,24 +727,27 @@ __global__ void __launch_bounds__(kBlock) k_wall_fast(Dev D, Params P)
         cell_origin(P, D.key_b[i], o0, o1, o2);      // the cell of slot i at the last sort
         D.f0[i] = make_float4((float)(pi.x - o0), (float)(pi.y - o1), (float)(pi.z - o2), (float)dw);
         D.f1[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)(Pw/(dw*dw)));
-        if (noslip) D.f2[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), (float)Cw);
-        else D.f2[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, (float)Cw);
-        D.f3[i] = make_float4((float)fpm, (float)(1.0/dw), 0.0f, (float)ci);
+        D.f2[i] = make_float4((float)Cw, (float)(-1 - ci), (float)fpm, (float)(1.0/dw));
+        if (noslip) D.fw[i] = make_float4((float)(2.0*vi.x-n0), (float)(2.0*vi.y-n1), (float)(2.0*vi.z-n2), 0.0f);
+        else D.fw[i] = make_float4((float)vi.x, (float)vi.y, (float)vi.z, 0.0f);
     } else {
         D.g0[i] = make_double4(pi.x, pi.y, pi.z, dw);
         D.g1[i] = make_double4(vi.x, vi.y, vi.z, Pw/(dw*dw));
-        if (noslip) D.g2[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, Cw);
-        else D.g2[i] = make_double4(vi.x, vi.y, vi.z, Cw);
-        D.g3[i] = make_double4(fpm, 1.0/dw, 0.0, (double)ci);
+        D.g2[i] = make_double4(Cw, (double)(-1 - ci), fpm, 1.0/dw);
+        if (noslip) D.gw[i] = make_double4(2.0*vi.x-n0, 2.0*vi.y-n1, 2.0*vi.z-n2, 0.0);
+        else D.gw[i] = make_double4(vi.x, vi.y, vi.z, 0.0);
     }
 }
 
 // ------------------------------------------------------------------------------------------
 // force pass: CalcForce11 / CalcForce2233 gathered per free particle
 // ------------------------------------------------------------------------------------------
-// Per-block partials of a force pass: in-support pair counts (free-free directed, free-fixed)
-// and the minimum dt candidate (bits of a positive double, so unsigned order is double order).
-// k_force_fold combines them; no global atomics in the pass itself.
+// Per-block sums of a force pass, added to the control block with one atomic per quantity and
+// block: in-support pair counts (free-free directed, free-fixed) and the minimum dt candidate
+// (bits of a positive double, so unsigned order is double order).  Integer sums and a minimum
+// are exact in any order, so the result is deterministic.  (A last-block fold of per-block
+// partials would need a device-scope fence per block, which on this chip writes back and
+// invalidates the L2 the gather lives in.)
 __device__ inline void force_partials(const Dev &D, unsigned long long dtb, unsigned nff, unsigned nfw)
 {
     __shared__ unsigned long long red[3][16];
@@ -724,42 +766,10 @@ __device__ inline void force_partials(const Dev &D, unsigned long long dtb, unsi
             sff += red[0][k]; sfw += red[1][k];
             sdt = red[2][k] < sdt ? red[2][k] : sdt;
         }
-        unsigned long long *fp = D.fpart + 3*(size_t)blockIdx.x;
-        fp[0] = sff; fp[1] = sfw; fp[2] = sdt;
-    }
-}
-
-// Folds the per-block partials of the force pass into the control block (one workgroup).
-__global__ void __launch_bounds__(1024) k_force_fold(Dev D, Params P, int nb)
-{
-    Ctl *c = D.ctl;
-    if (c->halt) return;
-    __shared__ unsigned long long red[3][16];
-    unsigned long long sff = 0, sfw = 0, sdt = ~0ull;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        const unsigned long long *fp = D.fpart + 3*(size_t)b;
-        sff += fp[0]; sfw += fp[1];
-        sdt = fp[2] < sdt ? fp[2] : sdt;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        sff += __shfl_down(sff, o, 64);
-        sfw += __shfl_down(sfw, o, 64);
-        unsigned long long t = __shfl_down(sdt, o, 64);
-        sdt = t < sdt ? t : sdt;
-    }
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][w] = sff; red[1][w] = sfw; red[2][w] = sdt; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-            sff += red[0][k]; sfw += red[1][k];
-            sdt = red[2][k] < sdt ? red[2][k] : sdt;
-        }
-        c->pairs_ff += sff;
-        c->pairs_fw += sfw;
-        if (sdt < c->dtmin_bits) c->dtmin_bits = sdt;
-        c->nl_over += c->nl_q;
-        c->nl_q = 0;
+        Ctl *c = D.ctl;
+        if (sff) atomicAdd(&c->pairs_ff, sff);
+        if (sfw) atomicAdd(&c->pairs_fw, sfw);
+        if (sdt != ~0ull) atomicMin(&c->dtmin_bits, sdt);
     }
 }
 
@@ -987,8 +997,9 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
 
 // T = float (SOLID only): the FP32-mixed soil tier; gravity is added in FP64 afterwards.
 template <bool SOLID, class T>
-__global__ void __launch_bounds__(kForceBlock) k_force(Dev D, Params P)
+__global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     __shared__ ClassParams cp[kMaxClasses];
     __shared__ int lst[kCap*kForceBlock];
@@ -1122,13 +1133,15 @@ __device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, 
 struct Acc11 { double a0, a1, a2, dd; };
 
 // SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
-// current positions (skin_member); i is the target (for its cell key).
+// current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
+// target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
 template <int DIM, bool SKIN>
 __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
-                                   double4 gi0, double4 gi1, double4 gi2, double4 gi3, int ci, int i, int j,
+                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
                                    double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
-    double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j], a3 = D.g3[j];
+    double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
+    const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
     double r2 = x0*x0 + x1*x1 + x2*x2;
     if (SKIN && !skin_member(D, P, i, j, r2)) return;
@@ -1147,10 +1160,12 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
         K = 0.0; GK = 0.0;
     }
     (void)K;
-    bool freej = a3.z != 0.0;
+    const bool freej = meta >= 0.0;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    int cj = (int)a3.w;
+    int cj = freej ? (int)meta : (int)(-1.0 - meta);
+    double w0 = a1.x, w1 = a1.y, w2 = a1.z;              // viscous velocity of the neighbour
+    if (!freej) { double4 vw = D.gw[j]; w0 = vw.x; w1 = vw.y; w2 = vw.z; }
     double alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
     double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     double dvx = u0*x0 + u1*x1 + u2*x2;
@@ -1158,19 +1173,19 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     double PIij = 0.0;
     if ((alpha != 0.0 || beta != 0.0) && dvx < 0.0) {
         double MU = P.h_uniform*dvx*rcp_nr(r2 + P.h2c);
-        PIij = (-alpha*0.5*(gi2.w + a2.w)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
+        PIij = (-alpha*0.5*(gi2.x + cj_snd)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
     }
-    double mi = gi3.x;
-    double mj = freej ? a3.x : a3.x*mi;
+    double mi = gi2.z;
+    double mj = freej ? mrec : mrec*mi;
     double mu = freej ? mu_ff : mu_w;
-    double vs = 2.0*mu*gi3.y*a3.y*GK;
+    double vs = 2.0*mu*gi2.w*invdj*GK;
     double s = -(gi1.w + a1.w + PIij)*GK;
-    double b0 = gi1.x - a2.x, b1 = gi1.y - a2.y, b2 = gi1.z - a2.z;
+    double b0 = gi1.x - w0, b1 = gi1.y - w1, b2 = gi1.z - w2;
     double t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
     if (DIM == 2) t2 = 0.0;
     double temp1 = (u0*x0 + u1*x1 + u2*x2)*GK;
     A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
-    A.dd += mj*(di*a3.y)*temp1;
+    A.dd += mj*(di*invdj)*temp1;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1269,8 +1284,9 @@ struct ListWriter {
 // only): the skin list, every candidate within the support radius plus the skin; one that
 // overflows makes every step rebuild until the host has grown the lists.
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_nlist(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     if (REACH == 2 && !D.ctl->rebuild) return;
     int t = xcd_block()*kBlock + threadIdx.x;
@@ -1294,8 +1310,9 @@ __global__ void __launch_bounds__(kBlock) k_nlist(Dev D, Params P)
 
 // FP64 fluid force pass over the neighbour lists.
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
+__global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     __shared__ double pa[64], pb[64];
     if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
@@ -1305,8 +1322,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
     unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
     bool act = t < P.nfree;
     int i = act ? D.free_list[t] : 0;
-    double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
-    int ci = (int)gi3.w;
+    double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i];
+    int ci = (int)gi2.y;
     const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     Acc11 A;
     A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
@@ -1322,8 +1339,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
         int sl = e & 3;
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
-        pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, codes ? nl_index<REACH>(v) : v,
-                                     mu_ff, mu_w, A, nff, nfw);
+        pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, codes ? nl_index<REACH>(v) : v,
+                                         mu_ff, mu_w, A, nff, nfw);
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_nl_over continues this target from these sums
@@ -1347,8 +1364,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D, Params P)
 // on steps that build them (a reused list that overflowed makes every step rebuild), so the
 // cell table is current.
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P)
+__global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     const int nq = D.ctl->nl_q;
     if (nq == 0) return;
@@ -1362,8 +1380,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P
     for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
         int t = D.ovq[q];
         int i = D.free_list[t];
-        double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i], gi3 = D.g3[i];
-        int ci = (int)gi3.w;
+        double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i];
+        int ci = (int)gi2.y;
         double4 s0 = D.ovs[2*q];
         Acc11 A;
         A.a0 = s0.x; A.a1 = s0.y; A.a2 = s0.z; A.dd = s0.w;
@@ -1373,7 +1391,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D, Params P
         int k = 0;
         walk_rows<DIM, REACH>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, cull2, [&](int j, int, double r2) {
             if (!((REACH == 2) ? (r2 <= reach2) : in_support(P, r2))) return;      // a list entry
-            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, mu_ff, mu_w, A, nff, nfw);
+            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, mu_ff, mu_w, A, nff, nfw);
             ++k;
         });
         D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
@@ -1422,12 +1440,13 @@ __device__ inline bool skin_member32(const Dev &D, const Params &P, const Band32
 // current positions (skin_member32); i is the target.
 template <int DIM, bool SKIN>
 __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
-                                  float4 gi0, float4 gi1, float4 gi2, float4 gi3, int ci, int i, int j,
+                                  float4 gi0, float4 gi1, float4 gi2, int ci, int i, int j,
                                   float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
                                   float kw, float kg, float kgh, float invh, float h, float h2c,
                                   float mu_ff, float mu_w, const Band32 &band)
 {
-    float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j], a3 = D.f3[j];
+    float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
+    const float cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     float x0 = (gi0.x - a0.x) - o0, x1 = (gi0.y - a0.y) - o1, x2 = (gi0.z - a0.z) - o2;
     float r2 = x0*x0 + x1*x1 + x2*x2;
     if (SKIN && !skin_member32(D, P, band, i, j, r2)) return;
@@ -1444,10 +1463,12 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
         GK = 0.0f;
     }
     (void)kw;
-    bool freej = a3.z != 0.0f;
+    const bool freej = meta >= 0.0f;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    int cj = (int)a3.w;
+    int cj = freej ? (int)meta : (int)(-1.0f - meta);
+    float w0 = a1.x, w1 = a1.y, w2 = a1.z;               // viscous velocity of the neighbour
+    if (!freej) { float4 vw = D.fw[j]; w0 = vw.x; w1 = vw.y; w2 = vw.z; }
     float alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
     float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     float dvx = u0*x0 + u1*x1 + u2*x2;
@@ -1455,19 +1476,19 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
     float PIij = 0.0f;
     if ((alpha != 0.0f || beta != 0.0f) && dvx < 0.0f) {
         float MU = h*dvx*__builtin_amdgcn_rcpf(r2 + h2c);
-        PIij = (-alpha*0.5f*(gi2.w + a2.w)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
+        PIij = (-alpha*0.5f*(gi2.x + cj_snd)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
     }
-    float mi = gi3.x;
-    float mj = freej ? a3.x : a3.x*mi;
+    float mi = gi2.z;
+    float mj = freej ? mrec : mrec*mi;
     float mu = freej ? mu_ff : mu_w;
-    float vs = 2.0f*mu*gi3.y*a3.y*GK;
+    float vs = 2.0f*mu*gi2.w*invdj*GK;
     float s = -(gi1.w + a1.w + PIij)*GK;
-    float b0 = gi1.x - a2.x, b1 = gi1.y - a2.y, b2 = gi1.z - a2.z;
+    float b0 = gi1.x - w0, b1 = gi1.y - w1, b2 = gi1.z - w2;
     float t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
     if (DIM == 2) t2 = 0.0f;
     A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
     // Kahan-compensated continuity sum
-    float c = mj*(di*a3.y)*(dvx*GK);
+    float c = mj*(di*invdj)*(dvx*GK);
     float y = c - A.ddc;
     float tt = A.dd + y;
     A.ddc = (tt - A.dd) - y;
@@ -1478,8 +1499,9 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
 // offset dk CellSize of the cell-relative FP32 positions (each relative to the particle's cell
 // at the last sort).
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
+__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     __shared__ float pa[64], pb[64];
     if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
@@ -1489,8 +1511,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
     unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
     bool act = t < P.nfree;
     int i = act ? D.free_list[t] : 0;
-    float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
-    int ci = (int)gi3.w;
+    float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
+    int ci = (int)gi2.y;
     const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
     const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
@@ -1507,7 +1529,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
         int dx, dy, dz;
         code_offset<REACH>(c, dx, dy, dz);
         float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
-        pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, p0, p1, p2,
+        pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
                                     A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
     };
     int4 w = make_int4(0, 0, 0, 0);
@@ -1538,8 +1560,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D, Params P)
 // Ends the overflowed targets of the FP32 list pass (see k_force11_nl_over); the queued FP32
 // sums are exact in FP64 and restored as FP32.
 template <int DIM, int REACH>
-__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Params P)
+__global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     if (D.ctl->halt) return;
     const int nq = D.ctl->nl_q;
     if (nq == 0) return;
@@ -1556,8 +1579,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Para
     for (int q = blockIdx.x*kForceBlock + threadIdx.x; q < nq; q += gridDim.x*kForceBlock) {
         int t = D.ovq[q];
         int i = D.free_list[t];
-        float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i], gi3 = D.f3[i];
-        int ci = (int)gi3.w;
+        float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
+        int ci = (int)gi2.y;
         double4 s0 = D.ovs[2*q], s1 = D.ovs[2*q+1];
         Acc32 A;
         A.a0 = (float)s0.x; A.a1 = (float)s0.y; A.a2 = (float)s0.z; A.dd = (float)s0.w; A.ddc = (float)s1.x;
@@ -1572,7 +1595,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D, Para
                 int dx, dy, dz;
                 code_offset<REACH>(c, dx, dy, dz);
                 float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
-                pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, gi3, ci, i, j, p0, p1, p2,
+                pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
                                             A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
             }
             ++k;
@@ -1695,8 +1718,9 @@ __device__ inline void dp_return(const ClassParams &c, double i22, double *St, c
 
 // SOLID = false compiles the fluid-only update (no Material 2/3 particles in the run).
 template <bool SOLID>
-__global__ void __launch_bounds__(kBlock) k_integrate(Dev D, Params P)
+__global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
 {
+    const Dev D = bind(D0);
     Ctl *ctl = D.ctl;
     if (ctl->halt) return;
     if (!ctl->resume && ctl->time >= ctl->tout) return;       // output stop before Move
@@ -1895,6 +1919,8 @@ __global__ void k_tail(Dev D, Params P)
 {
     Ctl *c = D.ctl;
     if (c->halt) return;
+    c->nl_over += c->nl_q;                 // the force pass's overflow queue has been ended
+    c->nl_q = 0;
     c->last_pairs = c->pairs_ff/2 + c->pairs_fw;
     double deltatmin = bitsd(c->dtmin_bits);
     c->deltatmin = deltatmin;
@@ -1907,6 +1933,8 @@ __global__ void k_tail(Dev D, Params P)
     c->steps_done += 1;
     c->resume = 0;
     c->mid_step = 0;
+    // the step's working copy is the next step's state
+    c->side ^= c->rebuild;
     if (P.verlet) {
         c->rebuild = (c->need_rebuild || c->list_over || P.verlet_force) ? 1 : 0;
         c->need_rebuild = 0;
@@ -1944,9 +1972,10 @@ __global__ void k_head(Dev D, Params P)
 // Domain.cpp:808-945 + the rij/h <= Cellfac test of CalcForce11/2233).  Pass 0 counts pairs
 // per particle, pass 1 writes them at offsets from an exclusive scan of the counts.
 // ------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_pairs(Dev D, Params P, int pass, const long long *off,
+__global__ void __launch_bounds__(kBlock) k_pairs(Dev D0, Params P, int pass, const long long *off,
                                                   int *cnt_out, long long *out)
 {
+    const Dev D = bind(D0);
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
     __syncthreads();
@@ -2040,7 +2069,6 @@ void launch_force(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfree <= 0) return;
     launch_force_pass(D, P, s);
-    hipLaunchKernelGGL(k_force_fold, dim3(1), dim3(1024), 0, s, D, P, nblk(P.nfree, kForceBlock));
 }
 
 // The list passes of the fast fluid path: build (every step, or with list reuse on the steps
