@@ -448,9 +448,13 @@ static int forward(psph_gnn *g)
         uint16_t *hin = g->H[g->slot(l)], *hout = g->H[g->train ? l + 1 : ((l + 1) & 1)];
         uint16_t *ein = g->E[g->slot(l)], *eout = g->E[g->train ? l + 1 : ((l + 1) & 1)];
         uint16_t *ze = g->ZE[g->train ? l : 0], *ag = g->AG[g->train ? l : 0], *zn = g->ZN[g->train ? l : 0];
-        if (ne > 0) {
+        if (ne > 0 && g->train) {
+            // training keeps the first layer's activation for the backward pass
             gemm_fwd_edgecat(s, ne, ein, hin, g->recv, g->send, g->Wt(g->pe1(l)), g->b(g->pe1(l)), ze);
             gemm_fwd_resid(s, ne, ze, g->Wt(g->pe2(l)), g->b(g->pe2(l)), ein, eout);
+        } else if (ne > 0) {
+            edge_mlp_fused(s, ne, ein, hin, g->recv, g->send, g->Wt(g->pe1(l)), g->b(g->pe1(l)),
+                           g->Wt(g->pe2(l)), g->b(g->pe2(l)), eout);
         }
         segsum(s, n, g->rp, eout, ag);
         gemm_fwd_nodecat(s, n, hin, ag, g->Wt(g->pn1(l)), g->b(g->pn1(l)), zn);

@@ -31,6 +31,9 @@ void gemm_fwd_nodecat(hipStream_t s, int M, const uint16_t *H, const uint16_t *A
                       const float *b, uint16_t *Y);
 void gemm_fwd_resid(hipStream_t s, int M, const uint16_t *Z, const uint16_t *Wt, const float *b,
                     const uint16_t *R, uint16_t *Y);
+void edge_mlp_fused(hipStream_t s, int M, const uint16_t *E, const uint16_t *H, const int32_t *recv,
+                    const int32_t *send, const uint16_t *W1t, const float *b1, const uint16_t *W2t,
+                    const float *b2, uint16_t *Eout);
 void gemm_fwd_out(hipStream_t s, int M, const uint16_t *Z, const uint16_t *Wt, const float *b, int dout, float *O);
 void gemm_bwd_mask(hipStream_t s, int M, const uint16_t *dY, const uint16_t *W, const uint16_t *Z, uint16_t *dZ);
 void gemm_bwd_input(hipStream_t s, int M, int kin, const uint16_t *dY, const uint16_t *W, float *dX);

@@ -229,6 +229,145 @@ struct EpF32 {            // D = acc (FP32), leading dimension ld
     __device__ void store(int r, int c, float v) const { D[(size_t)r*ld + c] = v; }
 };
 
+// ---- fused edge update (inference) -------------------------------------------------------
+// e' = e + W2^T relu(W1^T [e | h_recv | h_send] + b1) + b2 for a tile of 128 edges: the first
+// layer's 128 x 128 activation tile stays in LDS as the second layer's A operand, so it never
+// goes to HBM.  Stage k+1 of the first layer's operands is loaded into registers while stage k
+// is multiplied (double-buffered LDS, one barrier per stage).  The result is staged through
+// LDS as FP32 and written with 16-byte row-contiguous stores, the residual read the same way.
+// The operations and their order are those of the unfused pair k_gemm<LdEdgeCat, ., EpBias> +
+// k_gemm<LdRows, ., EpResid>, so the result is the same to the bit.
+constexpr int ZLD = 128 + 8;           // LDS row stride (bf16) of a 128-wide tile
+constexpr int kEdgeStage = 2*BM*LDK + 2*BN*LDK;   // phase 1: A, B double-buffered
+constexpr int kEdgeZW = 2*128*ZLD;                 // phase 2: Z tile and W2^T
+constexpr int kEdgeOut = 2*128*128;                // epilogue: 128 x 128 FP32
+constexpr int kEdgeLds = kEdgeStage > kEdgeZW ? (kEdgeStage > kEdgeOut ? kEdgeStage : kEdgeOut)
+                                              : (kEdgeZW > kEdgeOut ? kEdgeZW : kEdgeOut);
+
+__global__ void __launch_bounds__(256) k_edge_mlp(int M, LdEdgeCat al, const uint16_t *W1t, const float *b1,
+                                                  const uint16_t *W2t, const float *b2, uint16_t *Eout)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t lds[kEdgeLds];
+    // the three phases reuse one LDS region: staging, then Z and W2^T, then the FP32 result
+    uint16_t *As = lds, *Bs = lds + 2*BM*LDK;          // phase 1 staging, [buf][row][k]
+    uint16_t *Zs = lds, *Ws = lds + 128*ZLD;            // phase 2: Z [row][k], W2^T [n][k]
+    float *Os = reinterpret_cast<float *>(lds);         // epilogue: FP32 [row][128]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.x*BM;
+    const LdRows bl{W1t, 384};
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
+    // phase 1: K = 384 in 12 stages of 32
+    uint4 ra[2], rb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            int r = (tid >> 2) + 64*p, kc = (tid & 3)*8;
+            ra[p] = (m0 + r < M) ? al.load8(m0 + r, k0 + kc) : make_uint4(0, 0, 0, 0);
+            rb[p] = bl.load8(r, k0 + kc);
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            int r = (tid >> 2) + 64*p, kc = (tid & 3)*8;
+            *(uint4 *)&As[(buf*BM + r)*LDK + kc] = ra[p];
+            *(uint4 *)&Bs[(buf*BN + r)*LDK + kc] = rb[p];
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int st = 0; st < 12; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < 12) fetch((st + 1)*BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                int rA = wm*64 + t*32 + (lane & 31);
+                a[t] = *(const bf16x8 *)&As[(buf*BM + rA)*LDK + ks + (lane >> 5)*8];
+                int rB = wn*64 + t*32 + (lane & 31);
+                b[t] = *(const bf16x8 *)&Bs[(buf*BN + rB)*LDK + ks + (lane >> 5)*8];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (st + 1 < 12) stash(buf ^ 1);
+        __syncthreads();
+    }
+    // Z = bf16(relu(acc + b1)) into LDS; W2^T into the staging space
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                int r = wm*64 + i*32 + 8*(q >> 2) + 4*(lane >> 5) + (q & 3);
+                int c = wn*64 + j*32 + (lane & 31);
+                float v = acc[i][j][q] + b1[c];
+                v = v > 0.0f ? v : 0.0f;
+                Zs[r*ZLD + c] = f2bf(v);
+                acc[i][j][q] = 0.0f;
+            }
+    for (int e = tid; e < 128*16; e += 256) {
+        int n = e >> 4, kc = (e & 15)*8;
+        *(uint4 *)&Ws[n*ZLD + kc] = *(const uint4 *)(W2t + (size_t)n*128 + kc);
+    }
+    __syncthreads();
+    // phase 2: K = 128 from LDS
+#pragma unroll
+    for (int k0 = 0; k0 < 128; k0 += 16) {
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            int rA = wm*64 + t*32 + (lane & 31);
+            a[t] = *(const bf16x8 *)&Zs[rA*ZLD + k0 + (lane >> 5)*8];
+            int rB = wn*64 + t*32 + (lane & 31);
+            b[t] = *(const bf16x8 *)&Ws[rB*ZLD + k0 + (lane >> 5)*8];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+    // epilogue: acc (FP32) through LDS, then row-contiguous residual read and store
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                int r = wm*64 + i*32 + 8*(q >> 2) + 4*(lane >> 5) + (q & 3);
+                int c = wn*64 + j*32 + (lane & 31);
+                Os[r*128 + c] = acc[i][j][q];
+            }
+    __syncthreads();
+    for (int e = tid; e < 128*16; e += 256) {
+        int r = e >> 4, c0 = (e & 15)*8;
+        if (m0 + r >= M) continue;
+        size_t o = (size_t)(m0 + r)*128 + c0;
+        uint4 ev = *(const uint4 *)(al.E + o);
+        const uint16_t *eh = reinterpret_cast<const uint16_t *>(&ev);
+        uint16_t out[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = f2bf(bf2f(eh[k]) + Os[r*128 + c0 + k] + b2[c0 + k]);
+        *(uint4 *)(Eout + o) = *reinterpret_cast<const uint4 *>(out);
+    }
+}
+
 // ---- elementwise / graph kernels ----------------------------------------------------------
 
 // column sums of a bf16 [M][128] matrix into db (FP32 atomics per block of rows)
@@ -603,6 +742,13 @@ void gemm_fwd_nodecat(hipStream_t s, int M, const uint16_t *H, const uint16_t *A
     dim3 g(nb(M, BM), 1);
     hipLaunchKernelGGL((k_gemm<LdNodeCat, LdRows, EpBias>), g, dim3(256), 0, s, M, 128, 256,
                        LdNodeCat{H, A}, LdRows{Wt, 256}, EpBias{Y, 128, b, 1});
+}
+void edge_mlp_fused(hipStream_t s, int M, const uint16_t *E, const uint16_t *H, const int32_t *recv,
+                    const int32_t *send, const uint16_t *W1t, const float *b1, const uint16_t *W2t,
+                    const float *b2, uint16_t *Eout)
+{
+    hipLaunchKernelGGL(k_edge_mlp, dim3(nb(M, BM)), dim3(256), 0, s, M, LdEdgeCat{E, H, recv, send}, W1t, b1,
+                       W2t, b2, Eout);
 }
 void gemm_fwd_resid(hipStream_t s, int M, const uint16_t *Z, const uint16_t *Wt, const float *b,
                     const uint16_t *R, uint16_t *Y)

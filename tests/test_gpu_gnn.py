@@ -74,6 +74,28 @@ def test_forward_matches_reference(dim, layers):
     g.close()
 
 
+@pytest.mark.parametrize("dim,layers", [(2, 2), (3, 3)])
+def test_fused_inference_equals_training_forward(dim, layers):
+    """Inference runs each edge update as one fused kernel (the first layer's activations stay
+    in LDS); training runs the two GEMMs apart and keeps the activations.  Same operations in
+    the same order: the predictions agree to the bit, and both meet the reference's bar."""
+    n = 300
+    x, rp, cols, recv, X, U = _inputs(n, dim, 0.6, 5)
+    print(f"[gnn fused] dim={dim} layers={layers} n={n} edges={len(cols)}", flush=True)
+    gt = _net(dim, layers, train=True)
+    gi = _net(dim, layers, train=False)
+    gi.set_params(gt.get_params())
+    for g in (gt, gi):
+        g.load_inputs(X, U, rp, cols)
+    yt, yi = gt.predict(), gi.predict()
+    np.testing.assert_array_equal(yi, yt)
+    p = gt.get_params().astype(np.float64)
+    y0 = R.forward(p, layers, _bf16(X), _bf16(U), recv, cols.astype(np.int64), 2*dim + 1)
+    assert np.linalg.norm(yi - y0)/np.linalg.norm(y0) < 3e-2
+    gt.close()
+    gi.close()
+
+
 @pytest.mark.parametrize("dim,layers", [(2, 2), (3, 2)])
 def test_gradients_match_reference(dim, layers):
     n = 250
