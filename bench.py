@@ -153,6 +153,16 @@ def main():
                 "algorithmic_bytes_per_launch": int(force_b)}
         log("[bench] kernel means (ms): " + ", ".join(f"{k}={v[0]:.4f}" for k, v in stats.items() if v[1]))
     pair_rate = st.pairs*args.steps/el
+    # one visualisation output of the current state: the fields WriteXDMF writes, packed to FP32
+    # on the GPU and copied to pinned host memory, against the full FP64 state download
+    t0 = time.perf_counter()
+    s.output_begin()
+    s.output_end(copy=False)
+    packed_ms = (time.perf_counter() - t0)*1e3
+    t0 = time.perf_counter()
+    s.particles()
+    download_ms = (time.perf_counter() - t0)*1e3
+    log(f"[bench] output: packed FP32 {packed_ms:.2f} ms, full download {download_ms:.1f} ms")
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -187,6 +197,8 @@ def main():
         "algorithmic_bytes_per_step": total_b,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "output": {"packed_f32_ms": round(packed_ms, 3), "full_download_ms": round(download_ms, 2),
+                   "packed_bytes": 104*n},
     }
     print(json.dumps(out))
 

@@ -24,7 +24,7 @@ extern "C" {
 /* ---- status codes (0 = ok, negative = error; message via psph_last_error) ---- */
 #define PSPH_OK               0
 #define PSPH_ERR_INVALID     -1  /* bad argument or enum value (reference: abort() in Functions.cpp switches) */
-#define PSPH_ERR_UNSUPPORTED -2  /* feature outside this path (periodic/in-out-flow BCs, FSI, multi-material pairs) */
+#define PSPH_ERR_UNSUPPORTED -2  /* feature outside this path (in-out-flow BCs, FSI, multi-material pairs) */
 #define PSPH_ERR_HIP         -3  /* HIP runtime failure */
 #define PSPH_ERR_TIMESTEP    -4  /* adaptive dt collapsed below deltatint/1e5 (Domain.cpp:150-151) */
 #define PSPH_ERR_CFL         -5  /* user dt above the CFL/viscous limit (Domain.cpp:1721-1733) */
@@ -52,7 +52,7 @@ typedef struct psph_config {
     int32_t gradient_type;   /* Gradient_Approach_Set: 0 squared density, 1 multiplied density */
     int32_t scheme;          /* Domain::Scheme: 0 modified Verlet, 1 Leapfrog */
     int32_t precision;       /* PSPH_FP64 or PSPH_FP32_MIXED */
-    int32_t periodic[3];     /* Boundary::Periodic (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    int32_t periodic[3];     /* Boundary::Periodic (z only in 3-D; needs 3+ cells across a period) */
     int32_t in_out_flow;     /* Boundary::InOutFlow (nonzero -> PSPH_ERR_UNSUPPORTED) */
     int32_t swi_type;        /* Domain::SWIType (only the single-material meaning is used) */
     int32_t fsi;             /* Domain::FSI (nonzero -> PSPH_ERR_UNSUPPORTED) */
@@ -139,6 +139,39 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
 
 /* Copies the particle state back in reference order (cap = capacity of p). */
 int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n);
+
+/* Device form of the body-force hook of 1-Poiseuille.cpp:23-29 (GeneralBefore = UserAcc, which
+ * adds a constant to the acceleration of every free particle after StartAcceleration and
+ * before the pair sums, Domain.cpp:1766-1770): from the next force pass on, every free
+ * particle's acceleration starts at Gravity + accel.  Kept across psph_upload; zero by default.
+ * Arbitrary GeneralBefore/GeneralAfter hooks need host state mid-step and are not offered. */
+int psph_set_body_force(psph_handle *h, const double accel[3]);
+
+/* The fields Domain::WriteXDMF (Domain.cpp:1869-1964) converts to float for its HDF5 file, in
+ * reference particle order, laid out as its datasets are: Position, Velocity, Acceleration
+ * (3 per particle), Pressure, Density, Mass, h, Tag (= Particle::ID), Sigma and Strain (6 per
+ * particle: xx xy xz yy yz zz).  Each value is the FP64 state psph_download would return,
+ * rounded to nearest float.  Fixed particles' Acceleration is Gravity (their pair sums are not
+ * formed; they feed nothing).  The pointers belong to the library and stay valid until the
+ * next psph_output_begin, psph_upload or psph_destroy. */
+typedef struct psph_output_f32 {
+    size_t n;
+    const float   *position, *velocity, *acceleration;
+    const float   *pressure, *density, *mass, *h;
+    const int32_t *tag;
+    const float   *sigma, *strain;
+} psph_output_f32;
+
+/* Starts an output of the current state (at a PSPH_STOP_OUTPUT stop, the mid-step state the
+ * reference writes at Domain.cpp:1773-1779): packs the fields to FP32 on the GPU and queues
+ * their copy to pinned host memory on the solver's stream, then returns without waiting.
+ * Later psph_run calls may be made before psph_output_end; they run after the copy.
+ * Replaces the conversion loop of WriteXDMF (Domain.cpp:1893-1928) and the full state
+ * download an output needs without it.  One output may be pending at a time. */
+int psph_output_begin(psph_handle *h);
+
+/* Waits for the pending output's copy and returns the packed fields. */
+int psph_output_end(psph_handle *h, psph_output_f32 *out);
 
 /* Test hook: the neighbour pair set of the current positions, as the reference's
  * MainNeighbourSearch + the rij/h <= Cellfac test in CalcForce11/2233 (Interaction.cpp:32,195)

@@ -663,7 +663,7 @@ static int move_leapfrog(psph_particle *p, double dt, double i22)
     return 0;
 }
 
-/* Particle::Move, Particle.cpp:119-144 (periodic wrap out of scope) */
+/* Particle::Move, Particle.cpp:119-144 (the periodic wrap is in domain_move) */
 int psph_o_particle_move(psph_particle *p, double dt, int scheme, double i22)
 {
     return scheme == 0 ? move_mverlet(p, dt, i22) : move_leapfrog(p, dt, i22);
@@ -710,11 +710,26 @@ struct psph_odomain {
     double I22;
     int mid_step;
     int64_t last_pairs;
+    double DomSize[3];          /* periodic length per direction, 0 where not periodic */
+    double body[3];             /* body-force hook: added to every free particle's a */
 };
 
 #define HOCI(d,i,j,k) ((d)->HOC[((size_t)(i)*(d)->CellNo[1] + (size_t)(j))*(d)->CellNo[2] + (size_t)(k)])
 
-/* CellInitiate, Domain.cpp:549-660 (periodic branches out of scope) */
+/* Periodic_X_Correction, Domain.cpp:110-115: a component longer than 2 Cellfac h crosses the
+ * periodic boundary and is brought back by the domain length, towards the particle in the
+ * lower cell */
+static inline void periodic_correction(const psph_odomain *d, double *x, double h, const OP *o1, const OP *o2)
+{
+    double Cellfac = d->cfg.cellfac;
+    for (int c = 0; c < 3; ++c)
+        if (d->DomSize[c] > 0.0)
+            if (x[c] > 2*Cellfac*h || x[c] < -2*Cellfac*h)
+                x[c] = (o1->CC[c] > o2->CC[c]) ? x[c] - d->DomSize[c] : x[c] + d->DomSize[c];
+}
+
+/* CellInitiate, Domain.cpp:549-660: a periodic direction takes the margin R instead of hmax/2,
+ * two more cell layers (aliases of the first two, filled by ListGenerate) and its DomSize */
 static int cell_initiate(psph_odomain *d)
 {
     OP **P = d->P;
@@ -739,7 +754,10 @@ static int cell_initiate(psph_odomain *d)
         if (d->cfg.dom_max[c] > d->TRPR[c]) d->TRPR[c] = d->cfg.dom_max[c];
         if (d->cfg.dom_min[c] < d->BLPF[c]) d->BLPF[c] = d->cfg.dom_min[c];
     }
-    for (int c = 0; c < 3; ++c) { d->TRPR[c] += d->hmax/2; d->BLPF[c] -= d->hmax/2; }
+    for (int c = 0; c < 3; ++c) {
+        if (!d->cfg.periodic[c]) { d->TRPR[c] += d->hmax/2; d->BLPF[c] -= d->hmax/2; }
+        else { d->TRPR[c] += d->cfg.R; d->BLPF[c] -= d->cfg.R; }
+    }
     double Cellfac = d->cfg.cellfac;
     int dims = d->cfg.dimension;
     for (int c = 0; c < dims; ++c) {
@@ -755,6 +773,10 @@ static int cell_initiate(psph_odomain *d)
         d->CellSize[2] = 0.0;
     } else {
         for (int c = 0; c < 3; ++c) d->CellSize[c] = (d->TRPR[c]-d->BLPF[c])/d->CellNo[c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        d->DomSize[c] = 0.0;
+        if (d->cfg.periodic[c]) { d->CellNo[c] += 2; d->DomSize[c] = (d->TRPR[c]-d->BLPF[c]); }
     }
     size_t nc = (size_t)d->CellNo[0]*d->CellNo[1]*d->CellNo[2];
     free(d->HOC);
@@ -785,6 +807,26 @@ static void list_generate(psph_odomain *d)
         o->CC[0] = ijk[0]; o->CC[1] = ijk[1]; o->CC[2] = ijk[2];
         if (!o->p.IsFree) d->fixed[d->nfixed++] = a;
     }
+    /* periodic directions: the last two cell layers alias the first two (Domain.cpp:757-783) */
+    int *N = d->CellNo;
+    if (d->cfg.periodic[0])
+        for (int j = 0; j < N[1]; j++)
+        for (int k = 0; k < N[2]; k++) {
+            HOCI(d, N[0]-1, j, k) = HOCI(d, 1, j, k);
+            HOCI(d, N[0]-2, j, k) = HOCI(d, 0, j, k);
+        }
+    if (d->cfg.periodic[1])
+        for (int i = 0; i < N[0]; i++)
+        for (int k = 0; k < N[2]; k++) {
+            HOCI(d, i, N[1]-1, k) = HOCI(d, i, 1, k);
+            HOCI(d, i, N[1]-2, k) = HOCI(d, i, 0, k);
+        }
+    if (d->cfg.periodic[2])
+        for (int i = 0; i < N[0]; i++)
+        for (int j = 0; j < N[1]; j++) {
+            HOCI(d, i, j, N[2]-1) = HOCI(d, i, j, 1);
+            HOCI(d, i, j, N[2]-2) = HOCI(d, i, j, 0);
+        }
 }
 
 /* CellReset, Domain.cpp:786-806 */
@@ -814,12 +856,14 @@ static void chain(psph_odomain *d, int T, int t1, int head)
     for (int t2 = head; t2 != -1; t2 = d->P[t2]->LL) classify(d, T, t1, t2);
 }
 
-/* YZPlaneCellsNeighbourSearch, Domain.cpp:824-945 */
+/* YZPlaneCellsNeighbourSearch, Domain.cpp:824-945 (home cells of a periodic direction skip
+ * the first layer and the last alias layer) */
 static void yz_plane_search(psph_odomain *d, int q1, int T)
 {
     int *N = d->CellNo;
-    for (int q3 = 0; q3 < N[2]; q3++)
-    for (int q2 = 0; q2 < N[1]; q2++) {
+    const int *per = d->cfg.periodic;
+    for (int q3 = per[2] ? 1 : 0; per[2] ? (q3 < N[2]-1) : (q3 < N[2]); q3++)
+    for (int q2 = per[1] ? 1 : 0; per[1] ? (q2 < N[1]-1) : (q2 < N[1]); q2++) {
         if (HOCI(d, q1, q2, q3) == -1) continue;
         for (int t1 = HOCI(d, q1, q2, q3); t1 != -1; t1 = d->P[t1]->LL) {
             chain(d, T, t1, d->P[t1]->LL);
@@ -838,9 +882,9 @@ static void yz_plane_search(psph_odomain *d, int q1, int T)
 /* MainNeighbourSearch, Domain.cpp:808-822 */
 static void main_neighbour_search(psph_odomain *d)
 {
-    int nx = d->CellNo[0];
+    int nx = d->CellNo[0], q0 = d->cfg.periodic[0] ? 1 : 0, q1e = d->cfg.periodic[0] ? nx - 1 : nx;
     #pragma omp parallel for schedule(dynamic) num_threads(d->nthreads)
-    for (int q1 = 0; q1 < nx; q1++) yz_plane_search(d, q1, omp_get_thread_num());
+    for (int q1 = q0; q1 < q1e; q1++) yz_plane_search(d, q1, omp_get_thread_num());
 }
 
 /* StartAcceleration, Domain.cpp:947-1018 */
@@ -886,6 +930,7 @@ static void primary_compute_acceleration(psph_odomain *d)
             psph_particle *P1 = &o1->p, *P2 = &o2->p;
             double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
             double h = (P1->h+P2->h)/2.0;
+            periodic_correction(d, xij, h, o1, o2);
             double K = psph_o_kernel(dim, KT, norm3(xij)/h, h);
             OP *ow = P1->IsFree ? o2 : o1;       /* the fixed one */
             psph_particle *W = &ow->p, *F = P1->IsFree ? P1 : P2;
@@ -936,6 +981,7 @@ static void calc_force11(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     double Cellfac = d->cfg.cellfac;
     double h = (P1->h+P2->h)/2;
     double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+    periodic_correction(d, xij, h, o1, o2);
     double rij = norm3(xij);
     if (!((rij/h) <= Cellfac)) return;
     (*cnt)++;
@@ -1063,6 +1109,7 @@ static void calc_force2233(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     double Cellfac = d->cfg.cellfac;
     double h = (P1->h+P2->h)/2;
     double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+    periodic_correction(d, xij, h, o1, o2);
     double rij = norm3(xij);
     if (!((rij/h) <= Cellfac)) return;
     (*cnt)++;
@@ -1275,6 +1322,12 @@ static int domain_move(psph_odomain *d, double dt)
                 #pragma omp critical
                 err = rc;
             }
+            /* periodic position update, Particle.cpp:127-142 */
+            for (int c = 0; c < 3; ++c)
+                if (d->DomSize[c] > 0.0) {
+                    if (p->x[c] > d->TRPR[c]) p->x[c] -= d->DomSize[c];
+                    if (p->x[c] < d->BLPF[c]) p->x[c] += d->DomSize[c];
+                }
         }
     }
     return err;
@@ -1329,8 +1382,8 @@ int psph_o_create(const psph_config *cfg, const psph_particle *p, size_t n, doub
                   int nthreads, int track_scales, psph_odomain **out)
 {
     if (!cfg || !p || !out || n == 0) return set_err(PSPH_ERR_INVALID, "null argument or no particles");
-    if (cfg->periodic[0] || cfg->periodic[1] || cfg->periodic[2] || cfg->in_out_flow || cfg->fsi)
-        return set_err(PSPH_ERR_UNSUPPORTED, "periodic / in-out-flow / FSI are outside this path");
+    if (cfg->in_out_flow || cfg->fsi)
+        return set_err(PSPH_ERR_UNSUPPORTED, "in-out-flow / FSI are outside this path");
     if (cfg->kernel_type < 0 || cfg->kernel_type > 2) return set_err(PSPH_ERR_INVALID, "Kernel Type No is out of range");
     if (cfg->visc_eq < 0 || cfg->visc_eq > 3) return set_err(PSPH_ERR_INVALID, "Viscosity Equation No is out of range");
     if (nthreads < 1) nthreads = 1;
@@ -1365,10 +1418,26 @@ int psph_o_create(const psph_config *cfg, const psph_particle *p, size_t n, doub
     return 0;
 }
 
+/* The GeneralBefore hook of 1-Poiseuille.cpp:23-29 (a body force on the free particles) */
+static void body_force_hook(psph_odomain *d)
+{
+    if (d->body[0] == 0.0 && d->body[1] == 0.0 && d->body[2] == 0.0) return;
+    for (size_t i = 0; i < d->n; i++) {
+        psph_particle *p = &d->P[i]->p;
+        if (p->IsFree) for (int c = 0; c < 3; ++c) p->a[c] += d->body[c];
+    }
+}
+
+void psph_o_set_body_force(psph_odomain *d, const double accel[3])
+{
+    for (int c = 0; c < 3; ++c) d->body[c] = accel[c];
+}
+
 static int force_pass(psph_odomain *d)
 {
     start_acceleration(d);
     main_neighbour_search(d);
+    body_force_hook(d);
     primary_compute_acceleration(d);
     return last_compute_acceleration(d);
 }
@@ -1447,9 +1516,11 @@ int psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs)
         pairlist *Ls[2] = {&d->SM[t], &d->FSM[t]};
         for (int l = 0; l < 2; ++l)
             for (size_t i = 0; i < Ls[l]->n; ++i) {
-                const psph_particle *P1 = &d->P[Ls[l]->v[2*i]]->p, *P2 = &d->P[Ls[l]->v[2*i+1]]->p;
+                const OP *o1 = d->P[Ls[l]->v[2*i]], *o2 = d->P[Ls[l]->v[2*i+1]];
+                const psph_particle *P1 = &o1->p, *P2 = &o2->p;
                 double h = (P1->h+P2->h)/2;
                 double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+                periodic_correction(d, xij, h, o1, o2);
                 double rij = norm3(xij);
                 if ((rij/h) <= d->cfg.cellfac) {
                     if (k < cap) { pairs[2*k] = Ls[l]->v[2*i]; pairs[2*k+1] = Ls[l]->v[2*i+1]; }

@@ -55,6 +55,8 @@ int  psph_o_grid(psph_odomain *d, psph_grid *g);
 /* In-support pairs (P1, P2) of the current positions, in the reference's search order
  * (before any force pass of the next step): SM pairs then FSM pairs of each thread list. */
 int  psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs);
+/* body-force hook: accel is added to every free particle's a after StartAcceleration */
+void psph_o_set_body_force(psph_odomain *d, const double accel[3]);
 /* Per-particle |contribution| sums of the last force pass, 24 doubles per particle:
  * a[3], dDensity, Pressure(wall), NSv[3], SumKernel, StrainRate[6], RotationRate[3],
  * VXSPH[3], ZWab, SumDen, (pad) -- the wall entries already divided by SumKernel. */

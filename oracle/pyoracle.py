@@ -53,6 +53,8 @@ def lib():
         L.psph_o_pairs.argtypes = [p, p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.psph_o_scales.restype = i
         L.psph_o_scales.argtypes = [p, p, ctypes.c_size_t]
+        L.psph_o_set_body_force.restype = None
+        L.psph_o_set_body_force.argtypes = [p, p]
         L.psph_o_destroy.restype = None
         L.psph_o_destroy.argtypes = [p]
         L.psph_o_last_error.restype = ctypes.c_char_p
@@ -98,6 +100,10 @@ class Oracle:
         out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
         _check(lib().psph_o_get(self._h, _ptr(out), n.value, ctypes.byref(n)))
         return out
+
+    def set_body_force(self, accel):
+        a = np.ascontiguousarray(accel, dtype=np.float64)
+        lib().psph_o_set_body_force(self._h, _ptr(a))
 
     def grid(self):
         g = Grid()

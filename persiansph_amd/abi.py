@@ -77,6 +77,22 @@ class Status(ctypes.Structure):
     ]
 
 
+class OutputF32(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_size_t),
+        ("position", ctypes.c_void_p), ("velocity", ctypes.c_void_p), ("acceleration", ctypes.c_void_p),
+        ("pressure", ctypes.c_void_p), ("density", ctypes.c_void_p), ("mass", ctypes.c_void_p),
+        ("h", ctypes.c_void_p), ("tag", ctypes.c_void_p), ("sigma", ctypes.c_void_p),
+        ("strain", ctypes.c_void_p),
+    ]
+
+
+# datasets of Domain::WriteXDMF (Domain.cpp:1934-1963) held by psph_output_f32: name, type, width
+OUTPUT_FIELDS = [("position", "f4", 3), ("velocity", "f4", 3), ("acceleration", "f4", 3),
+                 ("pressure", "f4", 1), ("density", "f4", 1), ("mass", "f4", 1), ("h", "f4", 1),
+                 ("tag", "i4", 1), ("sigma", "f4", 6), ("strain", "f4", 6)]
+
+
 class Grid(ctypes.Structure):
     _fields_ = [
         ("blpf", ctypes.c_double * 3), ("trpr", ctypes.c_double * 3),

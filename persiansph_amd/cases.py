@@ -327,3 +327,65 @@ def runout(p):
     """Runout of a granular column: the largest x reached by a free particle."""
     free = p["IsFree"] == 1
     return float(p["x"][free, 0].max())
+
+
+def poiseuille_2d(dx=2.5e-5, body_force=0.002, rho=998.21, mu=1.002e-3, cs=0.07, nx=20, ny=48):
+    """Plane Poiseuille flow as 1-Poiseuille.cpp:36-80 sets it up: a cubic box of nx x ny
+    particles from (0, -ny/2 dx), periodic in x; particles with |y| >= 20 dx are fixed
+    no-slip walls (ID 2); quintic spline, Takeda viscosity, squared-density form, modified
+    Verlet, linear EOS, h = 1.1 dx, dt = 0.2 h / Cs.  The flow is driven by the body force
+    of its GeneralBefore hook (a += (body_force, 0, 0) on free particles), which goes to
+    Solver.set_body_force.  Returns (config, particles, dt, body)."""
+    h = 1.1*dx
+    r = dx/2.0
+    p = add_box_length(2, (0.0, -(ny/2)*dx, 0.0), nx*dx + dx/10.0, ny*dx + dx/10.0, 0.0, r, rho, h, 1, 0, False)
+    p["Cs"] = cs
+    p["PresEq"] = 0
+    p["Mu"] = mu
+    p["MuRef"] = mu
+    p["Material"] = 1
+    p["ID"] = 1
+    yb = p["x"][:, 1]
+    wall = (yb >= 20.0*dx) | (yb <= -20.0*dx)
+    p["ID"][wall] = 2
+    p["IsFree"][wall] = 0
+    p["NoSlip"][wall] = 1
+    cfg = default_config(dimension=2, kernel_type=2, visc_eq=3, gradient_type=0, scheme=0)
+    cfg.periodic[0] = 1
+    cfg.R = r
+    dt = 0.2*h/cs
+    return cfg, p, dt, (body_force, 0.0, 0.0)
+
+
+def couette_2d(dx=2.5e-5, v_lid=2.5e-5, rho=998.21, mu=1.002e-3, cs=0.08, nx=20, ny=48):
+    """Plane Couette flow as 2-Couette.cpp:33-91 sets it up: a cubic box of nx x ny particles
+    from (0, -4 dx), periodic in x; particles with y >= 40 dx form the moving no-slip lid
+    (ID 3, v = (v_lid, 0, 0)), those with y < 0 the fixed no-slip floor (ID 2); LES on with
+    CSmag = 0 (the shear rate is formed, the viscosity stays MuRef); quintic spline, Takeda
+    viscosity, modified Verlet, linear EOS, h = 1.1 dx, dt = 0.2 h / Cs."""
+    h = 1.1*dx
+    r = dx/2.0
+    p = add_box_length(2, (0.0, -4.0*dx, 0.0), nx*dx + dx/10.0, ny*dx + dx/10.0, 0.0, r, rho, h, 1, 0, False)
+    p["LES"] = 1
+    p["CSmag"] = 0.0
+    p["Cs"] = cs
+    p["PresEq"] = 0
+    p["Mu"] = mu
+    p["MuRef"] = mu
+    p["Material"] = 1
+    p["ID"] = 1
+    yb = p["x"][:, 1]
+    lid = yb >= 40.0*dx
+    floor = yb < 0.0
+    p["ID"][lid] = 3
+    p["IsFree"][lid] = 0
+    p["NoSlip"][lid] = 1
+    p["v"][lid, 0] = v_lid
+    p["ID"][floor] = 2
+    p["IsFree"][floor] = 0
+    p["NoSlip"][floor] = 1
+    cfg = default_config(dimension=2, kernel_type=2, visc_eq=3, gradient_type=0, scheme=0)
+    cfg.periodic[0] = 1
+    cfg.R = r
+    dt = 0.2*h/cs
+    return cfg, p, dt

@@ -10,9 +10,17 @@
 // user code touches (element access, comma initialisation, arithmetic).  Errors surface as
 // `throw new Fatal(...)`, caught by MECHSYS_CATCH in main, as in the reference.
 //
-// Not available on the GPU path (Solve throws Fatal): periodic and in/out-flow boundaries,
-// FSI and multi-material pairs, and user hooks GeneralBefore/GeneralAfter other than the
-// default no-op.  XDMF output is not written; UserOutput is not called.
+// Output: WriteXDMF writes the reference's <key>.hdf5 + <key>.xmf pair (HDF5 high-level API;
+// link -lhdf5_hl -lhdf5).  Inside Solve, outputs with the default UserOutput are packed to FP32
+// on the GPU (psph_output_begin/end) and written by a background thread while the next steps
+// run; a user UserOutput gets the full state downloaded into Particles first, as it reads them.
+//
+// User hooks: GeneralBefore may be SPH::BodyForceHook (a constant acceleration, Domain::BodyForce,
+// on every free particle, as 1-Poiseuille.cpp's UserAcc), which Solve applies on the GPU.
+//
+// Not available on the GPU path (Solve throws Fatal): in/out-flow boundaries, FSI and
+// multi-material pairs, and other GeneralBefore/GeneralAfter hooks than the default no-op and
+// the body-force hook.
 #ifndef PERSIANSPH_AMD_DOMAIN_H
 #define PERSIANSPH_AMD_DOMAIN_H
 
@@ -22,9 +30,15 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include "hdf5.h"
+#include "hdf5_hl.h"
 
 #include "persiansph_amd.h"
 
@@ -201,11 +215,20 @@ public:
 // ----------------------------------------------------------------------------- Domain
 class Domain;
 inline void General(Domain &) {}
+inline void BodyForceHook(Domain &dom);
+
+// default UserOutput (Domain.cpp:28-33)
+inline void OutPut(Particle *, double &Prop1, double &Prop2, double &Prop3) { Prop1 = 0.0; Prop2 = 0.0; Prop3 = 0.0; }
+
+// default AllCon (Domain.cpp:47-51)
+inline void AllFlowCon(Vec3_t &, Vec3_t &Vel, double &Den, Boundary &bdry) { Vel = bdry.allv; Den = bdry.allDensity; }
 
 class Domain
 {
 public:
     typedef void (*PtDom)(Domain &dom);
+    typedef void (*PtOut)(Particle *Particles, double &Prop1, double &Prop2, double &Prop3);
+    typedef void (*PtVel)(Vec3_t &position, Vec3_t &Vel, double &Den, Boundary &bdry);
 
     Array<Particle *> Particles;
     double R = 0.0;
@@ -225,6 +248,10 @@ public:
     Vec3_t DomMax, DomMin;
     PtDom GeneralBefore = &General;
     PtDom GeneralAfter = &General;
+    PtOut UserOutput = &OutPut;
+    Vec3_t BodyForce;                    // acceleration added by BodyForceHook (this build's addition)
+    PtVel AllCon = &AllFlowCon;
+    std::string OutputName[3] = {"Property1", "Property2", "Property3"};
     size_t Scheme = 0;
     int Precision = PSPH_FP64;           // GPU arithmetic tier (this build's addition)
 
@@ -370,10 +397,12 @@ public:
     // Solve, Domain.cpp:1735-1797, with the time loop on the GPU
     void Solve(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx)
     {
-        (void)TheFileKey;
         std::cout << "\n--------------Solving---------------------------------------------------------------" << std::endl;
-        if (GeneralBefore != &General || GeneralAfter != &General)
-            throw new Fatal("Domain::Solve: user hooks GeneralBefore/GeneralAfter are not available on the GPU path");
+        const bool body_hook = GeneralBefore == &BodyForceHook;
+        if ((GeneralBefore != &General && !body_hook) || GeneralAfter != &General)
+            throw new Fatal("Domain::Solve: user hooks GeneralBefore/GeneralAfter are not available on the GPU path "
+                            "(a constant body force on the free particles: GeneralBefore = &SPH::BodyForceHook "
+                            "with Domain::BodyForce)");
         if (Particles.Size() == 0) throw new Fatal("Domain::Solve: no particles");
         psph_config cfg;
         std::memset(&cfg, 0, sizeof cfg);
@@ -387,14 +416,22 @@ public:
         cfg.in_out_flow = BC.InOutFlow; cfg.swi_type = (int)SWIType; cfg.fsi = FSI ? 1 : 0;
         cfg.cellfac = Cellfac; cfg.xsph = XSPH; cfg.sqrt_h_a = sqrt_h_a; cfg.R = R;
         cfg.mu_max = MuMax; cfg.cs_max = CsMax;
+        WholeVelocity();           // before the upload: its EOS pressure is the upload's (Domain.cpp:1656)
         std::vector<psph_particle> buf(Particles.Size());
         for (size_t i = 0; i < Particles.Size(); ++i) to_abi(*Particles[i], buf[i]);
         up_.assign(Particles.begin(), Particles.end());
         alive_.assign(up_.size(), 1);
         psph_handle *h = nullptr;
         check(psph_create(&cfg, &h));
-        struct Guard { psph_handle *h; ~Guard() { psph_destroy(h); } } guard{h};
+        struct Guard {
+            psph_handle *h; Domain *d;
+            ~Guard() { d->join_writer(false); psph_destroy(h); }
+        } guard{h, this};
         check(psph_upload(h, buf.data(), buf.size(), Time, dt));
+        if (body_hook) {
+            double b[3] = {BodyForce(0), BodyForce(1), BodyForce(2)};
+            check(psph_set_body_force(h, b));
+        }
         psph_grid g;
         check(psph_get_grid(h, &g));
         for (int c = 0; c < 3; ++c) { TRPR(c) = g.trpr[c]; BLPF(c) = g.blpf[c]; CellSize(c) = g.cell_size[c]; CellNo[c] = g.cell_no[c]; }
@@ -402,15 +439,26 @@ public:
         deltat = deltatint = deltatmin = dt;
         size_t idx_out = 1;
         double tout = Time;
+        if (TheFileKey != NULL) {
+            PrintInput(TheFileKey);
+            char fn[4096];
+            std::snprintf(fn, sizeof fn, "%s_Initial", TheFileKey);
+            output(h, fn);
+            std::cout << "\nInitial Condition has been generated\n" << std::endl;
+        }
         psph_status st;
         for (;;) {
             if (!(Time < tf && idx_out <= maxidx)) break;
             check(psph_run(h, INT64_MAX, tf, tout, &st));
             Time = st.time; deltat = st.deltat; deltatmin = st.deltatmin;
             if (st.stop_reason != PSPH_STOP_OUTPUT) break;
-            download(h);
-            std::cout << "\nOutput No. " << idx_out << " at " << Time << " has been generated" << std::endl;
-            std::cout << "Current Time Step = " << deltat << std::endl;
+            if (TheFileKey != NULL) {
+                char fn[4096];
+                std::snprintf(fn, sizeof fn, "%s_%04zu", TheFileKey, idx_out);
+                output(h, fn);
+                std::cout << "\nOutput No. " << idx_out << " at " << Time << " has been generated" << std::endl;
+                std::cout << "Current Time Step = " << deltat << std::endl;
+            }
             idx_out++;
             tout += dtOut;
             if (idx_out > maxidx) {                      // the current step still completes
@@ -419,8 +467,153 @@ public:
                 break;
             }
         }
+        join_writer(true);
         download(h);
         std::cout << "\n--------------Solving is finished---------------------------------------------------" << std::endl;
+    }
+
+    // WriteXDMF, Domain.cpp:1869-2064: <FileKey>.hdf5 with the particle fields as float and
+    // <FileKey>.xmf describing them, from the host Particles
+    void WriteXDMF(char const *FileKey)
+    {
+        const size_t n = Particles.Size();
+        std::vector<float> pos(3*n), vel(3*n), acc(3*n), pr(n), den(n), mass(n), sh(n), p1(n), p2(n), p3(n);
+        std::vector<float> sig(6*n), eps(6*n);
+        std::vector<int> tag(n);
+        static const int sym[6] = {0, 1, 2, 4, 5, 8};
+        for (size_t i = 0; i < n; ++i) {
+            Particle *p = Particles[i];
+            for (int c = 0; c < 3; ++c) {
+                pos[3*i+c] = float(p->x(c)); vel[3*i+c] = float(p->v(c)); acc[3*i+c] = float(p->a(c));
+            }
+            pr[i] = float(p->Pressure); den[i] = float(p->Density); mass[i] = float(p->Mass);
+            sh[i] = float(p->h); tag[i] = int(p->ID);
+            for (int c = 0; c < 6; ++c) { sig[6*i+c] = float(p->Sigma.d[sym[c]]); eps[6*i+c] = float(p->Strain.d[sym[c]]); }
+            double P1, P2, P3;
+            UserOutput(p, P1, P2, P3);
+            p1[i] = float(P1); p2[i] = float(P2); p3[i] = float(P3);
+        }
+        std::string err = write_xdmf(FileKey, n, pos.data(), vel.data(), acc.data(), tag.data(), pr.data(),
+                                     den.data(), p1.data(), p2.data(), p3.data(), mass.data(), sh.data(),
+                                     sig.data(), eps.data(), OutputName);
+        if (!err.empty()) throw new Fatal("Domain::WriteXDMF: %s", err.c_str());
+    }
+
+    // PrintInput, Domain.cpp:1799-1867: <FileKey>_log.dat with the run's settings
+    void PrintInput(char const *FileKey)
+    {
+        static const char *kt[3] = {"Qubic Spline\n", "Quintic\n", "Quintic Spline\n"};
+        static const char *ve[4] = {"0 => Morris et al 1997\n", "1 => Shao et al 2003\n",
+                                    "2 => Real viscosity for incompressible fluids\n",
+                                    "3 => Takeda et al 1994 (Real viscosity for compressible fluids)\n"};
+        std::ostringstream oss;
+        oss << "Dimension = " << Dimension << "D\n";
+        oss << "\nKernel Type = " << (KernelType < 3 ? kt[KernelType] : "");
+        oss << "\nViscosity Equation = " << (VisEq < 4 ? ve[VisEq] : "");
+        oss << "\nComputational domain size\n";
+        oss << "Bottom Left-Corner Front = " << BLPF << " m\n";
+        oss << "Top Right-Corner Rear    = " << TRPR << " m\n";
+        oss << "\nMax of the smoothing lengths, h = " << hmax << " m\n";
+        oss << "Cell factor in Linked List (based on kernels) = " << Cellfac << "\n";
+        oss << "\nCell Size in XYZ Directions = " << CellSize << " m\n";
+        oss << "No of Cells in XYZ Directions = ( " << CellNo[0] << " , " << CellNo[1] << " , " << CellNo[2] << " )\n";
+        oss << "\nInitial No of Particles = " << Particles.Size() << "\n";
+        oss << "\nInitial Time Step = " << deltatint << " S\n";
+        oss << "\nExternal Acceleration (Gravity enabled)= " << Gravity << " m/s2\n";
+        oss << "\nNo of Threads = " << Nproc << "\n";
+        oss << "\nPeriodic Boundary Condition X dir= " << (BC.Periodic[0] ? "True" : "False") << "\n";
+        oss << "Periodic Boundary Condition Y dir= " << (BC.Periodic[1] ? "True" : "False") << "\n";
+        oss << "Periodic Boundary Condition Z dir= " << (BC.Periodic[2] ? "True" : "False") << "\n";
+        std::ofstream of(std::string(FileKey) + "_log.dat", std::ios::out);
+        of << oss.str();
+    }
+
+    // WholeVelocity, Domain.cpp:1637-1660 (the pressure follows from the density at upload)
+    void WholeVelocity()
+    {
+        if (norm(BC.allv) > 0.0 || BC.allDensity > 0.0) {
+            for (size_t i = 0; i < Particles.Size(); i++) {
+                Vec3_t vel; double den = 0.0;
+                AllCon(Particles[i]->x, vel, den, BC);
+                if (Particles[i]->IsFree && norm(BC.allv) > 0.0) Particles[i]->v = vel;
+                if (Particles[i]->IsFree && BC.allDensity > 0.0) Particles[i]->Density = den;
+            }
+        }
+    }
+
+    // Writes <key>.hdf5 and <key>.xmf as WriteXDMF lays them out (datasets /NP, Position,
+    // Velocity, Acceleration, Tag, Pressure, Density, the three user properties, Mass, h, Sigma,
+    // Strain; Domain.cpp:1930-2063).  Returns an error message, empty on success.
+    static std::string write_xdmf(const char *key, size_t n, const float *pos, const float *vel,
+                                  const float *acc, const int *tag, const float *pr, const float *den,
+                                  const float *p1, const float *p2, const float *p3, const float *mass,
+                                  const float *sh, const float *sig, const float *eps,
+                                  const std::string *names)
+    {
+        const std::string fn = std::string(key) + ".hdf5";
+        hid_t f = H5Fcreate(fn.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+        if (f < 0) return "cannot create " + fn;
+        bool ok = true;
+        int np[1] = {(int)n};
+        hsize_t d1[1] = {1}, d3[1] = {3*n}, dn[1] = {n}, d6[1] = {6*n};
+        ok = ok && H5LTmake_dataset_int(f, "/NP", 1, d1, np) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Position", 1, d3, pos) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Velocity", 1, d3, vel) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Acceleration", 1, d3, acc) >= 0;
+        ok = ok && H5LTmake_dataset_int(f, "Tag", 1, dn, tag) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Pressure", 1, dn, pr) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Density", 1, dn, den) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, names[0].c_str(), 1, dn, p1) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, names[1].c_str(), 1, dn, p2) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, names[2].c_str(), 1, dn, p3) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Mass", 1, dn, mass) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "h", 1, dn, sh) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Sigma", 1, d6, sig) >= 0;
+        ok = ok && H5LTmake_dataset_float(f, "Strain", 1, d6, eps) >= 0;
+        H5Fflush(f, H5F_SCOPE_GLOBAL);
+        H5Fclose(f);
+        if (!ok) return "cannot write the datasets of " + fn;
+        std::ostringstream oss;
+        auto item = [&](const char *name, const char *type, const char *dims_suffix, const char *num, const char *prec) {
+            oss << "     <Attribute Name=\"" << name << "\" AttributeType=\"" << type << "\" Center=\"Node\">\n";
+            oss << "       <DataItem Dimensions=\"" << n << dims_suffix << "\" NumberType=\"" << num << "\"" << prec << " Format=\"HDF\">\n";
+            oss << "        " << fn << ":/" << name << " \n";
+            oss << "       </DataItem>\n";
+            oss << "     </Attribute>\n";
+        };
+        oss << "<?xml version=\"1.0\" ?>\n";
+        oss << "<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n";
+        oss << "<Xdmf Version=\"2.0\">\n";
+        oss << " <Domain>\n";
+        oss << "   <Grid Name=\"SPHCenter\" GridType=\"Uniform\">\n";
+        oss << "     <Topology TopologyType=\"Polyvertex\" NumberOfElements=\"" << n << "\"/>\n";
+        oss << "     <Geometry GeometryType=\"XYZ\">\n";
+        oss << "       <DataItem Format=\"HDF\" NumberType=\"Float\" Precision=\"10\" Dimensions=\"" << n << " 3\" >\n";
+        oss << "        " << fn << ":/Position \n";
+        oss << "       </DataItem>\n";
+        oss << "     </Geometry>\n";
+        oss << "     <Attribute Name=\"Tag\" AttributeType=\"Scalar\" Center=\"Node\">\n";
+        oss << "       <DataItem Dimensions=\"" << n << "\" NumberType=\"Int\" Format=\"HDF\">\n";
+        oss << "        " << fn << ":/Tag \n";
+        oss << "       </DataItem>\n";
+        oss << "     </Attribute>\n";
+        item("Position", "Vector", " 3", "Float", " Precision=\"10\"");
+        item("Velocity", "Vector", " 3", "Float", " Precision=\"10\"");
+        item("Acceleration", "Vector", " 3", "Float", " Precision=\"10\"");
+        item("Density", "Scalar", "", "Float", " Precision=\"10\" ");
+        item("Pressure", "Scalar", "", "Float", " Precision=\"10\" ");
+        item(names[0].c_str(), "Scalar", "", "Float", " Precision=\"10\" ");
+        item(names[1].c_str(), "Scalar", "", "Float", " Precision=\"10\" ");
+        item(names[2].c_str(), "Scalar", "", "Float", " Precision=\"10\" ");
+        item("Sigma", "Tensor6", " 6", "Float", " Precision=\"10\"");
+        item("Strain", "Tensor6", " 6", "Float", " Precision=\"10\"");
+        oss << "   </Grid>\n";
+        oss << " </Domain>\n";
+        oss << "</Xdmf>\n";
+        std::ofstream of(std::string(key) + ".xmf", std::ios::out);
+        of << oss.str();
+        if (!of) return "cannot write " + std::string(key) + ".xmf";
+        return std::string();
     }
 
     double SimTime() const { return Time; }
@@ -501,9 +694,55 @@ private:
         Particles.swap(next);
     }
 
+    // One output inside Solve.  With the default UserOutput the fields are packed on the GPU and
+    // written by a background thread (the packed buffers stay valid until the next
+    // psph_output_begin, which follows the join); otherwise the state is downloaded into
+    // Particles and written from there, as UserOutput reads the particles.
+    void output(psph_handle *h, const char *fn)
+    {
+        join_writer(true);
+        if (UserOutput != &OutPut) {
+            download(h);
+            WriteXDMF(fn);
+            return;
+        }
+        check(psph_output_begin(h));
+        psph_output_f32 o;
+        check(psph_output_end(h, &o));
+        std::string key(fn);
+        std::string names[3] = {OutputName[0], OutputName[1], OutputName[2]};
+        writer_err_.clear();
+        writer_ = std::thread([this, key, o, names] {
+            std::vector<float> zero(o.n, 0.0f);
+            writer_err_ = write_xdmf(key.c_str(), o.n, o.position, o.velocity, o.acceleration, o.tag,
+                                     o.pressure, o.density, zero.data(), zero.data(), zero.data(),
+                                     o.mass, o.h, o.sigma, o.strain, names);
+        });
+    }
+
+    void join_writer(bool raise)
+    {
+        if (writer_.joinable()) writer_.join();
+        if (raise && !writer_err_.empty()) {
+            std::string e;
+            e.swap(writer_err_);
+            throw new Fatal("Domain::WriteXDMF: %s", e.c_str());
+        }
+    }
+
     std::vector<Particle *> up_;     // Particles as uploaded by the current Solve
     std::vector<char> alive_;
+    std::thread writer_;             // background XDMF writer of the current Solve
+    std::string writer_err_;
 };
+
+// The body-force hook (1-Poiseuille.cpp:23-29 with a configurable constant): a += BodyForce
+// for every free particle.  Solve runs it on the GPU; this host form serves direct calls.
+inline void BodyForceHook(Domain &dom)
+{
+    for (size_t i = 0; i < dom.Particles.Size(); i++)
+        if (dom.Particles[i]->IsFree) dom.Particles[i]->a = dom.Particles[i]->a + dom.BodyForce;
+}
 
 }  // namespace SPH
 

@@ -2,12 +2,15 @@
 // 4-DamBreak3D.cpp drivers are: lattice box, walls fixed and no-slip, fluid above the column
 // deleted, hydrostatic density, Solve.
 //
-//   dam_break <dim> <dx> <steps> [out.bin]
-// runs `steps` Leapfrog steps (tf = steps * dt0 upper bound, maxidx = 1 output) and, if out.bin
-// is given, writes n followed by x, y, z, Density of every particle (FP64) for comparison.
+//   dam_break <dim> <dx> <steps> [out.bin [filekey [out_every]]]
+// runs `steps` Leapfrog steps (tf = steps * dt0 upper bound) and, if out.bin is given (or "-"
+// for none), writes n followed by x, y, z, Density of every particle (FP64) for comparison.
+// With a file key, Solve writes <filekey>_Initial and then an XDMF output every out_every dt0
+// of simulated time (<filekey>_0001, ...), as the reference's drivers do.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "SPH/Domain.h"
 
@@ -16,7 +19,9 @@ int main(int argc, char **argv) try
     int dim = argc > 1 ? std::atoi(argv[1]) : 2;
     double dx = argc > 2 ? std::atof(argv[2]) : 0.02;
     int steps = argc > 3 ? std::atoi(argv[3]) : 10;
-    const char *out = argc > 4 ? argv[4] : nullptr;
+    const char *out = (argc > 4 && std::string(argv[4]) != "-") ? argv[4] : nullptr;
+    const char *key = argc > 5 ? argv[5] : nullptr;
+    double every = argc > 6 ? std::atof(argv[6]) : 1.0e12;
 
     SPH::Domain dom;
     dom.Dimension = dim;
@@ -54,8 +59,7 @@ int main(int argc, char **argv) try
         }
     }
     dom.DelParticles(11);
-    // one output at t = 0 (maxidx = 1), then the loop ends after that step; extend with steps
-    dom.Solve(/*tf*/steps*t*(1.0 - 1e-9), /*dt*/t, /*dtOut*/1.0e9, "dam_break", 1000);
+    dom.Solve(/*tf*/steps*t*(1.0 - 1e-9), /*dt*/t, /*dtOut*/every*t, key, 1000);
     std::printf("particles %zu time %.9e\n", dom.Particles.Size(), dom.SimTime());
     if (out) {
         FILE *f = std::fopen(out, "wb");

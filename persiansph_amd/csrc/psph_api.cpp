@@ -36,6 +36,8 @@ void launch_tail(const Dev &D, const Params &P, hipStream_t s);
 void launch_pairs(const Dev &D, const Params &P, hipStream_t s, int pass, const long long *off,
                   int *cnt, long long *out);
 int scan_blocks(const Params &P);
+int output_floats();
+void launch_pack_output(const Dev &D, const Params &P, float *out, int acc_ok, int tensors, hipStream_t s);
 }  // namespace psph
 
 using namespace psph;
@@ -90,6 +92,7 @@ struct psph_handle {
     bool broken = false;
     psph_grid grid{};
     bool trans_ok = false;             // transient columns match the current order
+    double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
     // timing
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -103,6 +106,14 @@ struct psph_handle {
     // reuse); skin sub-lists of the wall targets (capacity Params::wl_cap)
     int32_t *nl = nullptr;
     int32_t *wl = nullptr;
+    // visualisation output: FP32 staging on the device, its pinned host copy (out_cap particles
+    // each), the copy's completion event; fields packed on the host instead (reference order)
+    float *out_dev = nullptr;
+    float *out_host = nullptr;
+    size_t out_cap = 0, out_n = 0;
+    hipEvent_t out_event = nullptr;
+    bool out_pending = false;
+    std::vector<float> out_acc, out_sig, out_eps;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -195,6 +206,11 @@ static void free_all(psph_handle *h)
     for (void *p : h->allocs) hipFree(p);
     h->allocs.clear();
     if (h->ctl_host) { hipHostFree(h->ctl_host); h->ctl_host = nullptr; }
+    if (h->out_host) { hipHostFree(h->out_host); h->out_host = nullptr; }
+    if (h->out_event) { hipEventDestroy(h->out_event); h->out_event = nullptr; }
+    h->out_dev = nullptr;               // in allocs
+    h->out_cap = 0;
+    h->out_pending = false;
     for (auto &e : h->pending) { hipEventDestroy(e.second.first); hipEventDestroy(e.second.second); }
     h->pending.clear();
     for (auto e : h->event_pool) hipEventDestroy(e);
@@ -292,7 +308,9 @@ static inline void from_sym(const double *s, double *M)
     M[6] = s[2]; M[7] = s[4]; M[8] = s[5];
 }
 
-// CellInitiate, Domain.cpp:549-660 (non-periodic)
+// CellInitiate, Domain.cpp:549-660.  A periodic direction takes the margin R instead of hmax/2
+// and two more cell layers (the reference's aliases of the first two; reported in cell_no, not
+// stored on the device).
 static int cell_initiate(psph_handle *h, const psph_particle *p, size_t n)
 {
     psph_grid &g = h->grid;
@@ -323,7 +341,10 @@ static int cell_initiate(psph_handle *h, const psph_particle *p, size_t n)
         if (h->cfg.dom_max[c] > TRPR[c]) TRPR[c] = h->cfg.dom_max[c];
         if (h->cfg.dom_min[c] < BLPF[c]) BLPF[c] = h->cfg.dom_min[c];
     }
-    for (int c = 0; c < 3; ++c) { TRPR[c] += hmax/2; BLPF[c] -= hmax/2; }
+    for (int c = 0; c < 3; ++c) {
+        if (!h->cfg.periodic[c]) { TRPR[c] += hmax/2; BLPF[c] -= hmax/2; }
+        else { TRPR[c] += h->cfg.R; BLPF[c] -= h->cfg.R; }
+    }
     double Cellfac = h->cfg.cellfac;
     int dims = h->cfg.dimension;
     int CellNo[3] = {1, 1, 1};
@@ -334,7 +355,14 @@ static int cell_initiate(psph_handle *h, const psph_particle *p, size_t n)
         else CellNo[c] = (int)std::floor(L);
         if (CellNo[c] < 1) return fail(PSPH_ERR_INVALID, "domain smaller than one cell in dimension %d", c);
         CellSize[c] = (TRPR[c]-BLPF[c])/CellNo[c];
+        // with fewer than three cells across a period the reference's alias layers pair some
+        // cells twice
+        if (h->cfg.periodic[c] && CellNo[c] < 3)
+            return fail(PSPH_ERR_UNSUPPORTED, "a periodic direction needs at least 3 cells across the period "
+                        "(dimension %d has %d)", c, CellNo[c]);
     }
+    for (int c = 0; c < 3; ++c)
+        if (h->cfg.periodic[c]) CellNo[c] += 2;
     double ncells = (double)CellNo[0]*CellNo[1]*CellNo[2];
     if (ncells > 2.0e9) return fail(PSPH_ERR_CAPACITY, "too many cells (%g)", ncells);
     std::memcpy(g.blpf, BLPF, sizeof BLPF);
@@ -593,6 +621,11 @@ static int read_state(psph_handle *h)
             if (P.scheme == 1) { from_sym(&sigh[6*i], q.Sigmaa); from_sym(&tauh[6*i], q.ShearStressa); from_sym(&epsh[6*i], q.Straina); }
             else { from_sym(&sigh[6*i], q.Sigmab); from_sym(&tauh[6*i], q.ShearStressb); from_sym(&epsh[6*i], q.Strainb); }
         }
+        else if (!fr && h->ctl_host->reorders > 0) {
+            // StartAcceleration zeroes a fixed particle's Sigma and ShearStress (Domain.cpp:991-992);
+            // fluid runs keep no tensors on the device
+            for (int k = 0; k < 9; ++k) { q.Sigma[k] = 0.0; q.ShearStress[k] = 0.0; }
+        }
         if (fr) {
             q.FirstStep = h->ctl_host->first_step;
             if (P.scheme == 0) { q.ct = h->ctl_host->ct; q.ShepardCounter = h->ctl_host->shep_counter; }
@@ -705,7 +738,7 @@ static int delete_leavers(psph_handle *h, long long *ndel)
 // ------------------------------------------------------------------------------------------
 extern "C" {
 
-int psph_version(void) { return 100; }
+int psph_version(void) { return 102; }
 
 int psph_last_error(char *buf, size_t len)
 {
@@ -726,8 +759,8 @@ int psph_create(const psph_config *cfg, psph_handle **out)
     if (cfg->scheme < 0 || cfg->scheme > 1) return fail(PSPH_ERR_INVALID, "Scheme must be 0 (modified Verlet) or 1 (Leapfrog)");
     if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_MIXED)
         return fail(PSPH_ERR_INVALID, "precision must be PSPH_FP64 or PSPH_FP32_MIXED");
-    if (cfg->periodic[0] || cfg->periodic[1] || cfg->periodic[2])
-        return fail(PSPH_ERR_UNSUPPORTED, "periodic boundaries are outside this path");
+    if (cfg->dimension == 2 && cfg->periodic[2])
+        return fail(PSPH_ERR_UNSUPPORTED, "a periodic z direction in a 2-D run is outside this path");
     if (cfg->in_out_flow) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries are outside this path");
     if (cfg->fsi) return fail(PSPH_ERR_UNSUPPORTED, "fluid-structure interaction is outside this path");
     int ndev = 0;
@@ -808,6 +841,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     for (int c = 0; c < 3; ++c) {
         P.blpf[c] = h->grid.blpf[c]; P.trpr[c] = h->grid.trpr[c]; P.csize[c] = h->grid.cell_size[c];
         P.nc[c] = h->grid.cell_no[c]; P.gravity[c] = h->cfg.gravity[c];
+        P.periodic[c] = h->cfg.periodic[c] ? 1 : 0;
+        if (P.periodic[c]) {
+            P.nc[c] -= 2;                    // the alias layers are not stored
+            P.domsize[c] = h->grid.trpr[c] - h->grid.blpf[c];
+            P.any_periodic = 1;
+        }
+        P.afree[c] = P.gravity[c] + h->body[c];
     }
     P.ncells = P.nc[0]*P.nc[1]*P.nc[2];
     P.hmax = h->grid.hmax;
@@ -847,7 +887,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             if (h->host[i].IsFree && h->host[i].Mu != mu0) { mu_uniform = false; break; }
         bool ok = mat0 == 1 && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
                   !P.any_ti && !P.any_sr && !P.any_xsph && !P.any_shepard && P.uniform_h &&
-                  mu_uniform && P.nclasses <= 8 && have_free && !std::getenv("PSPH_GENERAL_KERNELS");
+                  mu_uniform && P.nclasses <= 8 && have_free && !P.any_periodic &&
+                  !std::getenv("PSPH_GENERAL_KERNELS");
         P.fast11 = ok ? 1 : 0;
         if (ok) {
             double hh = P.h_uniform;
@@ -1045,6 +1086,93 @@ int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n)
     if (rc) return rc;
     std::memcpy(p, h->host.data(), sizeof(psph_particle)*h->host.size());
     for (size_t i = 0; i < h->host.size(); ++i) p[i].Index = h->upload_index[i];
+    return 0;
+}
+
+int psph_set_body_force(psph_handle *h, const double accel[3])
+{
+    if (!h || !accel) return fail(PSPH_ERR_INVALID, "null argument");
+    for (int c = 0; c < 3; ++c) {
+        if (!std::isfinite(accel[c])) return fail(PSPH_ERR_INVALID, "body force component %d is not finite", c);
+        h->body[c] = accel[c];
+        h->P.afree[c] = h->P.gravity[c] + accel[c];
+    }
+    return 0;
+}
+
+int psph_output_begin(psph_handle *h)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "output before upload");
+    if (h->out_pending) return fail(PSPH_ERR_STATE, "an output is pending (call psph_output_end first)");
+    int rc = sync_ctl(h);
+    if (rc) return rc;
+    const Params &P = h->P;
+    const size_t n = (size_t)P.n, nf = (size_t)output_floats();
+    if (n > h->out_cap) {
+        if (h->out_dev) { rc = dev_free(h, (void **)&h->out_dev); if (rc) return rc; }
+        if (h->out_host) { HIPCHECK(hipHostFree(h->out_host)); h->out_host = nullptr; }
+        h->out_cap = 0;
+        rc = dev_alloc(h, (void **)&h->out_dev, sizeof(float)*nf*n);
+        if (rc) return rc;
+        HIPCHECK(hipHostMalloc((void **)&h->out_host, sizeof(float)*nf*n, hipHostMallocDefault));
+        h->out_cap = n;
+    }
+    if (!h->out_event) HIPCHECK(hipEventCreateWithFlags(&h->out_event, hipEventDisableTiming));
+    const Dev D = with_state(h, state_buffer(h));
+    const bool acc_dev = h->trans_ok, tensors_dev = P.any_solid != 0;
+    launch_pack_output(D, P, h->out_dev, acc_dev ? 1 : 0, tensors_dev ? 1 : 0, h->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(h->out_host, h->out_dev, sizeof(float)*nf*n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipEventRecord(h->out_event, h->stream));
+    h->out_pending = true;
+    h->out_n = n;
+    // fields the device does not hold for this state come from the host mirror (reference
+    // order), as psph_download returns them; filled while the copy runs
+    h->out_acc.clear();
+    h->out_sig.clear();
+    h->out_eps.clear();
+    if (!acc_dev) {
+        h->out_acc.resize(3*n);
+        for (size_t o = 0; o < n; ++o)
+            for (int c = 0; c < 3; ++c) h->out_acc[3*o + c] = (float)h->host[o].a[c];
+    }
+    if (!tensors_dev) {
+        static const int sym[6] = {0, 1, 2, 4, 5, 8};      // xx xy xz yy yz zz of a row-major 3x3
+        const bool zero_fixed = h->ctl_host->reorders > 0;
+        h->out_sig.resize(6*n);
+        h->out_eps.resize(6*n);
+        for (size_t o = 0; o < n; ++o) {
+            const psph_particle &q = h->host[o];
+            const bool z = zero_fixed && !q.IsFree;
+            for (int c = 0; c < 6; ++c) {
+                h->out_sig[6*o + c] = z ? 0.0f : (float)q.Sigma[sym[c]];
+                h->out_eps[6*o + c] = (float)q.Strain[sym[c]];
+            }
+        }
+    }
+    return 0;
+}
+
+int psph_output_end(psph_handle *h, psph_output_f32 *out)
+{
+    if (!h || !out) return fail(PSPH_ERR_INVALID, "null argument");
+    if (!h->out_pending) return fail(PSPH_ERR_STATE, "no output pending (call psph_output_begin first)");
+    h->out_pending = false;
+    HIPCHECK(hipEventSynchronize(h->out_event));
+    const size_t n = h->out_n;
+    const float *b = h->out_host;
+    out->n = n;
+    out->position = b;
+    out->velocity = b + 3*n;
+    out->acceleration = h->out_acc.empty() ? b + 6*n : h->out_acc.data();
+    out->pressure = b + 9*n;
+    out->density = b + 10*n;
+    out->mass = b + 11*n;
+    out->h = b + 12*n;
+    out->tag = reinterpret_cast<const int32_t *>(b + 13*n);
+    out->sigma = h->out_sig.empty() ? b + 14*n : h->out_sig.data();
+    out->strain = h->out_eps.empty() ? b + 20*n : h->out_eps.data();
     return 0;
 }
 

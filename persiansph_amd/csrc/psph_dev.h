@@ -111,6 +111,14 @@ struct Params {
     double skin_trig2;                 // (skin/2)^2 less a small margin: the rebuild trigger
     double adj2;                       // squared smallest cell size (less a margin): pairs closer
                                        // than this are in neighbouring cells
+    // periodic directions (Boundary::Periodic): nc[] counts the cells that hold particles (the
+    // reference's two alias layers are not stored; neighbour cells wrap instead); domsize =
+    // DomSize, the period
+    int any_periodic, periodic[3];
+    double domsize[3];
+    // a free particle's acceleration at the start of its force pass: Gravity plus the body force
+    // of the body-force hook (GeneralBefore adding a constant to every free particle's a)
+    double afree[3];
 };
 
 // One set of the persistent columns.

@@ -399,6 +399,71 @@ __device__ inline void row_range(const Dev &D, const Params &P, int kx, int ky, 
     end = D.cstart[base + x1 + 1];
 }
 
+// The same row with periodic directions (Domain.cpp:757-783: the reference's alias layers make
+// the cells across a period neighbours): y and z wrap, and an x row that crosses the period is
+// two ranges of the sorted order (the second empty otherwise).  Needs 3+ cells per period.
+__device__ inline void row_range_periodic(const Dev &D, const Params &P, int kx, int ky, int kz, int r,
+                                          int &beg0, int &end0, int &beg1, int &end1)
+{
+    int dy = (r % 3) - 1, dz = (P.dim == 3) ? (r / 3) - 1 : 0;
+    int yy = ky + dy, zz = kz + dz;
+    beg0 = end0 = beg1 = end1 = 0;
+    if (P.periodic[1]) yy = (yy + P.nc[1]) % P.nc[1];
+    if (P.periodic[2]) zz = (zz + P.nc[2]) % P.nc[2];
+    if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) return;
+    int base = P.nc[0]*(yy + P.nc[1]*zz);
+    int nx = P.nc[0];
+    if (P.periodic[0] && kx == 0) {
+        beg0 = D.cstart[base + nx - 1]; end0 = D.cstart[base + nx];
+        beg1 = D.cstart[base];          end1 = D.cstart[base + 2];
+    } else if (P.periodic[0] && kx == nx - 1) {
+        beg0 = D.cstart[base + nx - 2]; end0 = D.cstart[base + nx];
+        beg1 = D.cstart[base];          end1 = D.cstart[base + 1];
+    } else {
+        int x0 = kx > 0 ? kx - 1 : 0;
+        int x1 = kx + 1 < nx ? kx + 1 : nx - 1;
+        beg0 = D.cstart[base + x0];
+        end0 = D.cstart[base + x1 + 1];
+    }
+}
+
+// Periodic_X_Correction (Domain.cpp:110-115) of xij = x_i - x_j: a component longer than
+// 2 Cellfac h crosses the period and is shifted by DomSize, in the direction set by the two
+// particles' cell indices (those of this step's cell keys, as the reference's CC[]).
+__device__ inline void periodic_xij(const Dev &D, const Params &P, double h, int key_i, int j,
+                                    double &x0, double &x1, double &x2)
+{
+    const double lim = 2*P.cellfac*h;
+    const bool c0 = P.periodic[0] && (x0 > lim || x0 < -lim);
+    const bool c1 = P.periodic[1] && (x1 > lim || x1 < -lim);
+    const bool c2 = P.periodic[2] && (x2 > lim || x2 < -lim);
+    if (!(c0 || c1 || c2)) return;
+    int ix, iy, iz, jx, jy, jz;
+    decode_key(P, key_i, ix, iy, iz);
+    decode_key(P, D.key[j], jx, jy, jz);
+    if (c0) x0 = (ix > jx) ? x0 - P.domsize[0] : x0 + P.domsize[0];
+    if (c1) x1 = (iy > jy) ? x1 - P.domsize[1] : x1 + P.domsize[1];
+    if (c2) x2 = (iz > jz) ? x2 - P.domsize[2] : x2 + P.domsize[2];
+}
+
+// Whether particle i is P1 of its pair with j in the reference's search: P1 is the particle
+// of the home cell; the partner sits in the same cell (then P1 is the one linked later, the
+// larger original index) or at a forward offset of the half stencil (dz > 0, or dz = 0 and
+// dy > 0, or dz = dy = 0 and dx > 0; Domain.cpp:824-945).  Without periodic directions this is
+// the smaller cell key; across a period the offset is taken through the boundary.
+__device__ inline int pair_first(const Params &P, int key_i, int key_j, int orig_i, int orig_j)
+{
+    if (!P.any_periodic) return (key_i < key_j) | ((key_i == key_j) & (orig_i > orig_j));
+    int ix, iy, iz, jx, jy, jz;
+    decode_key(P, key_i, ix, iy, iz);
+    decode_key(P, key_j, jx, jy, jz);
+    int d[3] = {jx - ix, jy - iy, jz - iz};
+    for (int c = 0; c < 3; ++c)
+        if (P.periodic[c]) d[c] = (d[c] > 1) ? d[c] - P.nc[c] : (d[c] < -1) ? d[c] + P.nc[c] : d[c];
+    int zf = d[2] > 0, zs = d[2] == 0, yf = d[1] > 0, ys = d[1] == 0, xf = d[0] > 0, xs = d[0] == 0;
+    return zf | (zs & yf) | (zs & ys & xf) | (zs & ys & xs & (orig_i > orig_j));
+}
+
 
 __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, int cj)
 {
@@ -439,6 +504,9 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     for (int cc = 0; cc < ncell; ++cc) {
         int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (P.dim == 3) ? (cc / 9) - 1 : 0;
         int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+        if (P.periodic[0]) xx = (xx + P.nc[0]) % P.nc[0];
+        if (P.periodic[1]) yy = (yy + P.nc[1]) % P.nc[1];
+        if (P.periodic[2]) zz = (zz + P.nc[2]) % P.nc[2];
         if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
         int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
         if (D.cfree[cell] == 0) continue;                  // no free particle to gather from
@@ -454,6 +522,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
             double4 pj = D.pos_b[j];
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
             double h = pair_h(cp, P, ci, flag_cls(fj));
+            if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
             double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
             if (!((rij/h) <= P.cellfac)) continue;      // outside the reference's pair set
             // orientation of the reference's pair (P1 = smaller key / larger index); xij is
@@ -462,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
             // negation when this was written as `if (!(a || (b && c))) x = -x;`.
             int kj = D.key[j];
             int oj = D.orig_b[j];
-            int wall_first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
+            int wall_first = pair_first(P, key_i, kj, orig_i, oj);
             double sgn = wall_first ? 1.0 : -1.0;
             x0 *= sgn; x1 *= sgn; x2 *= sgn;
             double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h);
@@ -798,6 +867,7 @@ __device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *
     bool freej = flag_free(fj);
     double h = pair_h(cp, P, ci, cj);
     double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+    if (P.any_periodic) periodic_xij(D, P, h, D.key[i], j, x0, x1, x2);
     double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
     double Alpha = (Ci_.Alpha + Cj_.Alpha)/2.0;
     double Beta = (Ci_.Beta + Cj_.Beta)/2.0;
@@ -901,7 +971,9 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
     double4 aj = D.aux[j];
     bool freej = flag_free(fj);
     T h = (T)pair_h(cp, P, ci, cj);
-    T x0 = (T)(pi.x - pj.x), x1 = (T)(pi.y - pj.y), x2 = (T)(pi.z - pj.z);
+    double dx0 = pi.x - pj.x, dx1 = pi.y - pj.y, dx2 = pi.z - pj.z;
+    if (P.any_periodic) periodic_xij(D, P, pair_h(cp, P, ci, cj), D.key[i], j, dx0, dx1, dx2);
+    T x0 = (T)dx0, x1 = (T)dx1, x2 = (T)dx2;
     T rij = sqrt(x0*x0 + x1*x1 + x2*x2);
     T Alpha = (T)((Ci_.Alpha + Cj_.Alpha)/2.0);
     T Beta = (T)((Ci_.Beta + Cj_.Beta)/2.0);
@@ -1022,7 +1094,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     int zw_on = shep_on;
     constexpr bool F32 = sizeof(T) == 4;
     AccT<T> A;
-    A.a0 = F32 ? (T)0 : (T)P.gravity[0]; A.a1 = F32 ? (T)0 : (T)P.gravity[1]; A.a2 = F32 ? (T)0 : (T)P.gravity[2];
+    A.a0 = F32 ? (T)0 : (T)P.afree[0]; A.a1 = F32 ? (T)0 : (T)P.afree[1]; A.a2 = F32 ? (T)0 : (T)P.afree[2];
     A.dd = 0.0; A.ddc = 0.0;
     A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0;
     for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
@@ -1035,9 +1107,19 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     int xmat = 0;
     int cnt = 0;
     const int nrows = (P.dim == 3) ? 9 : 3;
-    for (int r = 0; r < nrows; ++r) {
+    for (int rs = 0; rs < 2*nrows; ++rs) {
+        // row r = rs/2; its second range is non-empty only when the row crosses a period
         int beg = 0, end = 0;
-        if (act) row_range(D, P, kx, ky, kz, r, beg, end);
+        if (act) {
+            if (P.any_periodic) {
+                int b0, e0, b1, e1;
+                row_range_periodic(D, P, kx, ky, kz, rs >> 1, b0, e0, b1, e1);
+                beg = (rs & 1) ? b1 : b0;
+                end = (rs & 1) ? e1 : e0;
+            } else if (!(rs & 1)) {
+                row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+            }
+        }
         for (int j = beg; j < end; ++j) {
             if (j != i) {
                 uint32_t fj = D.flags_b[j];
@@ -1047,6 +1129,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
                     double4 pj = D.pos_b[j];
                     double h = pair_h(cp, P, ci, flag_cls(fj));
                     double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                    if (P.any_periodic) periodic_xij(D, P, h, D.key[i], j, x0, x1, x2);
                     double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
                     if ((rij/h) <= P.cellfac) {
                         lst[cnt*kForceBlock + tid] = j;
@@ -1073,7 +1156,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     if (act) {
         double a0 = A.a0, a1 = A.a1, a2 = A.a2, dd = A.dd;
         if (F32) {
-            a0 = P.gravity[0] + (double)A.a0; a1 = P.gravity[1] + (double)A.a1; a2 = P.gravity[2] + (double)A.a2;
+            a0 = P.afree[0] + (double)A.a0; a1 = P.afree[1] + (double)A.a1; a2 = P.afree[2] + (double)A.a2;
             dd = (double)A.dd + (double)(-A.ddc);
         }
         D.acc[i] = make_double4(a0, a1, a2, dd);
@@ -1326,7 +1409,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
     int ci = (int)gi2.y;
     const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     Acc11 A;
-    A.a0 = P.gravity[0]; A.a1 = P.gravity[1]; A.a2 = P.gravity[2]; A.dd = 0.0;
+    A.a0 = P.afree[0]; A.a1 = P.afree[1]; A.a2 = P.afree[2]; A.dd = 0.0;
     unsigned nff = 0, nfw = 0;
     const int cap = P.nl_cap;
     const bool codes = P.nl_codes != 0;
@@ -1548,7 +1631,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
         act = false;
     }
     if (act) {
-        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
+        double a0 = P.afree[0] + (double)A.a0, a1 = P.afree[1] + (double)A.a1, a2 = P.afree[2] + (double)A.a2;
         D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
@@ -1600,7 +1683,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
             }
             ++k;
         });
-        double a0 = P.gravity[0] + (double)A.a0, a1 = P.gravity[1] + (double)A.a1, a2 = P.gravity[2] + (double)A.a2;
+        double a0 = P.afree[0] + (double)A.a0, a1 = P.afree[1] + (double)A.a1, a2 = P.afree[2] + (double)A.a2;
         D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
@@ -1887,6 +1970,12 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
                 D.eps_b[6*i+k] = ep[k]; D.epsh_b[6*i+k] = eh[k];
             }
         }
+        // periodic position update (Particle.cpp:127-142)
+        if (P.any_periodic) {
+            if (P.periodic[0]) { if (p.x > P.trpr[0]) p.x -= P.domsize[0]; if (p.x < P.blpf[0]) p.x += P.domsize[0]; }
+            if (P.periodic[1]) { if (p.y > P.trpr[1]) p.y -= P.domsize[1]; if (p.y < P.blpf[1]) p.y += P.domsize[1]; }
+            if (P.periodic[2]) { if (p.z > P.trpr[2]) p.z -= P.domsize[2]; if (p.z < P.blpf[2]) p.z += P.domsize[2]; }
+        }
         p.w = rho;
         v.w = P_;
         D.pos_b[i] = p;
@@ -1989,20 +2078,28 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev D0, Params P, int pass, co
     int nrows = (P.dim == 3) ? 9 : 3;
     long long w = (pass == 1) ? off[i] : 0;
     int c = 0;
-    for (int r = 0; r < nrows; ++r) {
-        int beg, end;
-        row_range(D, P, kx, ky, kz, r, beg, end);
+    for (int rs = 0; rs < 2*nrows; ++rs) {
+        int beg = 0, end = 0;
+        if (P.any_periodic) {
+            int b0, e0, b1, e1;
+            row_range_periodic(D, P, kx, ky, kz, rs >> 1, b0, e0, b1, e1);
+            beg = (rs & 1) ? b1 : b0;
+            end = (rs & 1) ? e1 : e0;
+        } else if (!(rs & 1)) {
+            row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+        }
         for (int j = beg; j < end; ++j) {
             if (j == i) continue;
             uint32_t fj = D.flags_b[j];
             if (!(flag_free(fi) || flag_free(fj))) continue;
             if (flag_mat(fi) != flag_mat(fj)) continue;
             int kj = D.key[j], oj = D.orig_b[j];
-            int first = (key_i < kj) | ((key_i == kj) & (orig_i > oj));
+            int first = pair_first(P, key_i, kj, orig_i, oj);
             if (!first) continue;
             double4 pj = D.pos_b[j];
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
             double h = pair_h(cp, P, flag_cls(fi), flag_cls(fj));
+            if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
             double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
             if ((rij/h) <= P.cellfac) {
                 if (pass == 1) { out[2*w] = orig_i; out[2*w+1] = oj; ++w; }
@@ -2011,6 +2108,51 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev D0, Params P, int pass, co
         }
     }
     if (pass == 0) cnt_out[i] = c;
+}
+
+// ------------------------------------------------------------------------------------------
+// visualisation output (Domain::WriteXDMF, Domain.cpp:1893-1928): the fields the reference
+// converts to float, rounded to nearest, in reference particle order.  out holds, for n
+// particles: position 3n | velocity 3n | acceleration 3n | pressure n | density n | mass n |
+// h n | tag n (int32) | sigma 6n | strain 6n.  acc_ok: the acceleration columns hold this
+// state's force pass; tensors: the run keeps Sigma and Strain on the device.  The columns are
+// the caller's choice of state buffer (no bind()).
+// ------------------------------------------------------------------------------------------
+constexpr int kOutFloats = 26;
+
+__global__ void __launch_bounds__(kBlock) k_pack_output(Dev D, Params P, float *out, int acc_ok, int tensors)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const size_t n = (size_t)P.n, o = (size_t)D.orig[i];
+    const uint32_t f = D.flags[i];
+    const ClassParams &cp = D.cls[flag_cls(f)];
+    const double4 x = D.pos[i], v = D.vel[i];
+    float *po = out + 3*o, *vo = out + 3*n + 3*o;
+    po[0] = (float)x.x; po[1] = (float)x.y; po[2] = (float)x.z;
+    vo[0] = (float)v.x; vo[1] = (float)v.y; vo[2] = (float)v.z;
+    if (acc_ok) {
+        float *ao = out + 6*n + 3*o;
+        if (flag_free(f)) {
+            const double4 a = D.acc[i];
+            ao[0] = (float)a.x; ao[1] = (float)a.y; ao[2] = (float)a.z;
+        } else {
+            // a fixed particle's acceleration is the StartAcceleration reset (Domain.cpp:1002)
+            ao[0] = (float)P.gravity[0]; ao[1] = (float)P.gravity[1]; ao[2] = (float)P.gravity[2];
+        }
+    }
+    out[9*n + o] = (float)v.w;
+    out[10*n + o] = (float)x.w;
+    out[11*n + o] = (float)D.mmu[i].x;
+    out[12*n + o] = (float)cp.h;
+    reinterpret_cast<int32_t *>(out)[13*n + o] = cp.ID;
+    if (tensors) {
+        float *so = out + 14*n + 6*o, *eo = out + 20*n + 6*o;
+        for (int c = 0; c < 6; ++c) {
+            so[c] = (float)D.sig[6*(size_t)i + c];
+            eo[c] = (float)D.eps[6*(size_t)i + c];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2121,5 +2263,10 @@ void launch_tail(const Dev &D, const Params &P, hipStream_t s)
 void launch_pairs(const Dev &D, const Params &P, hipStream_t s, int pass, const long long *off,
                   int *cnt, long long *out)
 { hipLaunchKernelGGL(k_pairs, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P, pass, off, cnt, out); }
+
+int output_floats() { return kOutFloats; }
+
+void launch_pack_output(const Dev &D, const Params &P, float *out, int acc_ok, int tensors, hipStream_t s)
+{ hipLaunchKernelGGL(k_pack_output, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P, out, acc_ok, tensors); }
 
 }  // namespace psph

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .abi import ERRORS, PARTICLE_DTYPE, Config, Grid, Status, grid_dict
+from .abi import ERRORS, OUTPUT_FIELDS, PARTICLE_DTYPE, Config, Grid, OutputF32, Status, grid_dict
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # PSPH_LIB names another build of the same library (same C ABI), e.g. to compare builds
@@ -18,6 +18,7 @@ EXPORTS = [
     "psph_create", "psph_upload", "psph_run", "psph_download", "psph_neighbour_pairs",
     "psph_get_grid", "psph_set_timing", "psph_kernel_stats", "psph_stream",
     "psph_traffic_model", "psph_destroy", "psph_last_error", "psph_version",
+    "psph_output_begin", "psph_output_end", "psph_set_body_force",
 ]
 
 _lib = None
@@ -43,6 +44,9 @@ def lib():
         L.psph_stream.argtypes = [p, ctypes.POINTER(p)]
         L.psph_traffic_model.argtypes = [p, ctypes.POINTER(d), ctypes.POINTER(d)]
         L.psph_destroy.argtypes = [p]
+        L.psph_output_begin.argtypes = [p]
+        L.psph_set_body_force.argtypes = [p, p]
+        L.psph_output_end.argtypes = [p, ctypes.POINTER(OutputF32)]
         L.psph_last_error.argtypes = [ctypes.c_char_p, sz]
         for f in EXPORTS:
             getattr(L, f).restype = ctypes.c_int
@@ -93,6 +97,35 @@ class Solver:
         out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
         _check(lib().psph_download(self._h, _ptr(out), n.value, ctypes.byref(n)))
         return out
+
+    def set_body_force(self, accel):
+        """Every free particle's acceleration starts each force pass at Gravity + accel (the
+        body-force GeneralBefore hook, psph_set_body_force)."""
+        a = np.ascontiguousarray(accel, dtype=np.float64)
+        _check(lib().psph_set_body_force(self._h, _ptr(a)))
+
+    def output_begin(self):
+        """Starts the packed FP32 output of the current state (psph_output_begin)."""
+        _check(lib().psph_output_begin(self._h))
+
+    def output_end(self, copy=True):
+        """The packed output started by output_begin: WriteXDMF's datasets as numpy copies,
+        (n, width) float32 (tag: int32) in reference particle order (copy=False: only waits
+        for the packed state and returns the particle count)."""
+        o = OutputF32()
+        _check(lib().psph_output_end(self._h, ctypes.byref(o)))
+        if not copy:
+            return o.n
+        out = {}
+        for name, dt, w in OUTPUT_FIELDS:
+            buf = (ctypes.c_char*(4*w*o.n)).from_address(getattr(o, name))
+            a = np.frombuffer(buf, dtype=dt).copy()
+            out[name] = a.reshape(o.n, w) if w > 1 else a
+        return out
+
+    def output_f32(self):
+        self.output_begin()
+        return self.output_end()
 
     def pairs(self):
         n = ctypes.c_size_t()

@@ -40,7 +40,7 @@ def test_particle_struct_layout(tmp_path):
 
 
 @pytest.mark.parametrize("cls,struct", [(abi.Config, "psph_config"), (abi.Status, "psph_status"),
-                                        (abi.Grid, "psph_grid")])
+                                        (abi.Grid, "psph_grid"), (abi.OutputF32, "psph_output_f32")])
 def test_small_struct_layouts(tmp_path, cls, struct):
     names = [f[0] for f in cls._fields_]
     off = _c_offsets(tmp_path, struct, names)
@@ -62,7 +62,7 @@ def test_library_exports_every_declared_symbol():
     for name in decl:
         assert hasattr(L, name), name
     assert sorted(S.EXPORTS) == decl
-    assert L.psph_version() == 100
+    assert L.psph_version() == 102
 
 
 def test_errors_without_device_or_bad_config():
@@ -72,7 +72,7 @@ def test_errors_without_device_or_bad_config():
     cfg = abi.default_config(dimension=4)
     assert L.psph_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
     cfg = abi.default_config(dimension=2)
-    cfg.periodic[0] = 1
+    cfg.periodic[2] = 1
     assert L.psph_create(ctypes.byref(cfg), ctypes.byref(h)) == -2
     buf = ctypes.create_string_buffer(256)
     n = L.psph_last_error(buf, 256)

@@ -14,9 +14,11 @@ SRC = EXE + ".cpp"
 
 
 def test_dropin_compiles(tmp_path):
+    import __graft_entry__ as g
+    cflags, _ = g.hdf5_flags()
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
-                    "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "persiansph_amd", "cpp"),
-                    SRC], check=True)
+                    "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "persiansph_amd", "cpp")]
+                   + cflags + [SRC], check=True)
 
 
 def _read(path):
@@ -44,3 +46,78 @@ def test_dropin_solve_matches_binding(tmp_path):
     np.testing.assert_array_equal(q["x"], got[:, :3])
     np.testing.assert_array_equal(q["Density"], got[:, 3])
     assert f"{st.time:.9e}" in r.stdout
+
+
+def _solve_like_dropin(s, tf, dt_out, maxidx, on_output=None):
+    """Domain::Solve's loop (Domain.cpp:1762-1793) over the Python binding: stop at each output
+    time, go on; after output maxidx the current step completes."""
+    tout, idx = 0.0, 1
+    while True:
+        st = s.run(10**9, tf=tf, tout=tout)
+        if st.stop_reason != 2:
+            return st
+        if on_output:
+            on_output(idx)
+        idx += 1
+        tout += dt_out
+        if idx > maxidx:
+            return s.run(1, tf=tf, tout=tout)
+
+
+@pytest.mark.gpu
+def test_poiseuille_dropin_matches_binding(tmp_path):
+    """1-Poiseuille.cpp through the drop-in (periodic x, body-force hook) equals the same run
+    through the Python binding of the C ABI, bit for bit."""
+    from persiansph_amd import Solver, cases
+    exe = os.path.join(ROOT, "persiansph_amd", "cpp", "examples", "poiseuille")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    cfg, p, dt, body = cases.poiseuille_2d()
+    tf, dt_out, maxidx = 60*dt, 20*dt, 2
+    out = tmp_path / "out.bin"
+    print(f"[dropin] poiseuille tf={tf:.3e}", flush=True)
+    r = subprocess.run([exe, repr(tf), repr(dt_out), "-", str(maxidx), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _read(out)
+    s = Solver(cfg, p, dt)
+    s.set_body_force(body)
+    st = _solve_like_dropin(s, tf, dt_out, maxidx)
+    q = s.particles()
+    assert len(q) == len(got)
+    np.testing.assert_array_equal(q["x"], got[:, :3])
+    np.testing.assert_array_equal(q["v"][:, 0], got[:, 3])
+    assert f"{st.time:.9e}" in r.stdout
+    s.close()
+
+
+@pytest.mark.gpu
+def test_couette_dropin_user_output(tmp_path):
+    """2-Couette.cpp through the drop-in: its UserOutput puts the shear rate into the XDMF
+    output as "ShearRate"; the values are those of the binding's state at that output."""
+    import test_xdmf as X
+    from persiansph_amd import Solver, cases
+    exe = os.path.join(ROOT, "persiansph_amd", "cpp", "examples", "couette")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    tool = X._tool(tmp_path)
+    cfg, p, dt = cases.couette_2d()
+    tf, dt_out, maxidx = 50*dt, 20*dt, 3
+    key = str(tmp_path / "cou")
+    print(f"[dropin] couette tf={tf:.3e}", flush=True)
+    r = subprocess.run([exe, repr(tf), repr(dt_out), key, str(maxidx)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s = Solver(cfg, p, dt)
+    states = {}
+
+    def grab(idx):
+        states[idx] = s.particles()
+    _solve_like_dropin(s, tf, dt_out, maxidx, grab)
+    assert sorted(states) == [1, 2, 3]
+    for idx, q in states.items():
+        d = X._read(tool, f"{key}_{idx:04d}.hdf5")
+        np.testing.assert_array_equal(d["ShearRate"], q["ShearRate"].astype(np.float32))
+        np.testing.assert_array_equal(d["Position"], q["x"].astype(np.float32).ravel())
+        np.testing.assert_array_equal(d["Velocity"], q["v"].astype(np.float32).ravel())
+        assert not d["Property2"].any() and not d["Property3"].any()
+    assert np.abs(states[3]["ShearRate"]).max() > 0.0
+    s.close()
