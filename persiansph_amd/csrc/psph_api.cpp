@@ -951,6 +951,15 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     h->nl = h->wl = nullptr;
     if (P.mixed32 && !P.nl_codes)
         return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier holds at most 2^27 particles");
+    P.glist = 0;
+    if (!P.fast11 && !P.mixed32 && P.nfree > 0 && !std::getenv("PSPH_NO_GLIST")) {
+        // pair lists of the general kernels: radius Cellfac hmax (the largest pair support)
+        int cap0 = nl_initial_cap(h, P.cellfac*P.hmax);
+        if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);
+        rc = nl_alloc(h, cap0);
+        if (rc) return rc;
+        P.glist = 1;
+    }
     if (P.fast11 || P.mixed32) {
         double radius = P.verlet ? std::sqrt(P.reach2) : support;
         int cap0 = nl_initial_cap(h, radius);

@@ -116,6 +116,10 @@ struct Params {
     // DomSize, the period
     int any_periodic, periodic[3];
     double domsize[3];
+    // general kernels: the pairs of each free target are listed by a light kernel (k_glist) in
+    // the force kernel's walk order, in nl/ncnt with capacity nl_cap; the force kernel walks
+    // the cells itself only for the entries beyond the capacity
+    int glist;
     // a free particle's acceleration at the start of its force pass: Gravity plus the body force
     // of the body-force hook (GeneralBefore adding a constant to every free particle's a)
     double afree[3];

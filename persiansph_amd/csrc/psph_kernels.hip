@@ -1068,6 +1068,82 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
 }
 
 // T = float (SOLID only): the FP32-mixed soil tier; gravity is added in FP64 afterwards.
+// A lane's list in slot groups of four (one 16-byte store per group): entry e of the lane at
+// out[(e/4)*gs] component e%4.  Entries past the capacity are counted, not stored.
+struct ListWriter {
+    int4 *out;
+    size_t gs;
+    int cap, cnt;
+    int w0, w1, w2, w3;
+    __device__ ListWriter(int4 *o, size_t g, int c) : out(o), gs(g), cap(c), cnt(0), w0(0), w1(0), w2(0), w3(0) {}
+    __device__ void push(int v)
+    {
+        int sl = cnt & 3;
+        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
+        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+        ++cnt;
+    }
+    __device__ void flush()
+    {
+        if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+    }
+};
+
+// Pair lists of the general kernels: each free target's pairs of this step, in the order the
+// force kernel's row walk meets them (rows of three cells, y then z; a row across a period in
+// two ranges), by the reference's test rij/h <= Cellfac on the periodic-corrected xij.  The
+// candidates' tests run here at high occupancy instead of inside the force kernel's
+// register-heavy loop.  Entries beyond the capacity are counted (ncnt), raise Ctl::nl_over (the
+// host grows the lists) and are found again by the force kernel.  A cross-material candidate
+// raises Ctl::xmat.
+__global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    __shared__ double hcls[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) hcls[k] = D.cls[k].h;
+    __syncthreads();
+    int t = xcd_block()*kBlock + threadIdx.x;
+    if (t >= P.nfree) return;
+    int i = D.free_list[t];
+    uint32_t fi = D.flags_b[i];
+    int ci = flag_cls(fi), mat = flag_mat(fi);
+    double4 pi = D.pos_b[i];
+    int key_i = D.key[i];
+    int kx, ky, kz;
+    decode_key(P, key_i, kx, ky, kz);
+    ListWriter out(reinterpret_cast<int4 *>(D.nl) + t, (size_t)P.nfree, P.nl_cap);
+    int xmat = 0;
+    const int nrows = (P.dim == 3) ? 9 : 3;
+    for (int rs = 0; rs < 2*nrows; ++rs) {
+        int beg = 0, end = 0;
+        if (P.any_periodic) {
+            int b0, e0, b1, e1;
+            row_range_periodic(D, P, kx, ky, kz, rs >> 1, b0, e0, b1, e1);
+            beg = (rs & 1) ? b1 : b0;
+            end = (rs & 1) ? e1 : e0;
+        } else if (!(rs & 1)) {
+            row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+        }
+        for (int j = beg; j < end; ++j) {
+            if (j == i) continue;
+            uint32_t fj = D.flags_b[j];
+            if (flag_mat(fj) != mat) { xmat = 1; continue; }
+            double4 pj = D.pos_b[j];
+            double h = P.uniform_h ? P.h_uniform : (hcls[ci] + hcls[flag_cls(fj)])/2;
+            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+            if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
+            double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+            if ((rij/h) <= P.cellfac) out.push(j);
+        }
+    }
+    out.flush();
+    D.ncnt[t] = out.cnt;
+    if (out.cnt > P.nl_cap) atomicAdd(&D.ctl->nl_over, 1);
+    int anyx = wave_any(xmat != 0);
+    if ((threadIdx.x & 63) == 0 && anyx) atomicOr(&D.ctl->xmat, 1);
+}
+
 template <bool SOLID, class T>
 __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
 {
@@ -1106,11 +1182,28 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     unsigned long long nff = 0, nfw = 0;
     int xmat = 0;
     int cnt = 0;
+    // listed pairs first (k_glist, same order as the walk below); the walk then runs only for
+    // a target whose pairs overflowed the list, and skips the `skip` pairs already taken
+    int skip = 0;
+    bool walk = act;
+    if (P.glist && act) {
+        const int total = D.ncnt[t], m = total < P.nl_cap ? total : P.nl_cap;
+        const int *lst_g = D.nl;
+        for (int e = 0; e < m; ++e) {
+            int jj = lst_g[((size_t)(e >> 2)*P.nfree + t)*4 + (e & 3)];
+            if (flag_free(D.flags_b[jj])) ++nff; else ++nfw;
+            if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+            else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+        }
+        skip = m;
+        walk = total > P.nl_cap;
+    }
+    int seen = 0;
     const int nrows = (P.dim == 3) ? 9 : 3;
     for (int rs = 0; rs < 2*nrows; ++rs) {
         // row r = rs/2; its second range is non-empty only when the row crosses a period
         int beg = 0, end = 0;
-        if (act) {
+        if (walk) {
             if (P.any_periodic) {
                 int b0, e0, b1, e1;
                 row_range_periodic(D, P, kx, ky, kz, rs >> 1, b0, e0, b1, e1);
@@ -1131,7 +1224,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
                     double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
                     if (P.any_periodic) periodic_xij(D, P, h, D.key[i], j, x0, x1, x2);
                     double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-                    if ((rij/h) <= P.cellfac) {
+                    if ((rij/h) <= P.cellfac && seen++ >= skip) {
                         lst[cnt*kForceBlock + tid] = j;
                         ++cnt;
                         if (flag_free(fj)) ++nff; else ++nfw;
@@ -1342,26 +1435,6 @@ __device__ inline void walk_rows(const Dev &D, const Params &P, int i, double px
     }
 }
 
-// A lane's list in slot groups of four (one 16-byte store per group): entry e of the lane at
-// out[(e/4)*gs] component e%4.  Entries past the capacity are counted, not stored.
-struct ListWriter {
-    int4 *out;
-    size_t gs;
-    int cap, cnt;
-    int w0, w1, w2, w3;
-    __device__ ListWriter(int4 *o, size_t g, int c) : out(o), gs(g), cap(c), cnt(0), w0(0), w1(0), w2(0), w3(0) {}
-    __device__ void push(int v)
-    {
-        int sl = cnt & 3;
-        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
-        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-        ++cnt;
-    }
-    __device__ void flush()
-    {
-        if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-    }
-};
 
 // List build.  REACH = 1: the pairs of this step.  REACH = 2 (list reuse, steps that rebuild
 // only): the skin list, every candidate within the support radius plus the skin; one that
@@ -2231,7 +2304,9 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
 
 static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
 {
-    if (P.nl_cap > 0) {
+    if (P.glist)
+        hipLaunchKernelGGL(k_glist, dim3(nblk(P.nfree, kBlock)), dim3(kBlock), 0, s, D, P);
+    if (P.nl_cap > 0 && !P.glist) {
         if (P.dim == 3) {
             if (P.verlet) launch_list_pass<3, 2>(D, P, s);
             else launch_list_pass<3, 1>(D, P, s);

@@ -96,3 +96,33 @@ def test_variant_force_pass_and_trajectory(name):
     np.testing.assert_allclose(G["x"], O["x"], rtol=0, atol=1e-11)
     np.testing.assert_allclose(G["Density"], O["Density"], rtol=1e-11, atol=0)
     g.close()
+
+
+@pytest.mark.parametrize("name", ["takeda", "artificial_stress"])
+def test_general_pair_lists_are_exact(name, monkeypatch):
+    """The general kernels take each target's pairs from a list built in the walk order; the
+    walk itself serves only entries beyond the capacity.  Lists of capacity 8 (most targets
+    overflow, the host then grows them) and no lists at all give bitwise the same run."""
+    cfg, p, dt = _variant(**VARIANTS[name])
+    runs = {}
+    for mode in ("lists", "cap8", "nolists"):
+        monkeypatch.delenv("PSPH_NL_CAP", raising=False)
+        monkeypatch.delenv("PSPH_NO_GLIST", raising=False)
+        if mode == "cap8":
+            monkeypatch.setenv("PSPH_NL_CAP", "8")
+        if mode == "nolists":
+            monkeypatch.setenv("PSPH_NO_GLIST", "1")
+        print(f"[variant] {name} {mode}", flush=True)
+        g = Solver(cfg, p, dt)
+        st1 = g.run(1, tout=0.0)
+        a1 = g.particles()["a"].copy()
+        st = g.run(12)
+        q = g.particles()
+        runs[mode] = (st1.pairs, a1, q["x"].copy(), q["Density"].copy(), st.pairs)
+        g.close()
+    ref = runs["lists"]
+    for mode in ("cap8", "nolists"):
+        got = runs[mode]
+        assert got[0] == ref[0] and got[4] == ref[4], mode
+        for k in (1, 2, 3):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=mode)
