@@ -86,10 +86,14 @@ typedef struct psph_particle {
     double Alpha, Beta, MuRef, T0, m, CSmag;
     double TI, TIn, TIInitDist;
     double G, K, Sigmay, c, phi, psi;
+    double FSIPressure, FSISumKernel, FSINSv[3];   /* fluid-solid interaction (Domain.cpp:1089-1160) */
+    double n, n0, k, k2, d, V, RhoF, S;            /* soil-water interaction (Interaction.cpp:510-653) */
     int32_t IsFree, NoSlip, Shepard, LES;
     int32_t Material, PresEq, VisM, Fail;
     int32_t ID, ct, FirstStep, InOut;
     int32_t ShepardCounter, ShepardStep;
+    int32_t SeepageType, IsSat, SatCheck;
+    int32_t reserved2;
     int32_t Index;      /* out: position of this particle in the array given to psph_upload */
     int32_t reserved;
 } psph_particle;

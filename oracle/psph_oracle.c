@@ -676,8 +676,9 @@ typedef struct OP {
     psph_particle p;
     int LL;
     int CC[3];
+    int fsi_queued;  /* FSI pairs that queued this particle for the division this step */
     omp_lock_t lock;
-    double sc[24];   /* |contribution| sums (see psph_o_scales) */
+    double sc[32];   /* |contribution| sums (see psph_o_scales) */
 } OP;
 
 typedef struct { int64_t *v; size_t n, cap; } pairlist;
@@ -703,7 +704,7 @@ struct psph_odomain {
     double TRPR[3], BLPF[3], CellSize[3], hmax, rhomax, MuMax, CsMax;
     int CellNo[3];
     int *HOC;
-    pairlist *SM, *FSM;
+    pairlist *SM, *FSM, *NSM;
     int64_t *nsm;
     size_t *fixed;
     size_t nfixed;
@@ -846,6 +847,7 @@ static inline void classify(psph_odomain *d, int T, int t1, int t2)
             if (p1->IsFree*p2->IsFree) pl_push(&d->SM[T], t1, t2);
             else pl_push(&d->FSM[T], t1, t2);
         } else {
+            pl_push(&d->NSM[T], t1, t2);
             d->nsm[T]++;
         }
     }
@@ -912,6 +914,11 @@ static void start_acceleration(psph_odomain *d)
         if (dim == 2) p->v[2] = 0.0;
         memset(p->StrainRate, 0, sizeof p->StrainRate);
         memset(p->RotationRate, 0, sizeof p->RotationRate);
+        p->SatCheck = 0;
+        p->FSISumKernel = 0.0;
+        memset(p->FSINSv, 0, sizeof p->FSINSv);
+        p->FSIPressure = 0.0;
+        p->S = 0.0;
         if (d->track) memset(d->P[i]->sc, 0, sizeof d->P[i]->sc);
     }
 }
@@ -950,6 +957,50 @@ static void primary_compute_acceleration(psph_odomain *d)
                 }
             omp_unset_lock(&ow->lock);
         }
+        /* soil next to water, from above (Domain.cpp:1060-1088) */
+        pairlist *NL = &d->NSM[k];
+        if (d->cfg.swi_type != 2)
+            for (size_t a = 0; a < NL->n; a++) {
+                OP *o1 = d->P[NL->v[2*a]], *o2 = d->P[NL->v[2*a+1]];
+                psph_particle *P1 = &o1->p, *P2 = &o2->p;
+                if (P1->Material == 3 && P1->Material*P2->Material == 3)
+                    if (!P1->SatCheck && o2->CC[1] >= o1->CC[1] && P2->x[1] >= P1->x[1]) {
+                        omp_set_lock(&o1->lock); P1->SatCheck = 1; omp_unset_lock(&o1->lock);
+                    }
+                if (P2->Material == 3 && P1->Material*P2->Material == 3)
+                    if (!P2->SatCheck && o1->CC[1] >= o2->CC[1] && P1->x[1] >= P2->x[1]) {
+                        omp_set_lock(&o2->lock); P2->SatCheck = 1; omp_unset_lock(&o2->lock);
+                    }
+            }
+        /* fluid-solid smoothing onto the free solid particles (Domain.cpp:1089-1144): the
+         * FSI velocity takes only the pairs in which the solid particle is P1 (the other
+         * orientation adds to NSv), and every pair queues the solid particle for the division
+         * once more */
+        if (d->cfg.fsi)
+            for (size_t a = 0; a < NL->n; a++) {
+                OP *o1 = d->P[NL->v[2*a]], *o2 = d->P[NL->v[2*a+1]];
+                psph_particle *P1 = &o1->p, *P2 = &o2->p;
+                if (P1->Material*P2->Material == 2 && (P1->IsFree*P2->IsFree)) {
+                    double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+                    double h = (P1->h+P2->h)/2.0;
+                    periodic_correction(d, xij, h, o1, o2);
+                    double K = psph_o_kernel(dim, KT, norm3(xij)/h, h);
+                    OP *os = (P1->Material == 1) ? o2 : o1;
+                    psph_particle *F = (P1->Material == 1) ? P1 : P2, *Sd = &os->p;
+                    omp_set_lock(&os->lock);
+                    os->fsi_queued++;
+                    Sd->FSISumKernel += K;
+                    if (P1->Material == 1) for (int c = 0; c < 3; ++c) Sd->NSv[c] += F->v[c] * K;
+                    else for (int c = 0; c < 3; ++c) Sd->FSINSv[c] += F->v[c] * K;
+                    double t = F->Pressure * K + dot3(G, xij)*F->Density*K;
+                    Sd->FSIPressure += t;
+                    if (d->track) {
+                        os->sc[24] += fabs(t);
+                        if (P1->Material != 1) for (int c = 0; c < 3; ++c) os->sc[25+c] += fabs(F->v[c]*K);
+                    }
+                    omp_unset_lock(&os->lock);
+                }
+            }
     }
     #pragma omp parallel for schedule(static) num_threads(d->nthreads)
     for (size_t i = 0; i < d->nfixed; i++) {
@@ -966,6 +1017,42 @@ static void primary_compute_acceleration(psph_odomain *d)
             if (p->Material > 1 && p->TI > 0.0) psph_o_tir(dim, p->Sigma, p->TI, p->Density, p->TIR);
         }
     }
+    /* FSI division (Domain.cpp:1146-1160), once per queued pair */
+    if (d->cfg.fsi)
+        for (size_t i = 0; i < d->n; i++) {
+            OP *o = d->P[i];
+            psph_particle *p = &o->p;
+            for (int q = 0; q < o->fsi_queued; ++q) {
+                p->FSIPressure = p->FSIPressure/p->FSISumKernel;
+                for (int c = 0; c < 3; ++c) p->FSINSv[c] = p->FSINSv[c]/p->FSISumKernel;
+                if (d->track) {
+                    o->sc[24] /= p->FSISumKernel;
+                    for (int c = 0; c < 3; ++c) o->sc[25+c] /= p->FSISumKernel;
+                }
+            }
+            o->fsi_queued = 0;
+        }
+    /* saturation of soil next to water (Domain.cpp:1209-1233) */
+    if (d->cfg.swi_type != 2)
+        for (size_t i = 0; i < d->n; i++) {
+            psph_particle *p = &d->P[i]->p;
+            if (p->Material == 3) {
+                if (p->SatCheck && !p->IsSat) {
+                    p->Mass = p->V*(p->RefDensity - p->RhoF);
+                    p->Density = p->Density - p->RhoF;
+                    p->Densityb = p->Densityb - p->RhoF;
+                    p->RefDensity = p->RefDensity - p->RhoF;
+                    p->IsSat = 1;
+                }
+                if (!p->SatCheck && p->IsSat) {
+                    p->Mass = p->V*(p->RefDensity + p->RhoF);
+                    p->Density = p->Density + p->RhoF;
+                    p->Densityb = p->Densityb + p->RhoF;
+                    p->RefDensity = p->RefDensity + p->RhoF;
+                    p->IsSat = 0;
+                }
+            }
+        }
 }
 
 static inline void sc_add3(double *sc, const double *v, double s)
@@ -1063,6 +1150,11 @@ static void calc_force11(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     P1->dDensity += dd1;
     if (P1->IsFree) {
         if (sr_on && (P1->T0 > 0.0 || P1->LES)) { double f = mj/dj; for (int q = 0; q < 9; ++q) P1->StrainRate[q] = P1->StrainRate[q] + SR[q]*f; }
+        if (d->cfg.swi_type == 1) {
+            double t = mj/dj*vab[0]*xij[1]*-GK;
+            P1->S = P1->S + t;
+            if (d->track) o1->sc[28] += fabs(t);
+        }
         P1->ZWab += mj/dj* K;
     } else P1->ZWab = 1.0;
     if (P1->Shepard && P1->ShepardCounter == P1->ShepardStep) P1->SumDen += mj*K;
@@ -1087,6 +1179,11 @@ static void calc_force11(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     P2->dDensity += dd2;
     if (P2->IsFree) {
         if (sr_on && (P2->T0 > 0.0 || P2->LES)) { double f = mi/di; for (int q = 0; q < 9; ++q) P2->StrainRate[q] = P2->StrainRate[q] + SR[q]*f; }
+        if (d->cfg.swi_type == 1) {
+            double t = mi/di*vab[0]*xij[1]*-GK;
+            P2->S = P2->S + t;
+            if (d->track) o2->sc[28] += fabs(t);
+        }
         P2->ZWab += mi/di* K;
     } else P2->ZWab = 1.0;
     if (P2->Shepard && P2->ShepardCounter == P2->ShepardStep) P2->SumDen += mi*K;
@@ -1211,6 +1308,11 @@ static void calc_force2233(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
         double f = mj/dj;
         for (int q = 0; q < 9; ++q) P1->StrainRate[q] = P1->StrainRate[q] + SR[q]*f;
         for (int q = 0; q < 9; ++q) P1->RotationRate[q] = P1->RotationRate[q] + RR[q]*f;
+        if (d->cfg.swi_type == 1) {
+            double t = mj/dj*vab[0]*xij[1]*-GK;
+            P1->S = P1->S + t;
+            if (d->track) o1->sc[28] += fabs(t);
+        }
     } else P1->ZWab = 1.0;
     if (P1->Shepard && P1->ShepardCounter == P1->ShepardStep) P1->SumDen += mj* K;
     if (d->track) {
@@ -1239,6 +1341,11 @@ static void calc_force2233(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
         double f = mi/di;
         for (int q = 0; q < 9; ++q) P2->StrainRate[q] = P2->StrainRate[q] + SR[q]*f;
         for (int q = 0; q < 9; ++q) P2->RotationRate[q] = P2->RotationRate[q] + RR[q]*f;
+        if (d->cfg.swi_type == 1) {
+            double t = mi/di*vab[0]*xij[1]*-GK;
+            P2->S = P2->S + t;
+            if (d->track) o2->sc[28] += fabs(t);
+        }
     } else P2->ZWab = 1.0;
     if (P2->Shepard && P2->ShepardCounter == P2->ShepardStep) P2->SumDen += mi* K;
     if (d->track) {
@@ -1254,13 +1361,153 @@ static void calc_force2233(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     omp_unset_lock(&o2->lock);
 }
 
+
+/* CalcForce12, Interaction.cpp:379-506: fluid against solid.  The fluid particle's EOS turns the
+ * solid particle's FSI pressure into a density; both masses are the fluid particle's; the
+ * artificial viscosity term is compiled out in the reference (Alpha = Beta = 0 before the test) */
+static void calc_force12(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
+{
+    psph_particle *P1 = &o1->p, *P2 = &o2->p;
+    int Dimension = d->cfg.dimension, KernelType = d->cfg.kernel_type;
+    double h = (P1->h+P2->h)/2;
+    double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+    periodic_correction(d, xij, h, o1, o2);
+    double rij = norm3(xij);
+    if (!((rij/h) <= d->cfg.cellfac)) return;
+    (*cnt)++;
+    double di, dj, mi, mj;
+    double vij[3] = {P1->v[0]-P2->v[0], P1->v[1]-P2->v[1], P1->v[2]-P2->v[2]};
+    double GK = psph_o_grad_kernel(Dimension, KernelType, rij/h, h);
+    int f1 = P1->Material == 1;
+    if (f1) {
+        di = P1->Density; mi = P1->Mass;
+        dj = psph_o_density_solid(P1->PresEq, P1->Cs, P1->P0, P2->FSIPressure, P1->RefDensity);
+        mj = P1->Mass;
+    } else {
+        di = psph_o_density_solid(P2->PresEq, P2->Cs, P2->P0, P1->FSIPressure, P2->RefDensity);
+        mi = P2->Mass;
+        dj = P2->Density;
+        mj = P2->Mass;
+    }
+    double PIij = 0.0;
+    double SR[9] = {0}, VI[3] = {0, 0, 0}, vab[3], Mu;
+    if (f1) { for (int c = 0; c < 3; ++c) vab[c] = P1->v[c] - (2.0*P2->v[c]-P2->FSINSv[c]); Mu = P1->Mu; }
+    else { for (int c = 0; c < 3; ++c) vab[c] = (2.0*P1->v[c]-P1->FSINSv[c]) - P2->v[c]; Mu = P2->Mu; }
+    int sr_on = (P1->T0 > 0.0 || P2->T0 > 0.0) || (P1->LES || P2->LES);
+    if (sr_on) {
+        SR[0] = 2.0*vab[0]*xij[0];           SR[1] = vab[0]*xij[1]+vab[1]*xij[0]; SR[2] = vab[0]*xij[2]+vab[2]*xij[0];
+        SR[3] = vab[0]*xij[1]+vab[1]*xij[0]; SR[4] = 2.0*vab[1]*xij[1];           SR[5] = vab[1]*xij[2]+vab[2]*xij[1];
+        SR[6] = vab[0]*xij[2]+vab[2]*xij[0]; SR[7] = vab[1]*xij[2]+vab[2]*xij[1]; SR[8] = 2.0*vab[2]*xij[2];
+        for (int q = 0; q < 9; ++q) SR[q] = SR[q]*(-GK);
+    }
+    psph_o_viscous_force(d->cfg.visc_eq, VI, Mu, di, dj, GK, vab, Dimension, KernelType, rij, h, xij, vij);
+    double temp[3], temp1, gx[3], sfac;
+    double Pi_ = f1 ? P1->Pressure : P1->FSIPressure, Pj_ = f1 ? P2->FSIPressure : P2->Pressure;
+    if (d->cfg.gradient_type == 0) sfac = -1.0*( Pi_/(di*di) + Pj_/(dj*dj) + PIij ) * GK;
+    else sfac = -1.0*( (Pi_ + Pj_)/(di*dj) + PIij ) * GK;
+    for (int c = 0; c < 3; ++c) temp[c] = sfac*xij[c] + VI[c];
+    if (Dimension == 2) temp[2] = 0.0;
+    for (int c = 0; c < 3; ++c) gx[c] = GK*xij[c];
+    temp1 = dot3(vij, gx);
+    if (f1) {
+        omp_set_lock(&o1->lock);
+        for (int c = 0; c < 3; ++c) P1->a[c] += mj * temp[c];
+        double dd = mj * (di/dj) * temp1;
+        P1->dDensity += dd;
+        if (P1->T0 > 0.0 || P1->LES) { double f = mj/dj; for (int q = 0; q < 9; ++q) P1->StrainRate[q] = P1->StrainRate[q] + SR[q]*f; }
+        if (d->track) { sc_add3(o1->sc, temp, mj); o1->sc[3] += fabs(dd); }
+        omp_unset_lock(&o1->lock);
+        omp_set_lock(&o2->lock);
+        for (int c = 0; c < 3; ++c) P2->a[c] -= mi * temp[c];
+        if (d->track) sc_add3(o2->sc, temp, mi);
+        omp_unset_lock(&o2->lock);
+    } else {
+        omp_set_lock(&o1->lock);
+        for (int c = 0; c < 3; ++c) P1->a[c] += mj * temp[c];
+        if (d->track) sc_add3(o1->sc, temp, mj);
+        omp_unset_lock(&o1->lock);
+        omp_set_lock(&o2->lock);
+        for (int c = 0; c < 3; ++c) P2->a[c] -= mi * temp[c];
+        double dd = mi * (dj/di) * temp1;
+        P2->dDensity += dd;
+        if (P2->T0 > 0.0 || P2->LES) { double f = mi/di; for (int q = 0; q < 9; ++q) P2->StrainRate[q] = P2->StrainRate[q] + SR[q]*f; }
+        if (d->track) { sc_add3(o2->sc, temp, mi); o2->sc[3] += fabs(dd); }
+        omp_unset_lock(&o2->lock);
+    }
+}
+
+/* Seepage, Functions.cpp:266-298 */
+static int seepage(int ST, double k, double k2, double mu, double rho, double *SF1, double *SF2)
+{
+    switch (ST) {
+    case 0: case 1: *SF1 = mu/k; *SF2 = 0.0; return 0;
+    case 2: case 3: *SF1 = mu/k; *SF2 = k2*rho; return 0;
+    default: return set_err(PSPH_ERR_INVALID, "Seepage Type No is out of range");
+    }
+}
+
+/* CalcForce13, Interaction.cpp:508-653: soil against water, by SWIType: 0 seepage force,
+ * 1 seepage or (ZWab < 0.25) drag and lift of surface erosion, 2 seepage force and the pore
+ * pressure of the water particle, 3 none.  h is the smaller of the two. */
+static int calc_force13(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
+{
+    psph_particle *P1 = &o1->p, *P2 = &o2->p;
+    int Dimension = d->cfg.dimension, KernelType = d->cfg.kernel_type;
+    double h = fmin(P1->h, P2->h);
+    double xij[3] = {P1->x[0]-P2->x[0], P1->x[1]-P2->x[1], P1->x[2]-P2->x[2]};
+    periodic_correction(d, xij, h, o1, o2);
+    double rij = norm3(xij);
+    if (!((rij/h) <= d->cfg.cellfac)) return 0;
+    (*cnt)++;
+    double K = psph_o_kernel(Dimension, KernelType, rij/h, h)/(P1->Density*P2->Density);
+    double SF1 = 0.0, SF2 = 0.0, SFt[3] = {0, 0, 0}, v[3];
+    int s1 = P1->Material == 3;          /* P1 is the soil particle */
+    psph_particle *So = s1 ? P1 : P2, *W = s1 ? P2 : P1;
+    OP *oS = s1 ? o1 : o2, *oW = s1 ? o2 : o1;
+    int swi = d->cfg.swi_type, rc;
+    if (swi == 3) return 0;
+    if (swi < 0 || swi > 3) return set_err(PSPH_ERR_INVALID, "Soil-Water Interaction Type No is out of range");
+    for (int c = 0; c < 3; ++c) v[c] = W->v[c] - So->v[c];
+    double nv = norm3(v);
+    if (swi == 1 && So->ZWab < 0.25) {
+        double Cd = 24.0*(W->MuRef/W->RefDensity)/(So->d*nv+0.01*h*h) + 2.0;
+        for (int c = 0; c < 3; ++c) SFt[c] = (3.0/(4.0*So->d)*W->RefDensity*(1.0-So->n0)*Cd*nv*v[c]) *K;
+        SFt[1] += (W->RefDensity*(1.0-So->n0)*nv*fabs(W->S-So->S)) *K;
+    } else {
+        rc = seepage(So->SeepageType, So->k, So->k2, W->MuRef, W->RefDensity, &SF1, &SF2);
+        if (rc) return rc;
+        for (int c = 0; c < 3; ++c) SFt[c] = (SF1*v[c] + SF2*nv*v[c]) *K;
+    }
+    if (Dimension == 2) SFt[2] = 0.0;
+    /* the soil particle gains m_w SFt, the water particle loses m_s SFt; with SWIType 2 the
+     * soil particle also feels the water particle's pressure */
+    double pore[3] = {0, 0, 0};
+    if (swi == 2) {
+        double GK = psph_o_grad_kernel(Dimension, KernelType, rij/h, h)/(P1->Density*P2->Density);
+        for (int c = 0; c < 3; ++c) pore[c] = W->Mass*W->Pressure*GK*xij[c];
+    }
+    omp_set_lock(&oS->lock);
+    for (int c = 0; c < 3; ++c) {
+        double t = W->Mass*SFt[c];
+        if (swi == 2) t = s1 ? t - pore[c] : t + pore[c];
+        So->a[c] += t;
+        if (d->track) oS->sc[c] += fabs(t);
+    }
+    omp_unset_lock(&oS->lock);
+    omp_set_lock(&oW->lock);
+    for (int c = 0; c < 3; ++c) {
+        W->a[c] -= So->Mass*SFt[c];
+        if (d->track) oW->sc[c] += fabs(So->Mass*SFt[c]);
+    }
+    omp_unset_lock(&oW->lock);
+    return 0;
+}
+
 /* LastComputeAcceleration, Domain.cpp:1238-1297 */
 static int last_compute_acceleration(psph_odomain *d)
 {
-    int64_t total = 0, nsm = 0;
-    for (int k = 0; k < d->nthreads; ++k) nsm += d->nsm[k];
-    if (nsm > 0)
-        return set_err(PSPH_ERR_UNSUPPORTED, "%lld pairs between different materials (CalcForce12/13) are outside this path", (long long)nsm);
+    int64_t total = 0;
+    int err = 0;
     #pragma omp parallel for schedule(static) num_threads(d->nthreads) reduction(+:total)
     for (int k = 0; k < d->nthreads; k++) {
         int64_t cnt = 0;
@@ -1276,8 +1523,27 @@ static int last_compute_acceleration(psph_odomain *d)
         }
         total += cnt;
     }
+    /* pairs of different materials, after all others (Domain.cpp:1256-1271) */
+    #pragma omp parallel for schedule(static) num_threads(d->nthreads) reduction(+:total)
+    for (int k = 0; k < d->nthreads; k++) {
+        int64_t cnt = 0;
+        pairlist *L = &d->NSM[k];
+        for (size_t i = 0; i < L->n; i++) {
+            OP *o1 = d->P[L->v[2*i]], *o2 = d->P[L->v[2*i+1]];
+            int prod = o1->p.Material*o2->p.Material, rc = 0;
+            if (prod == 3) rc = calc_force13(d, o1, o2, &cnt);
+            else if (prod == 2) calc_force12(d, o1, o2, &cnt);
+            else rc = set_err(PSPH_ERR_INVALID, "Out of Interaction types (materials %d and %d)", o1->p.Material, o2->p.Material);
+            if (rc) {
+                #pragma omp critical
+                err = rc;
+            }
+        }
+        total += cnt;
+    }
     d->last_pairs = total;
-    for (int k = 0; k < d->nthreads; ++k) { d->SM[k].n = 0; d->FSM[k].n = 0; d->nsm[k] = 0; }
+    for (int k = 0; k < d->nthreads; ++k) { d->SM[k].n = 0; d->FSM[k].n = 0; d->NSM[k].n = 0; d->nsm[k] = 0; }
+    if (err) return err;
 
     double test;
     d->deltatmin = d->deltatint;
@@ -1363,6 +1629,18 @@ static int initial_checks(psph_odomain *d)
     for (size_t i = 0; i < d->n; i++) {
         psph_particle *p = &d->P[i]->p;
         if (p->Material < 3) p->Pressure = psph_o_eos(p->PresEq, p->Cs, p->P0, p->Density, p->RefDensity);
+        /* permeability of soil particles (Domain.cpp:1683-1716) */
+        if (p->Material == 3) {
+            double n0 = p->n0, dd = p->d;
+            switch (p->SeepageType) {
+            case 0: break;
+            case 1: p->k = n0*n0*n0*dd*dd/(180.0*(1.0-n0)*(1.0-n0)); break;
+            case 2: p->k = n0*n0*n0*dd*dd/(150.0*(1.0-n0)*(1.0-n0)); p->k2 = 1.75*(1.0-n0)/(n0*n0*n0*dd); break;
+            case 3: p->k = n0*n0*n0*dd*dd/(150.0*(1.0-n0)*(1.0-n0)); p->k2 = 0.4/(n0*n0*dd); break;
+            default: return set_err(PSPH_ERR_INVALID, "Seepage Type No is out of range");
+            }
+            p->n = p->n0;
+        }
     }
     return 0;
 }
@@ -1382,8 +1660,8 @@ int psph_o_create(const psph_config *cfg, const psph_particle *p, size_t n, doub
                   int nthreads, int track_scales, psph_odomain **out)
 {
     if (!cfg || !p || !out || n == 0) return set_err(PSPH_ERR_INVALID, "null argument or no particles");
-    if (cfg->in_out_flow || cfg->fsi)
-        return set_err(PSPH_ERR_UNSUPPORTED, "in-out-flow / FSI are outside this path");
+    if (cfg->in_out_flow)
+        return set_err(PSPH_ERR_UNSUPPORTED, "in-out-flow boundaries are outside this path");
     if (cfg->kernel_type < 0 || cfg->kernel_type > 2) return set_err(PSPH_ERR_INVALID, "Kernel Type No is out of range");
     if (cfg->visc_eq < 0 || cfg->visc_eq > 3) return set_err(PSPH_ERR_INVALID, "Viscosity Equation No is out of range");
     if (nthreads < 1) nthreads = 1;
@@ -1403,6 +1681,7 @@ int psph_o_create(const psph_config *cfg, const psph_particle *p, size_t n, doub
     }
     d->SM = (pairlist*)calloc(nthreads, sizeof(pairlist));
     d->FSM = (pairlist*)calloc(nthreads, sizeof(pairlist));
+    d->NSM = (pairlist*)calloc(nthreads, sizeof(pairlist));
     d->nsm = (int64_t*)calloc(nthreads, sizeof(int64_t));
     memcpy(d->TRPR, cfg->trpr, sizeof d->TRPR);
     memcpy(d->BLPF, cfg->blpf, sizeof d->BLPF);
@@ -1527,7 +1806,7 @@ int psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs)
                     k++;
                 }
             }
-        d->SM[t].n = 0; d->FSM[t].n = 0; d->nsm[t] = 0;
+        d->SM[t].n = 0; d->FSM[t].n = 0; d->NSM[t].n = 0; d->nsm[t] = 0;
     }
     *npairs = k;
     if (k > cap) return set_err(PSPH_ERR_CAPACITY, "need %zu pairs", k);
@@ -1537,7 +1816,7 @@ int psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs)
 int psph_o_scales(psph_odomain *d, double *out, size_t cap)
 {
     if (cap < d->n) return set_err(PSPH_ERR_CAPACITY, "need %zu particles", d->n);
-    for (size_t i = 0; i < d->n; ++i) memcpy(out + 24*i, d->P[i]->sc, 24*sizeof(double));
+    for (size_t i = 0; i < d->n; ++i) memcpy(out + 32*i, d->P[i]->sc, 32*sizeof(double));
     return 0;
 }
 
@@ -1545,8 +1824,8 @@ void psph_o_destroy(psph_odomain *d)
 {
     if (!d) return;
     for (size_t i = 0; i < d->n; ++i) { omp_destroy_lock(&d->P[i]->lock); free(d->P[i]); }
-    for (int k = 0; k < d->nthreads; ++k) { free(d->SM[k].v); free(d->FSM[k].v); }
-    free(d->SM); free(d->FSM); free(d->nsm);
+    for (int k = 0; k < d->nthreads; ++k) { free(d->SM[k].v); free(d->FSM[k].v); free(d->NSM[k].v); }
+    free(d->SM); free(d->FSM); free(d->NSM); free(d->nsm);
     free(d->P); free(d->fixed); free(d->HOC);
     free(d);
 }

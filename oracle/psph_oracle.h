@@ -57,9 +57,10 @@ int  psph_o_grid(psph_odomain *d, psph_grid *g);
 int  psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs);
 /* body-force hook: accel is added to every free particle's a after StartAcceleration */
 void psph_o_set_body_force(psph_odomain *d, const double accel[3]);
-/* Per-particle |contribution| sums of the last force pass, 24 doubles per particle:
- * a[3], dDensity, Pressure(wall), NSv[3], SumKernel, StrainRate[6], RotationRate[3],
- * VXSPH[3], ZWab, SumDen, (pad) -- the wall entries already divided by SumKernel. */
+/* Per-particle |contribution| sums of the last force pass, 32 doubles per particle:
+ * a[3], dDensity, Pressure(wall), NSv[3], SumKernel, StrainRate[6], RotationRate[3], (pad),
+ * VXSPH[3], ZWab, SumDen, FSIPressure, FSINSv[3], S, (pad x3) -- the wall and FSI
+ * entries already divided by their kernel sums. */
 int  psph_o_scales(psph_odomain *d, double *out, size_t cap);
 void psph_o_destroy(psph_odomain *d);
 const char *psph_o_last_error(void);

@@ -118,7 +118,7 @@ class Oracle:
         return out
 
     def scales(self, n):
-        out = np.zeros((n, 24))
+        out = np.zeros((n, 32))
         _check(lib().psph_o_scales(self._h, _ptr(out), n))
         return out
 

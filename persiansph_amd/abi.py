@@ -28,17 +28,21 @@ _PARTICLE_FIELDS = [
     ("Alpha", _D, 1), ("Beta", _D, 1), ("MuRef", _D, 1), ("T0", _D, 1), ("m", _D, 1), ("CSmag", _D, 1),
     ("TI", _D, 1), ("TIn", _D, 1), ("TIInitDist", _D, 1),
     ("G", _D, 1), ("K", _D, 1), ("Sigmay", _D, 1), ("c", _D, 1), ("phi", _D, 1), ("psi", _D, 1),
+    ("FSIPressure", _D, 1), ("FSISumKernel", _D, 1), ("FSINSv", _D, 3),
+    ("n", _D, 1), ("n0", _D, 1), ("k", _D, 1), ("k2", _D, 1), ("d", _D, 1), ("V", _D, 1),
+    ("RhoF", _D, 1), ("S", _D, 1),
     ("IsFree", _I, 1), ("NoSlip", _I, 1), ("Shepard", _I, 1), ("LES", _I, 1),
     ("Material", _I, 1), ("PresEq", _I, 1), ("VisM", _I, 1), ("Fail", _I, 1),
     ("ID", _I, 1), ("ct", _I, 1), ("FirstStep", _I, 1), ("InOut", _I, 1),
     ("ShepardCounter", _I, 1), ("ShepardStep", _I, 1),
+    ("SeepageType", _I, 1), ("IsSat", _I, 1), ("SatCheck", _I, 1), ("reserved2", _I, 1),
     ("Index", _I, 1), ("reserved", _I, 1),
 ]
 
 PARTICLE_DTYPE = np.dtype([(n, t, (k,)) if k > 1 else (n, t) for n, t, k in _PARTICLE_FIELDS],
                           align=True)
 PARTICLE_FIELDS = [(n, k) for n, _, k in _PARTICLE_FIELDS]
-assert PARTICLE_DTYPE.itemsize == 161 * 8 + 16 * 4, PARTICLE_DTYPE.itemsize
+assert PARTICLE_DTYPE.itemsize == 174 * 8 + 20 * 4, PARTICLE_DTYPE.itemsize
 
 FP64 = 0
 FP32_MIXED = 1
@@ -124,7 +128,8 @@ def default_config(dimension=2, kernel_type=0, visc_eq=0, gradient_type=0, schem
 
 
 def new_particles(n):
-    """n particles with the Particle constructor's defaults (Particle.cpp:25-117)."""
+    """n particles with the Particle constructor's defaults (Particle.cpp:25-117).  The
+    constructor also sets V = Mass/RefDensity from its arguments; set V with the masses."""
     p = np.zeros(n, dtype=PARTICLE_DTYPE)
     p["TIn"] = 4.0
     p["FPMassC"] = 1.0

@@ -128,6 +128,9 @@ def add_box_length(dim, V, Lx, Ly, Lz, r, density, h, ptype=1, rotation=0, fixed
     p["h"] = h
     p["IsFree"] = 0 if fixed else 1
     p["ID"] = 1
+    # V = Mass/RefDensity in the Particle constructor (Particle.cpp:88): 2-D boxes construct
+    # with their mass, 3-D boxes with mass 0 and set it afterwards (Domain.cpp:401-418)
+    p["V"] = p["Mass"]/density if dim == 2 else 0.0
     return p
 
 

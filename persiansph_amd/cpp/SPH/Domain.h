@@ -200,6 +200,12 @@ public:
     size_t Fail = 0;
     double c = 0.0, phi = 0.0, psi = 0.0;
     double h = 0.0;
+    // fluid-solid and soil-water interaction (Particle.h:84-101)
+    double FSIPressure = 0.0, FSISumKernel = 0.0;
+    Vec3_t FSINSv;
+    double n = 0.0, n0 = 0.0, k = 0.0, k2 = 0.0, d = 0.0, V = 0.0, RhoF = 0.0, S = 0.0;
+    size_t SeepageType = 0;
+    bool IsSat = false, SatCheck = false;
     int ct = 0;
     double SumKernel = 0.0;
     bool FirstStep = true;
@@ -209,6 +215,7 @@ public:
     {
         ID = Tag; x = x0; v = v0; Mass = Mass0; Density = Density0; RefDensity = Density0;
         h = h0; IsFree = !Fixed;
+        V = Mass/RefDensity;
     }
 };
 
@@ -640,8 +647,8 @@ private:
 #define PSPH_FIELDS_SCALAR(X) X(Density) X(Densitya) X(Densityb) X(dDensity) X(Pressure) X(Mass) \
     X(ZWab) X(SumDen) X(SumKernel) X(Mu) X(ShearRate) X(SBar) X(h) X(Cs) X(P0) X(RefDensity)           \
     X(FPMassC) X(Alpha) X(Beta) X(MuRef) X(T0) X(m) X(CSmag) X(TI) X(TIn) X(TIInitDist) X(G) X(K)     \
-    X(Sigmay) X(c) X(phi) X(psi)
-#define PSPH_FIELDS_VEC(X) X(x) X(v) X(va) X(vb) X(a) X(NSv) X(VXSPH)
+    X(Sigmay) X(c) X(phi) X(psi) X(FSIPressure) X(FSISumKernel) X(n) X(n0) X(k) X(k2) X(d) X(V) X(RhoF) X(S)
+#define PSPH_FIELDS_VEC(X) X(x) X(v) X(va) X(vb) X(a) X(NSv) X(VXSPH) X(FSINSv)
 #define PSPH_FIELDS_MAT(X) X(StrainRate) X(RotationRate) X(ShearStress) X(ShearStressa) X(ShearStressb) \
     X(Sigma) X(Sigmaa) X(Sigmab) X(Strain) X(Straina) X(Strainb) X(TIR)
 
@@ -659,6 +666,7 @@ private:
         q.Material = p.Material; q.PresEq = (int32_t)p.PresEq; q.VisM = (int32_t)p.VisM; q.Fail = (int32_t)p.Fail;
         q.ID = p.ID; q.ct = p.ct; q.FirstStep = p.FirstStep; q.InOut = (int32_t)p.InOut;
         q.ShepardCounter = (int32_t)p.ShepardCounter; q.ShepardStep = (int32_t)p.ShepardStep;
+        q.SeepageType = (int32_t)p.SeepageType; q.IsSat = p.IsSat; q.SatCheck = p.SatCheck;
     }
 
     static void from_abi(const psph_particle &q, Particle &p)
@@ -671,6 +679,7 @@ private:
 #undef V_
 #undef M_
         p.ct = q.ct; p.FirstStep = q.FirstStep != 0; p.ShepardCounter = (size_t)q.ShepardCounter;
+        p.IsSat = q.IsSat != 0; p.SatCheck = q.SatCheck != 0;
     }
 
     // Copies the device state back; particles deleted by the leave check are removed from
