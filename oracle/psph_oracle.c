@@ -800,6 +800,10 @@ static void list_generate(psph_odomain *d)
             int i = (int)(floor((x[c] - d->BLPF[c]) / d->CellSize[c]));
             if (i < 0) { if ((d->BLPF[c] - x[c]) <= d->hmax) i = 0; }
             if (i >= d->CellNo[c]) { if ((x[c] - d->TRPR[c]) <= d->hmax) i = d->CellNo[c]-1; }
+            /* a position that is not finite (a blown-up run) stays in the box: the reference
+             * indexes outside HOC here (undefined behaviour) */
+            if (!(i >= 0)) i = 0;
+            if (i >= d->CellNo[c]) i = d->CellNo[c]-1;
             ijk[c] = i;
         }
         int temp = HOCI(d, ijk[0], ijk[1], ijk[2]);

@@ -392,3 +392,118 @@ def couette_2d(dx=2.5e-5, v_lid=2.5e-5, rho=998.21, mu=1.002e-3, cs=0.08, nx=20,
     cfg.R = r
     dt = 0.2*h/cs
     return cfg, p, dt
+
+
+def _tank_2d(dx, TL, TH, layers=3):
+    """Cubic lattice covering a 2-D tank and `layers` wall layers left, right and below."""
+    lay = float(layers)
+    return add_box_length(2, (-lay*dx, -lay*dx, 0.0), 2*lay*dx + TL + dx/10.0, lay*dx + TH + dx/10.0,
+                          0.0, dx/2.0, 1000.0, 1.3*dx, 1, 0, False)
+
+
+def soil_water_2d(dx=0.01, swi_type=0, k=1e-6, rho_s=2000.0, rho_w=1000.0, rho_f=1000.0, scheme=0,
+                  seepage_type=0, fsi=False):
+    """A water column collapsing onto a soil bed in a 2-D tank (the setting of the reference's
+    Old_Tests/Dam_Scouring*.cpp): tank 0.6 x 0.4 m with 3 layers of fixed no-slip water walls
+    (Material 1), a soil bed 0.1 m deep on a fixed soil floor (Material 3, Drucker-Prager,
+    phi = 30 deg, c = 0, E = 5 MPa, nu = 0.3, TI 0.5), a water column 0.2 x 0.2 m on the bed at
+    the left, the bed under geostatic stress.  Soil-water interaction SWIType `swi_type` with
+    Darcy permeability k (or the seepage type with n0 = 0.4, d = 1 mm), RhoF = rho_f for the
+    saturation of soil under water.
+    Cubic spline, h = 1.3 dx, water Cs = 10 sqrt(2 g 0.3), soil Cs = sqrt(K/rho_s)."""
+    g = 9.81
+    TL, TH, Hs, Lw, Hw = 0.6, 0.4, 0.1, 0.2, 0.2
+    p = _tank_2d(dx, TL, TH)
+    x = p["x"]
+    wall = (x[:, 0] < 0.0) | (x[:, 0] > TL) | (x[:, 1] < 0.0)
+    soil = ~wall & (x[:, 1] <= Hs)
+    water = ~wall & ~soil & (x[:, 1] <= Hs + Hw) & (x[:, 0] <= Lw)
+    floor = (x[:, 1] < 0.0) & (x[:, 0] >= 0.0) & (x[:, 0] <= TL)
+    keep = wall | soil | water
+    p, wall, soil, water, floor = p[keep], wall[keep], soil[keep], water[keep], floor[keep]
+    E, nu = 5.0e6, 0.3
+    K = E/(3.0*(1.0 - 2.0*nu))
+    G = E/(2.0*(1.0 + nu))
+    cs_w = 10.0*math.sqrt(2.0*g*0.3)
+    cs_s = math.sqrt(K/rho_s)
+    sm = soil | floor
+    p["Material"] = np.where(sm, 3, 1)
+    p["IsFree"] = np.where(wall, 0, 1)
+    p["NoSlip"] = np.where(wall & ~floor, 1, 0)
+    p["ID"] = np.where(wall, 2, np.where(soil, 3, 1))
+    p["Density"] = np.where(sm, rho_s, rho_w)
+    p["RefDensity"] = p["Density"]
+    p["Mass"] = p["Density"]*dx*dx
+    p["V"] = p["Mass"]/p["RefDensity"]
+    p["Cs"] = np.where(sm, cs_s, cs_w)
+    p["PresEq"] = np.where(sm, 0, 1)
+    p["Mu"] = np.where(sm, 0.0, 1e-3)
+    p["MuRef"] = p["Mu"]
+    p["Alpha"] = np.where(sm, 0.1, 0.05)
+    p["Beta"] = np.where(sm, 0.1, 0.0)
+    p["G"] = np.where(sm, G, 0.0)
+    p["K"] = np.where(sm, K, 0.0)
+    p["phi"] = np.where(sm, math.radians(30.0), 0.0)
+    p["Fail"] = np.where(sm, 3, 0)
+    p["TI"] = np.where(soil, 0.5, 0.0)
+    p["TIn"] = np.where(soil, 2.55, 4.0)
+    p["TIInitDist"] = np.where(soil, dx, 0.0)
+    p["SeepageType"] = np.where(sm, seepage_type, 0)
+    p["k"] = np.where(sm, k, 0.0)
+    p["n0"] = np.where(sm, 0.4, 0.0)
+    p["d"] = np.where(sm, 1e-3, 0.0)
+    p["RhoF"] = np.where(sm, rho_f, 0.0)
+    # geostatic stress of the bed under its own weight and the water column's (away from the
+    # stress-free apex of the yield surface, where the flow direction is made of roundoff)
+    y, xx = p["x"][:, 1], p["x"][:, 0]
+    syy = np.where(soil, -rho_s*g*(Hs - y) - np.where(xx <= Lw, rho_w*g*Hw, 0.0), 0.0)
+    sig = np.zeros((len(p), 9))
+    sig[:, 4] = syy
+    sig[:, 0] = nu/(1.0 - nu)*syy
+    p["Sigma"] = sig
+    cfg = default_config(dimension=2, kernel_type=0, scheme=scheme, gravity=(0.0, -g, 0.0))
+    cfg.swi_type = swi_type
+    cfg.fsi = 1 if fsi else 0
+    dt = 0.2*1.3*dx/max(cs_w, cs_s)
+    return cfg, p, dt
+
+
+def fsi_2d(dx=0.01, scheme=0, E=1.0e6, rho_s=1200.0):
+    """Fluid-structure interaction in a 2-D tank (FSI on): a water column 0.2 x 0.3 m at the
+    left of a 0.6 x 0.4 m tank with fixed no-slip walls (Material 1), and a free elastic block
+    (Material 2, E, nu = 0.3, no failure) 0.06 x 0.15 m standing on the floor against the
+    column's right face.
+    Cubic spline, h = 1.3 dx, water Cs = 10 sqrt(2 g 0.3), solid Cs = sqrt(K/rho_s)."""
+    g = 9.81
+    TL, TH = 0.6, 0.4
+    p = _tank_2d(dx, TL, TH)
+    x = p["x"]
+    wall = (x[:, 0] < 0.0) | (x[:, 0] > TL) | (x[:, 1] < 0.0)
+    water = ~wall & (x[:, 0] <= 0.2) & (x[:, 1] <= 0.3)
+    solid = ~wall & (x[:, 0] > 0.2) & (x[:, 0] <= 0.26) & (x[:, 1] <= 0.15)
+    keep = wall | water | solid
+    p, wall, water, solid = p[keep], wall[keep], water[keep], solid[keep]
+    nu = 0.3
+    K = E/(3.0*(1.0 - 2.0*nu))
+    G = E/(2.0*(1.0 + nu))
+    cs_w = 10.0*math.sqrt(2.0*g*0.3)
+    cs_s = math.sqrt(K/rho_s)
+    p["Material"] = np.where(solid, 2, 1)
+    p["IsFree"] = np.where(wall, 0, 1)
+    p["NoSlip"] = np.where(wall, 1, 0)
+    p["ID"] = np.where(wall, 2, np.where(solid, 4, 1))
+    p["Density"] = np.where(solid, rho_s, 1000.0)
+    p["RefDensity"] = p["Density"]
+    p["Mass"] = p["Density"]*dx*dx
+    p["V"] = p["Mass"]/p["RefDensity"]
+    p["Cs"] = np.where(solid, cs_s, cs_w)
+    p["PresEq"] = np.where(solid, 0, 1)
+    p["Mu"] = np.where(solid, 0.0, 1e-3)
+    p["MuRef"] = p["Mu"]
+    p["Alpha"] = np.where(solid, 0.1, 0.05)
+    p["G"] = np.where(solid, G, 0.0)
+    p["K"] = np.where(solid, K, 0.0)
+    cfg = default_config(dimension=2, kernel_type=0, scheme=scheme, gravity=(0.0, -g, 0.0))
+    cfg.fsi = 1
+    dt = 0.2*1.3*dx/max(cs_w, cs_s)
+    return cfg, p, dt

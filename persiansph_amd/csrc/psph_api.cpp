@@ -287,6 +287,10 @@ static ClassParams class_of(const psph_particle &p)
     c.IsFree = p.IsFree ? 1 : 0; c.NoSlip = p.NoSlip ? 1 : 0; c.Shepard = p.Shepard ? 1 : 0;
     c.LES = p.LES ? 1 : 0; c.Material = p.Material; c.PresEq = p.PresEq; c.VisM = p.VisM;
     c.Fail = p.Fail; c.ID = p.ID; c.ShepardStep = p.ShepardStep; c.InOut = p.InOut;
+    c.V = p.V; c.RhoF = p.RhoF; c.k = p.k; c.k2 = p.k2; c.d = p.d; c.n0 = p.n0;
+    c.SeepageType = p.SeepageType;
+    // a saturated soil particle belongs to the class of its unsaturated RefDensity
+    if (p.Material == 3 && p.IsSat) c.RefDensity = p.RefDensity + p.RhoF;
     fill_derived(c);
     return c;
 }
@@ -425,6 +429,11 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.wal, sizeof(double4)*N);
     AL(a.acc, sizeof(double4)*N);
     AL(a.xsp, sizeof(double4)*N);
+    if (P.multi) {
+        AL(a.fsi, sizeof(double4)*N);
+        AL(a.fsk, sizeof(double)*N);
+        AL(a.xflag, sizeof(int32_t)*N);
+    }
     AL(a.shep, sizeof(double2)*N);
     if (P.any_solid || P.any_sr) AL(a.sr, sizeof(double)*6*N);
     if (P.any_solid) { AL(a.rr, sizeof(double)*3*N); AL(a.tir, sizeof(double)*6*N); }
@@ -509,6 +518,7 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
         mmu[i] = make_double2(q.Mass, q.Mu);
         flags[i] = (uint32_t)ci | (q.IsFree ? kFree : 0u) | (q.NoSlip ? kNoSlip : 0u) |
                    ((uint32_t)(q.Material & 3) << kMatShift);
+        if (q.Material == 3 && q.IsSat) flags[i] |= kSat;
         orig[i] = (int32_t)i;
         if (P.any_solid) {
             to_sym(q.Sigma, &sig[6*i]);
@@ -557,6 +567,13 @@ static int read_state(psph_handle *h)
     HIPCHECK(hipMemcpy(flags.data(), D.flags, sizeof(uint32_t)*n, hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(orig.data(), D.orig, sizeof(int32_t)*n, hipMemcpyDeviceToHost));
     bool have_transients = h->trans_ok;
+    std::vector<double4> fsi;
+    std::vector<double> fsk;
+    if (have_transients && P.multi) {
+        fsi.resize(n); fsk.resize(n);
+        HIPCHECK(hipMemcpy(fsi.data(), D.fsi, sizeof(double4)*n, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(fsk.data(), D.fsk, sizeof(double)*n, hipMemcpyDeviceToHost));
+    }
     if (have_transients) {
         HIPCHECK(hipMemcpy(wal.data(), D.wal, sizeof(double4)*n, hipMemcpyDeviceToHost));
         HIPCHECK(hipMemcpy(acc.data(), D.acc, sizeof(double4)*n, hipMemcpyDeviceToHost));
@@ -590,6 +607,16 @@ static int read_state(psph_handle *h)
         if (P.scheme == 1 && fr) { q.va[0] = vh[i].x; q.va[1] = vh[i].y; q.va[2] = vh[i].z; q.Densitya = vh[i].w; }
         else { q.vb[0] = vh[i].x; q.vb[1] = vh[i].y; q.vb[2] = vh[i].z; q.Densityb = vh[i].w; }
         q.Mass = mmu[i].x; q.Mu = mmu[i].y;
+        if (P.multi) {
+            const ClassParams &cl = h->classes[flag_cls(flags[i])];
+            q.IsSat = (flags[i] & kSat) ? 1 : 0;
+            q.SatCheck = (flags[i] & kSatCheck) ? 1 : 0;
+            if (q.Material == 3) q.RefDensity = q.IsSat ? cl.RefDensity - cl.RhoF : cl.RefDensity;
+            if (have_transients) {
+                q.FSIPressure = fsi[i].w; q.FSISumKernel = fsk[i];
+                q.FSINSv[0] = fsi[i].x; q.FSINSv[1] = fsi[i].y; q.FSINSv[2] = fsi[i].z;
+            }
+        }
         if (have_transients) {
             if (fr) {
                 q.a[0] = acc[i].x; q.a[1] = acc[i].y; q.a[2] = acc[i].z; q.dDensity = acc[i].w;
@@ -762,7 +789,6 @@ int psph_create(const psph_config *cfg, psph_handle **out)
     if (cfg->dimension == 2 && cfg->periodic[2])
         return fail(PSPH_ERR_UNSUPPORTED, "a periodic z direction in a 2-D run is outside this path");
     if (cfg->in_out_flow) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries are outside this path");
-    if (cfg->fsi) return fail(PSPH_ERR_UNSUPPORTED, "fluid-structure interaction is outside this path");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PSPH_ERR_NO_DEVICE, "no HIP device visible");
     psph_handle *h = new psph_handle();
@@ -788,8 +814,21 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     h->upload_index.resize(n);
     for (size_t i = 0; i < n; ++i) h->upload_index[i] = (int32_t)i;
     // InitialChecks, Domain.cpp:1662-1719: EOS pressure of fluid and solid particles
-    for (auto &q : h->host)
+    for (auto &q : h->host) {
         if (q.Material < 3) q.Pressure = host_eos(q.PresEq, q.Cs, q.P0, q.Density, q.RefDensity);
+        // permeability of soil particles (Domain.cpp:1683-1716)
+        if (q.Material == 3) {
+            double n0 = q.n0, dd = q.d;
+            switch (q.SeepageType) {
+            case 0: break;
+            case 1: q.k = n0*n0*n0*dd*dd/(180.0*(1.0-n0)*(1.0-n0)); break;
+            case 2: q.k = n0*n0*n0*dd*dd/(150.0*(1.0-n0)*(1.0-n0)); q.k2 = 1.75*(1.0-n0)/(n0*n0*n0*dd); break;
+            case 3: q.k = n0*n0*n0*dd*dd/(150.0*(1.0-n0)*(1.0-n0)); q.k2 = 0.4/(n0*n0*dd); break;
+            default: return fail(PSPH_ERR_INVALID, "Seepage Type No is out of range");
+            }
+            q.n = q.n0;
+        }
+    }
     int rc = cell_initiate(h, h->host.data(), n);
     if (rc) return rc;
     // TimestepCheck, Domain.cpp:1721-1733
@@ -806,12 +845,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     bool have_free = false;
     psph_particle ref{};
     int nfree = 0;
+    bool mats[4] = {false, false, false, false}, free_m[4] = {false, false, false, false};
+    psph_particle ref_m[4];
     for (size_t i = 0; i < n; ++i) {
         const psph_particle &q = h->host[i];
         if (q.Material < 1 || q.Material > 3)
             return fail(PSPH_ERR_INVALID, "Material type %d out of range (1 fluid, 2 solid, 3 soil)", q.Material);
-        if (q.Material != mat0)
-            return fail(PSPH_ERR_UNSUPPORTED, "particles of different materials (CalcForce12/13 pairs) are outside this path");
+        mats[q.Material] = true;
         if (q.InOut != 0) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow particles are outside this path");
         if (q.PresEq < 0 || q.PresEq > 2) return fail(PSPH_ERR_INVALID, "Pressure Equation No %d out of range", q.PresEq);
         if (q.Material == 3 && q.Fail > 1 && q.Fail != 2 && q.Fail != 3) return fail(PSPH_ERR_INVALID, "Failure Type %d out of range", q.Fail);
@@ -826,10 +866,14 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         }
         if (q.IsFree) {
             ++nfree;
+            const int m = q.Material;
+            const double r0 = (m == 3 && q.IsSat) ? q.RefDensity + q.RhoF : q.RefDensity;
+            if (!free_m[m]) { ref_m[m] = q; ref_m[m].RefDensity = r0; free_m[m] = true; }
+            else if (q.PresEq != ref_m[m].PresEq || q.Cs != ref_m[m].Cs || q.P0 != ref_m[m].P0 ||
+                     r0 != ref_m[m].RefDensity)
+                return fail(PSPH_ERR_UNSUPPORTED, "free particles of one material with different equations of state");
             if (!have_free) { ref = q; have_free = true; }
             else {
-                if (q.PresEq != ref.PresEq || q.Cs != ref.Cs || q.P0 != ref.P0 || q.RefDensity != ref.RefDensity)
-                    return fail(PSPH_ERR_UNSUPPORTED, "free particles with different equations of state");
                 if (q.FirstStep != ref.FirstStep || q.ct != ref.ct || q.ShepardCounter != ref.ShepardCounter ||
                     q.ShepardStep != ref.ShepardStep)
                     return fail(PSPH_ERR_UNSUPPORTED, "free particles with different integrator counters");
@@ -868,6 +912,24 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         P.eos_eq = ref.PresEq; P.eos_cs = ref.Cs; P.eos_p0 = ref.P0; P.eos_rho0 = ref.RefDensity;
         P.shepard_step = ref.ShepardStep;
     }
+    for (int m = 1; m <= 3; ++m)
+        if (free_m[m]) {
+            P.eos_eq_m[m] = ref_m[m].PresEq; P.eos_cs_m[m] = ref_m[m].Cs;
+            P.eos_p0_m[m] = ref_m[m].P0; P.eos_rho0_m[m] = ref_m[m].RefDensity;
+        }
+    // pairs of different materials: fluid with solid (CalcForce12, FSI smoothing) or fluid with
+    // soil (CalcForce13, saturation); solid with soil has no interaction in the reference
+    P.fsi = h->cfg.fsi ? 1 : 0;
+    P.swi_type = h->cfg.swi_type;
+    P.multi = ((int)mats[1] + (int)mats[2] + (int)mats[3]) > 1;
+    if (mats[2] && mats[3])
+        return fail(PSPH_ERR_UNSUPPORTED, "solid and soil particles in one run (no interaction type between them)");
+    if (P.multi && mats[3] && (P.swi_type < 0 || P.swi_type > 3))
+        return fail(PSPH_ERR_INVALID, "Soil-Water Interaction Type No %d is out of range", P.swi_type);
+    if (P.multi && mats[3] && P.swi_type == 1)
+        return fail(PSPH_ERR_UNSUPPORTED, "the surface-erosion soil-water interaction (SWIType 1) is outside this path");
+    P.any_soil_fluid = P.multi && mats[3];
+    P.any_solid_fluid = P.multi && mats[2];
     P.uniform_h = 1;
     P.h_uniform = h->classes[0].h;
     for (auto &c : h->classes) {
@@ -885,7 +947,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         double mu0 = have_free ? ref.Mu : 0.0;
         for (size_t i = 0; i < n; ++i)
             if (h->host[i].IsFree && h->host[i].Mu != mu0) { mu_uniform = false; break; }
-        bool ok = mat0 == 1 && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
+        bool ok = mat0 == 1 && !P.multi && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
                   !P.any_ti && !P.any_sr && !P.any_xsph && !P.any_shepard && P.uniform_h &&
                   mu_uniform && P.nclasses <= 8 && have_free && !P.any_periodic &&
                   !std::getenv("PSPH_GENERAL_KERNELS");
@@ -910,6 +972,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         }
     }
     P.soil32 = 0;
+    if (P.precision == PSPH_FP32_MIXED && P.multi)
+        return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tiers cover runs of one material");
     if (P.precision == PSPH_FP32_MIXED) {
         if (mat0 > 1) {
             P.soil32 = 1;            // solid/soil: FP32 pair sums, FP64 stress update
@@ -952,7 +1016,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     if (P.mixed32 && !P.nl_codes)
         return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier holds at most 2^27 particles");
     P.glist = 0;
-    if (!P.fast11 && !P.mixed32 && P.nfree > 0 && !std::getenv("PSPH_NO_GLIST")) {
+    if (!P.fast11 && !P.mixed32 && P.nfree > 0 && (!std::getenv("PSPH_NO_GLIST") || P.multi)) {
         // pair lists of the general kernels: radius Cellfac hmax (the largest pair support)
         int cap0 = nl_initial_cap(h, P.cellfac*P.hmax);
         if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);

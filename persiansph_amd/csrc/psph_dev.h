@@ -25,7 +25,8 @@ struct ClassParams {
     // derived constants (computed once on the host from the fields above)
     double dp_alpha2, dp_kf2, dp_alpha3, dp_kf3;   // Drucker-Prager alpha, kf (plane strain, 3-D)
     double sin_psi;                                // sin(psi) for the non-associated flow rule
-    int32_t IsFree, NoSlip, Shepard, LES, Material, PresEq, VisM, Fail, ID, ShepardStep, InOut, pad;
+    double V, RhoF, k, k2, d, n0;                  // soil-water interaction (Particle.h:84-93)
+    int32_t IsFree, NoSlip, Shepard, LES, Material, PresEq, VisM, Fail, ID, ShepardStep, InOut, SeepageType;
 };
 
 // Flags word per particle.
@@ -33,6 +34,8 @@ constexpr uint32_t kClsMask = 0xffffu;
 constexpr uint32_t kFree = 1u << 16;
 constexpr uint32_t kNoSlip = 1u << 17;
 constexpr int kMatShift = 18;      // 2 bits: Material 1..3
+constexpr uint32_t kSat = 1u << 20;       // IsSat: saturated soil (density and RefDensity less RhoF)
+constexpr uint32_t kSatCheck = 1u << 21;  // SatCheck of the last force pass
 
 __host__ __device__ inline int flag_cls(uint32_t f) { return (int)(f & kClsMask); }
 __host__ __device__ inline bool flag_free(uint32_t f) { return (f & kFree) != 0; }
@@ -81,6 +84,13 @@ struct Params {
     double gravity[3];
     double eos_cs, eos_p0, eos_rho0;   // the free particles' EOS (uniform, checked at upload)
     int eos_eq;
+    // per material (index 1..3): the EOS of its free particles, which a wall of that material
+    // takes for its density (uniform per material, checked at upload)
+    int eos_eq_m[4];
+    double eos_cs_m[4], eos_p0_m[4], eos_rho0_m[4];
+    // pairs of different materials (fluid with solid, or fluid with soil): CalcForce12/13 in
+    // k_cross, the FSI smoothing (fsi) and the saturation of soil (swi_type != 2)
+    int multi, fsi, swi_type, any_soil_fluid, any_solid_fluid;
     int nc[3], ncells;
     int n, nfree, nfixed, nclasses;
     int dim, kernel_type, visc_eq, gradient_type, scheme, precision;
@@ -177,6 +187,9 @@ struct Dev {
     double4 *wal;      // fixed: NSv x,y,z, wall density d_w
     double4 *acc;      // a x,y,z, dDensity
     double4 *xsp;      // VXSPH x,y,z, ZWab
+    double4 *fsi;      // FSINSv x,y,z, FSIPressure (free solid particles next to fluid; zero otherwise)
+    double *fsk;       // FSISumKernel
+    int32_t *xflag;    // free target has pairs with particles of another material (per target)
     double2 *shep;     // SumDen, SumKernel
     double *sr;        // StrainRate (6)
     double *rr;        // RotationRate (3: xy xz yz)

@@ -563,8 +563,8 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     D.vel_b[i] = vi;
     D.shep[i] = make_double2(0.0, SK);
     // wall density and sound speed as the free partner sees them (CalcForce11 :41-61, :69-70)
-    double dw = density_solid(P.eos_eq, P.eos_cs, P.eos_p0, Pw, P.eos_rho0);
-    double Cw = sound_speed(P.eos_eq, P.eos_cs, dw, P.eos_rho0);
+    double dw = density_solid(P.eos_eq_m[mat], P.eos_cs_m[mat], P.eos_p0_m[mat], Pw, P.eos_rho0_m[mat]);
+    double Cw = sound_speed(P.eos_eq_m[mat], P.eos_cs_m[mat], dw, P.eos_rho0_m[mat]);
     double2 mm = D.mmu_b[i];
     D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
     D.wal[i] = make_double4(n0, n1, n2, dw);
@@ -1113,7 +1113,7 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
     int kx, ky, kz;
     decode_key(P, key_i, kx, ky, kz);
     ListWriter out(reinterpret_cast<int4 *>(D.nl) + t, (size_t)P.nfree, P.nl_cap);
-    int xmat = 0;
+    int xmat = 0, xc = 0;
     const int nrows = (P.dim == 3) ? 9 : 3;
     for (int rs = 0; rs < 2*nrows; ++rs) {
         int beg = 0, end = 0;
@@ -1128,7 +1128,12 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
         for (int j = beg; j < end; ++j) {
             if (j == i) continue;
             uint32_t fj = D.flags_b[j];
-            if (flag_mat(fj) != mat) { xmat = 1; continue; }
+            if (flag_mat(fj) != mat) {
+                // pairs of different materials go to k_cross; none between solid and soil
+                const int prod = mat*flag_mat(fj);
+                if (P.multi && (prod == 2 || prod == 3)) xc = 1; else xmat = 1;
+                continue;
+            }
             double4 pj = D.pos_b[j];
             double h = P.uniform_h ? P.h_uniform : (hcls[ci] + hcls[flag_cls(fj)])/2;
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
@@ -1139,6 +1144,7 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
     }
     out.flush();
     D.ncnt[t] = out.cnt;
+    if (P.multi) D.xflag[t] = xc;
     if (out.cnt > P.nl_cap) atomicAdd(&D.ctl->nl_over, 1);
     int anyx = wave_any(xmat != 0);
     if ((threadIdx.x & 63) == 0 && anyx) atomicOr(&D.ctl->xmat, 1);
@@ -1179,6 +1185,19 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     if (SOLID) {
         for (int k = 0; k < 6; ++k) { sgi[k] = D.sig_b[6*i+k]; tri[k] = D.tir[6*i+k]; }
     }
+    // a pair of the target's own material: CalcForce11 for fluid targets, CalcForce2233 for
+    // solid and soil ones (LastComputeAcceleration, Domain.cpp:1244-1254); runs with fluid and
+    // soil or solid together take the SOLID kernel
+    auto pair_same = [&](int jj) {
+        if constexpr (SOLID) {
+            if constexpr (sizeof(T) == 8) {
+                if (mat == 1) { pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on); return; }
+            }
+            pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+        } else {
+            pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+        }
+    };
     unsigned long long nff = 0, nfw = 0;
     int xmat = 0;
     int cnt = 0;
@@ -1192,8 +1211,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
         for (int e = 0; e < m; ++e) {
             int jj = lst_g[((size_t)(e >> 2)*P.nfree + t)*4 + (e & 3)];
             if (flag_free(D.flags_b[jj])) ++nff; else ++nfw;
-            if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
-            else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+            pair_same(jj);
         }
         skip = m;
         walk = total > P.nl_cap;
@@ -1217,7 +1235,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
             if (j != i) {
                 uint32_t fj = D.flags_b[j];
                 if (flag_mat(fj) != mat) {
-                    xmat = 1;
+                    if (!P.multi) xmat = 1;           // with several materials: k_cross
                 } else {
                     double4 pj = D.pos_b[j];
                     double h = pair_h(cp, P, ci, flag_cls(fj));
@@ -1234,8 +1252,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
             if (wave_any(cnt == kCap)) {
                 for (int e = 0; e < cnt; ++e) {
                     int jj = lst[e*kForceBlock + tid];
-                    if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
-                    else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+                    pair_same(jj);
                 }
                 cnt = 0;
             }
@@ -1243,8 +1260,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     }
     for (int e = 0; e < cnt; ++e) {
         int jj = lst[e*kForceBlock + tid];
-        if constexpr (SOLID) pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
-        else pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
+        pair_same(jj);
     }
     if (act) {
         double a0 = A.a0, a1 = A.a1, a2 = A.a2, dd = A.dd;
@@ -1257,11 +1273,14 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
         if (shep_on) D.shep[i] = make_double2(A.sd, 0.0);
         if (SOLID || P.any_sr) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
         if (SOLID) for (int k = 0; k < 3; ++k) D.rr[3*i+k] = A.rr[k];
-        // dt control: min over free particles of sqrt_h_a*sqrt(h/|a|) (Domain.cpp:1281-1296)
-        double na = sqrt(a0*a0 + a1*a1 + a2*a2);
-        double test = sqrt(cp[ci].h/na);
-        double cand = P.sqrt_h_a*test;
-        dtb = dbits(cand);
+        // dt control: min over free particles of sqrt_h_a*sqrt(h/|a|) (Domain.cpp:1281-1296);
+        // targets with pairs of another material take theirs after k_cross
+        if (!(P.multi && D.xflag[t])) {
+            double na = sqrt(a0*a0 + a1*a1 + a2*a2);
+            double test = sqrt(cp[ci].h/na);
+            double cand = P.sqrt_h_a*test;
+            dtb = dbits(cand);
+        }
     }
     // pair statistics and the cross-material check
     int anyx = wave_any(xmat != 0);
@@ -1269,6 +1288,258 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     force_partials(D, dtb, nff, nfw);
 }
 
+
+// ------------------------------------------------------------------------------------------
+// pairs of different materials (LastComputeAcceleration's NSM loop, Domain.cpp:1256-1271)
+// ------------------------------------------------------------------------------------------
+
+// The 3^DIM cells around a particle's cell (periodic directions wrap), as the reference's
+// half-stencil search pairs them: f(cell index, its y index) for every cell in the grid.
+template <class F>
+__device__ inline void for_cells(const Params &P, int kx, int ky, int kz, F &&f)
+{
+    const int ncell = (P.dim == 3) ? 27 : 9;
+    for (int cc = 0; cc < ncell; ++cc) {
+        int xx = kx + (cc % 3) - 1, yy = ky + ((cc / 3) % 3) - 1, zz = (P.dim == 3) ? kz + (cc / 9) - 1 : kz;
+        if (P.periodic[0]) xx = (xx + P.nc[0]) % P.nc[0];
+        if (P.periodic[1]) yy = (yy + P.nc[1]) % P.nc[1];
+        if (P.periodic[2]) zz = (zz + P.nc[2]) % P.nc[2];
+        if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+        f(xx + P.nc[0]*(yy + P.nc[1]*zz), yy);
+    }
+}
+
+// FSI smoothing (PrimaryComputeAcceleration, Domain.cpp:1089-1160) onto free solid particles
+// from their free fluid neighbours (same or adjacent cells, no distance test: K vanishes beyond
+// the support).  As in the reference, the FSI velocity takes only the pairs whose P1 is the
+// solid particle (the others go to its NSv, which nothing reads), and the pressure and velocity
+// are divided by the kernel sum once per pair.  Every other particle gets zeros.
+__global__ void __launch_bounds__(kBlock) k_fsi(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t fi = D.flags_b[i];
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, fp = 0.0, sk = 0.0;
+    if (P.fsi && flag_free(fi) && flag_mat(fi) == 2) {
+        const double4 pi = D.pos_b[i];
+        const int key_i = D.key[i], orig_i = D.orig_b[i];
+        const double hi = D.cls[flag_cls(fi)].h;
+        const double *G = P.gravity;
+        int kx, ky, kz, m = 0;
+        decode_key(P, key_i, kx, ky, kz);
+        for_cells(P, kx, ky, kz, [&](int cell, int) {
+            for (int j = D.cstart[cell]; j < D.cstart[cell + 1]; ++j) {
+                const uint32_t fj = D.flags_b[j];
+                if (!flag_free(fj) || flag_mat(fj) != 1) continue;
+                const double4 pj = D.pos_b[j];
+                const double h = (hi + D.cls[flag_cls(fj)].h)/2.0;
+                double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
+                const int first = pair_first(P, key_i, D.key[j], orig_i, D.orig_b[j]);
+                const double sgn = first ? 1.0 : -1.0;          // x_P1 - x_P2, exact negation
+                x0 *= sgn; x1 *= sgn; x2 *= sgn;
+                const double K = kernel_w<double>(P.dim, P.kernel_type, sqrt(x0*x0 + x1*x1 + x2*x2)/h, h);
+                const double4 vj = D.vel_b[j];
+                ++m;
+                sk += K;
+                if (first) { n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K; }
+                fp += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
+            }
+        });
+        for (int q = 0; q < m; ++q) { fp = fp/sk; n0 = n0/sk; n1 = n1/sk; n2 = n2/sk; }
+    }
+    D.fsi[i] = make_double4(n0, n1, n2, fp);
+    D.fsk[i] = sk;
+}
+
+// Saturation (Domain.cpp:1060-1088, 1209-1233): a soil particle with a fluid particle in its
+// own or a neighbouring cell, in a cell row not below its own and not lower than itself, one
+// of the two free, is saturated for this step: its density, Densityb and RefDensity drop by
+// RhoF and its mass becomes V (RefDensity - RhoF); it returns when no such neighbour remains.
+__global__ void __launch_bounds__(kBlock) k_sat(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t fi = D.flags_b[i];
+    if (flag_mat(fi) != 3) return;
+    const double4 pi = D.pos_b[i];
+    int kx, ky, kz;
+    decode_key(P, D.key[i], kx, ky, kz);
+    bool check = false;
+    for_cells(P, kx, ky, kz, [&](int cell, int yy) {
+        if (check || yy < ky) return;
+        for (int j = D.cstart[cell]; j < D.cstart[cell + 1]; ++j) {
+            const uint32_t fj = D.flags_b[j];
+            if (flag_mat(fj) != 1 || !(flag_free(fi) || flag_free(fj))) continue;
+            if (D.pos_b[j].y >= pi.y) { check = true; return; }
+        }
+    });
+    const ClassParams &c = D.cls[flag_cls(fi)];
+    const bool sat = (fi & kSat) != 0;
+    uint32_t f = (fi & ~kSatCheck) | (check ? kSatCheck : 0u);
+    if (check != sat) {
+        double4 p = D.pos_b[i];
+        double2 mm = D.mmu_b[i];
+        double4 vh = D.vh_b[i];
+        const double r0 = c.RefDensity - c.RhoF;              // the saturated RefDensity
+        if (check) { mm.x = c.V*(c.RefDensity - c.RhoF); p.w = p.w - c.RhoF; vh.w = vh.w - c.RhoF; f |= kSat; }
+        else { mm.x = c.V*(r0 + c.RhoF); p.w = p.w + c.RhoF; vh.w = vh.w + c.RhoF; f &= ~kSat; }
+        D.pos_b[i] = p;
+        D.mmu_b[i] = mm;
+        if (P.scheme == 0 || !flag_free(fi)) D.vh_b[i] = vh;   // Densityb (dead under Leapfrog)
+        if (flag_free(fi)) { double4 a = D.aux[i]; a.x = mm.x; D.aux[i] = a; }
+    }
+    D.flags_b[i] = f;
+}
+
+struct XAcc { double a0, a1, a2, dd; double sr[6]; };
+
+// CalcForce12 (Interaction.cpp:379-506) contribution to free particle i as P1 from j, one of
+// the two fluid and the other solid.  The fluid particle's EOS turns the solid particle's FSI
+// pressure into its density; both masses are the fluid particle's; no artificial viscosity
+// (the reference zeroes Alpha and Beta before testing them).
+__device__ inline void pair12(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+                              uint32_t fi, uint32_t fj, double4 pi, double4 pj, double x0, double x1,
+                              double x2, double rij, double h, XAcc &A)
+{
+    const ClassParams &Ci = cp[flag_cls(fi)], &Cj = cp[flag_cls(fj)];
+    const double4 vi = D.vel_b[i], vj = D.vel_b[j];
+    const bool f1 = flag_mat(fi) == 1;
+    const double GK = grad_kernel<double>(P.dim, P.kernel_type, rij/h, h);
+    double di, dj, mi, mj, Pi_, Pj_, Mu, b0, b1, b2;
+    if (f1) {
+        const double4 fs = D.fsi[j];
+        di = pi.w; mi = D.mmu_b[i].x;
+        dj = density_solid(Ci.PresEq, Ci.Cs, Ci.P0, fs.w, Ci.RefDensity);
+        mj = mi;
+        Pi_ = vi.w; Pj_ = fs.w; Mu = D.mmu_b[i].y;
+        b0 = vi.x - (2.0*vj.x-fs.x); b1 = vi.y - (2.0*vj.y-fs.y); b2 = vi.z - (2.0*vj.z-fs.z);
+    } else {
+        const double4 fs = D.fsi[i];
+        di = density_solid(Cj.PresEq, Cj.Cs, Cj.P0, fs.w, Cj.RefDensity);
+        mi = D.mmu_b[j].x;
+        dj = pj.w;
+        mj = mi;
+        Pi_ = fs.w; Pj_ = vj.w; Mu = D.mmu_b[j].y;
+        b0 = (2.0*vi.x-fs.x) - vj.x; b1 = (2.0*vi.y-fs.y) - vj.y; b2 = (2.0*vi.z-fs.z) - vj.z;
+    }
+    const double u0 = vi.x - vj.x, u1 = vi.y - vj.y, u2 = vi.z - vj.z;
+    double VI0 = 0.0, VI1 = 0.0, VI2 = 0.0;
+    viscous_force<double>(P.visc_eq, VI0, VI1, VI2, Mu, di, dj, GK, b0, b1, b2, P.dim,
+                          P.kernel_type, rij, h, x0, x1, x2, u0, u1, u2);
+    const double PIij = 0.0;
+    double sf;
+    if (P.gradient_type == 0) sf = -1.0*( Pi_/(di*di) + Pj_/(dj*dj) + PIij ) * GK;
+    else sf = -1.0*( (Pi_ + Pj_)/(di*dj) + PIij ) * GK;
+    double t0 = sf*x0 + VI0, t1 = sf*x1 + VI1, t2 = sf*x2 + VI2;
+    if (P.dim == 2) t2 = 0.0;
+    A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
+    if (f1) {
+        const double temp1 = u0*(GK*x0) + u1*(GK*x1) + u2*(GK*x2);
+        A.dd += mj * (di/dj) * temp1;
+        if ((Ci.T0 > 0.0 || Cj.T0 > 0.0 || Ci.LES || Cj.LES) && (Ci.T0 > 0.0 || Ci.LES)) {
+            const double g = -GK, f = mj/dj;
+            A.sr[0] += ((2.0*b0*x0)*g)*f; A.sr[1] += ((b0*x1+b1*x0)*g)*f; A.sr[2] += ((b0*x2+b2*x0)*g)*f;
+            A.sr[3] += ((2.0*b1*x1)*g)*f; A.sr[4] += ((b1*x2+b2*x1)*g)*f; A.sr[5] += ((2.0*b2*x2)*g)*f;
+        }
+    }
+}
+
+// Seepage (Functions.cpp:266-298): Darcy (0, 1) or Forchheimer (2, 3) coefficients
+__device__ inline void seepage(int st, double k, double k2, double mu, double rho, double &SF1, double &SF2)
+{
+    SF1 = mu/k;
+    SF2 = (st >= 2) ? k2*rho : 0.0;
+}
+
+// CalcForce13 (Interaction.cpp:508-653) contribution to free particle i as P1 from j, one of
+// the two soil and the other water: the seepage force (SWIType 0 and 2), with SWIType 2 the
+// water pressure on the soil particle, nothing with SWIType 3.  K is over both densities.
+__device__ inline void pair13(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+                              uint32_t fi, uint32_t fj, double4 pi, double4 pj, double x0, double x1,
+                              double x2, double rij, double h, XAcc &A)
+{
+    if (P.swi_type == 3) return;
+    const bool s1 = flag_mat(fi) == 3;                 // i (P1) is the soil particle
+    const int so = s1 ? i : j, w = s1 ? j : i;
+    const ClassParams &Cs = cp[flag_cls(s1 ? fi : fj)], &Cw = cp[flag_cls(s1 ? fj : fi)];
+    const double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h)/(pi.w*pj.w);
+    const double4 vw = D.vel_b[w], vs = D.vel_b[so];
+    const double v0 = vw.x - vs.x, v1 = vw.y - vs.y, v2 = vw.z - vs.z;
+    const double nv = sqrt(v0*v0 + v1*v1 + v2*v2);
+    double SF1, SF2;
+    seepage(Cs.SeepageType, Cs.k, Cs.k2, Cw.MuRef, Cw.RefDensity, SF1, SF2);
+    double s0 = (SF1*v0 + SF2*nv*v0) *K, s1v = (SF1*v1 + SF2*nv*v1) *K, s2 = (SF1*v2 + SF2*nv*v2) *K;
+    if (P.dim == 2) s2 = 0.0;
+    if (s1) {
+        const double mw = D.mmu_b[w].x;
+        double t0 = mw*s0, t1 = mw*s1v, t2 = mw*s2;
+        if (P.swi_type == 2) {
+            const double GK = grad_kernel<double>(P.dim, P.kernel_type, rij/h, h)/(pi.w*pj.w);
+            const double pw = vw.w;
+            t0 = t0 - mw*pw*GK*x0; t1 = t1 - mw*pw*GK*x1; t2 = t2 - mw*pw*GK*x2;
+        }
+        A.a0 += t0; A.a1 += t1; A.a2 += t2;
+    } else {
+        const double ms = D.mmu_b[so].x;
+        A.a0 -= ms*s0; A.a1 -= ms*s1v; A.a2 -= ms*s2;
+    }
+}
+
+// The pairs of each free target with particles of another material, after every other pair
+// (the reference's NSM loop runs after SM and FSM): CalcForce12 (fluid with solid, mean h) and
+// CalcForce13 (fluid with soil, the smaller h).  Adds to a, dDensity and the fluid's strain
+// rate, then takes the target's dt candidate.  Targets without such neighbours return at once.
+__global__ void __launch_bounds__(kForceBlock) k_cross(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kForceBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    const int t = xcd_block()*kForceBlock + threadIdx.x;
+    unsigned long long dtb = ~0ull, nff = 0, nfw = 0;
+    if (t < P.nfree && D.xflag[t]) {
+        const int i = D.free_list[t];
+        const uint32_t fi = D.flags_b[i];
+        const int ci = flag_cls(fi), mat = flag_mat(fi);
+        const double4 pi = D.pos_b[i];
+        const int key_i = D.key[i];
+        int kx, ky, kz;
+        decode_key(P, key_i, kx, ky, kz);
+        const double4 a = D.acc[i];
+        XAcc A;
+        A.a0 = a.x; A.a1 = a.y; A.a2 = a.z; A.dd = a.w;
+        const bool sr_on = P.any_sr && (cp[ci].T0 > 0.0 || cp[ci].LES);
+        for (int k = 0; k < 6; ++k) A.sr[k] = sr_on ? D.sr[6*i+k] : 0.0;
+        for_cells(P, kx, ky, kz, [&](int cell, int) {
+            for (int j = D.cstart[cell]; j < D.cstart[cell + 1]; ++j) {
+                const uint32_t fj = D.flags_b[j];
+                const int prod = mat*flag_mat(fj);
+                if (flag_mat(fj) == mat || (prod != 2 && prod != 3)) continue;
+                const double4 pj = D.pos_b[j];
+                const double hi = cp[ci].h, hj = cp[flag_cls(fj)].h;
+                const double h = (prod == 3) ? fmin(hi, hj) : (hi + hj)/2;
+                double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
+                const double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+                if (!((rij/h) <= P.cellfac)) continue;
+                if (flag_free(fj)) ++nff; else ++nfw;
+                if (prod == 2) pair12(D, P, cp, i, j, fi, fj, pi, pj, x0, x1, x2, rij, h, A);
+                else pair13(D, P, cp, i, j, fi, fj, pi, pj, x0, x1, x2, rij, h, A);
+            }
+        });
+        D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        if (sr_on) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
+        const double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+        dtb = dbits(P.sqrt_h_a*sqrt(cp[ci].h/na));
+    }
+    force_partials(D, dtb, nff, nfw);
+}
 
 // ------------------------------------------------------------------------------------------
 // fast fluid force pass: CalcForce11 for cubic spline, Morris viscosity, squared-density
@@ -2258,7 +2529,21 @@ void launch_cellsort(const Dev &D, const Params &P, hipStream_t s)
 void launch_reorder(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_reorder, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P); }
 
+static void launch_wall_pass(const Dev &D, const Params &P, hipStream_t s);
+
+// The wall pass, then with several materials the FSI smoothing and the saturation of soil,
+// in the order of PrimaryComputeAcceleration (Domain.cpp:1020-1234).
 void launch_wall(const Dev &D, const Params &P, hipStream_t s)
+{
+    launch_wall_pass(D, P, s);
+    if (P.multi) {
+        hipLaunchKernelGGL(k_fsi, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+        if (P.any_soil_fluid && P.swi_type != 2)
+            hipLaunchKernelGGL(k_sat, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+    }
+}
+
+static void launch_wall_pass(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfixed <= 0) return;
     if (P.fast11 || P.mixed32) {
@@ -2284,6 +2569,8 @@ void launch_force(const Dev &D, const Params &P, hipStream_t s)
 {
     if (P.nfree <= 0) return;
     launch_force_pass(D, P, s);
+    if (P.multi)
+        hipLaunchKernelGGL(k_cross, dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
 }
 
 // The list passes of the fast fluid path: build (every step, or with list reuse on the steps

@@ -1,0 +1,88 @@
+"""GPU parity for pairs of different materials: CalcForce12 with the FSI smoothing and
+CalcForce13 with saturation (Interaction.cpp:379-653, Domain.cpp:1060-1234).
+
+Cases (persiansph_amd.cases): a water column collapsing onto a saturating soil bed with the
+seepage force (SWIType 0) or the seepage force and pore pressure (SWIType 2), under modified
+Verlet and Leapfrog; a water column against a free elastic block with FSI on.  Against the
+oracle on the same inputs:
+  * the first force pass: a and dDensity of the free particles to 1e-12 of (|a| + the sum of
+    |pair contributions|), the FSI pressure and velocity of the solid particles to 1e-12 of
+    their kernel-weighted sums, the saturation state and the saturated masses and densities;
+  * trajectories: positions within 1e-11 m, densities within 1e-11 relative, over 20 steps
+    (soil-water; the bed yields after that and runs 1 ulp apart part) or the one step the
+    reference's FSI division quirk leaves finite (FSI).
+"""
+import numpy as np
+import pytest
+
+from persiansph_amd import Solver, cases
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(name, g, o, scale, rel=1e-12, floor=0.0):
+    g = np.asarray(g, dtype=np.float64)
+    o = np.asarray(o, dtype=np.float64)
+    bad = np.abs(g - o) > rel*(np.abs(o) + scale) + floor
+    if bad.any():
+        k = tuple(np.argwhere(bad)[0])
+        raise AssertionError(f"{name}: {bad.sum()} entries beyond {rel}; first {k} gpu={g[k]!r} oracle={o[k]!r}")
+
+
+CASES = {
+    "seepage_mverlet": lambda: cases.soil_water_2d(swi_type=0, scheme=0),
+    "seepage_leapfrog": lambda: cases.soil_water_2d(swi_type=0, scheme=1),
+    "pore_pressure": lambda: cases.soil_water_2d(swi_type=2, scheme=0),
+    "forchheimer": lambda: cases.soil_water_2d(swi_type=0, scheme=0, seepage_type=2),
+    "fsi": lambda: cases.fsi_2d(scheme=0),
+    "fsi_leapfrog": lambda: cases.fsi_2d(scheme=1),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_multimaterial_parity(case):
+    cfg, p, dt = CASES[case]()
+    print(f"[multi] {case} n={len(p)}", flush=True)
+    g = Solver(cfg, p, dt)
+    o = Oracle(cfg, p, dt, track_scales=True)
+    sg, so = g.run(1, tout=0.0), o.run(1, tout=0.0)
+    assert sg.pairs == so.pairs
+    G, O = g.particles(), o.particles()
+    sc = o.scales(len(O))
+    fr = O["IsFree"] == 1
+    # a floor of 1e-13 of the largest |pair contribution| sum: the FSI pressures the reference
+    # divides once per pair (below) leave terms near 1e-150 whose sums cancel
+    floor = 1e-13*sc[fr, 0:3].max()
+    _check("a", G["a"][fr], O["a"][fr], sc[fr, 0:3], floor=floor)
+    _check("dDensity", G["dDensity"][fr], O["dDensity"][fr], sc[fr, 3])
+    _check("wall Pressure", G["Pressure"][~fr], O["Pressure"][~fr], sc[~fr, 4])
+    # a solid particle whose FSI neighbours all lie beyond the support divides 0 by 0, as the
+    # reference does (nothing reads it: such a particle has no CalcForce12 pair)
+    np.testing.assert_array_equal(np.isnan(G["FSIPressure"]), np.isnan(O["FSIPressure"]))
+    _check("FSIPressure", G["FSIPressure"], O["FSIPressure"], sc[:, 24])
+    _check("FSINSv", G["FSINSv"], O["FSINSv"], sc[:, 25:28])
+    np.testing.assert_array_equal(G["IsSat"], O["IsSat"])
+    np.testing.assert_array_equal(G["SatCheck"], O["SatCheck"])
+    np.testing.assert_array_equal(G["Mass"], O["Mass"])
+    np.testing.assert_array_equal(G["RefDensity"], O["RefDensity"])
+    np.testing.assert_allclose(G["Density"], O["Density"], rtol=1e-14, atol=0)
+    if "fsi" in case:
+        assert np.nanmax(np.abs(O["FSIPressure"])) > 0.0
+    elif case != "pore_pressure":
+        assert O["IsSat"].sum() > 0
+    # The reference divides a solid particle's FSI pressure and velocity by the kernel sum
+    # once per FSI pair (FreeFSIParticles holds a particle once per pair, Domain.cpp:1117,
+    # 1131, 1150-1155), so they vanish or overflow; in the FSI case a few particles leave the
+    # finite range within the first step, on both paths alike, and the run goes on from there
+    # as the reference's would.  Its trajectory is compared over that one step.
+    # The soil bed yields between steps 20 and 30: two oracle runs 1 ulp apart part there by
+    # ~1e-9 m (tests/test_oracle_multimaterial.py), so the soil-water window is 20 steps.
+    steps = 1 if "fsi" in case else 20
+    sg, so = g.run(steps), o.run(steps)
+    assert sg.steps == so.steps == steps
+    G, O = g.particles(), o.particles()
+    np.testing.assert_allclose(G["x"], O["x"], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(G["Density"], O["Density"], rtol=1e-11, atol=0)
+    np.testing.assert_array_equal(G["IsSat"], O["IsSat"])
+    g.close()
