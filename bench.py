@@ -155,6 +155,8 @@ def main():
     pair_rate = st.pairs*args.steps/el
     # one visualisation output of the current state: the fields WriteXDMF writes, packed to FP32
     # on the GPU and copied to pinned host memory, against the full FP64 state download
+    s.output_begin()                 # the first output allocates its device and pinned buffers
+    s.output_end(copy=False)
     t0 = time.perf_counter()
     s.output_begin()
     s.output_end(copy=False)

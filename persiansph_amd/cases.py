@@ -402,14 +402,16 @@ def _tank_2d(dx, TL, TH, layers=3):
 
 
 def soil_water_2d(dx=0.01, swi_type=0, k=1e-6, rho_s=2000.0, rho_w=1000.0, rho_f=1000.0, scheme=0,
-                  seepage_type=0, fsi=False):
+                  seepage_type=0, fsi=False, grains=0, stir=0.0, seed=3):
     """A water column collapsing onto a soil bed in a 2-D tank (the setting of the reference's
     Old_Tests/Dam_Scouring*.cpp): tank 0.6 x 0.4 m with 3 layers of fixed no-slip water walls
     (Material 1), a soil bed 0.1 m deep on a fixed soil floor (Material 3, Drucker-Prager,
     phi = 30 deg, c = 0, E = 5 MPa, nu = 0.3, TI 0.5), a water column 0.2 x 0.2 m on the bed at
     the left, the bed under geostatic stress.  Soil-water interaction SWIType `swi_type` with
     Darcy permeability k (or the seepage type with n0 = 0.4, d = 1 mm), RhoF = rho_f for the
-    saturation of soil under water.
+    saturation of soil under water.  grains > 0 turns every grains-th water particle into a
+    free soil grain suspended in the column (few soil neighbours: the surface-erosion branch of
+    SWIType 1); stir > 0 gives the water seeded random velocities up to stir m/s.
     Cubic spline, h = 1.3 dx, water Cs = 10 sqrt(2 g 0.3), soil Cs = sqrt(K/rho_s)."""
     g = 9.81
     TL, TH, Hs, Lw, Hw = 0.6, 0.4, 0.1, 0.2, 0.2
@@ -421,6 +423,11 @@ def soil_water_2d(dx=0.01, swi_type=0, k=1e-6, rho_s=2000.0, rho_w=1000.0, rho_f
     floor = (x[:, 1] < 0.0) & (x[:, 0] >= 0.0) & (x[:, 0] <= TL)
     keep = wall | soil | water
     p, wall, soil, water, floor = p[keep], wall[keep], soil[keep], water[keep], floor[keep]
+    if grains > 0:
+        wi = np.where(water)[0]
+        g_idx = wi[::grains]
+        soil[g_idx] = True
+        water[g_idx] = False
     E, nu = 5.0e6, 0.3
     K = E/(3.0*(1.0 - 2.0*nu))
     G = E/(2.0*(1.0 + nu))
@@ -461,6 +468,11 @@ def soil_water_2d(dx=0.01, swi_type=0, k=1e-6, rho_s=2000.0, rho_w=1000.0, rho_f
     sig[:, 4] = syy
     sig[:, 0] = nu/(1.0 - nu)*syy
     p["Sigma"] = sig
+    if stir > 0.0:
+        rng = np.random.default_rng(seed)
+        v = rng.uniform(-stir, stir, size=(int(water.sum()), 3))
+        v[:, 2] = 0.0
+        p["v"][water] = v
     cfg = default_config(dimension=2, kernel_type=0, scheme=scheme, gravity=(0.0, -g, 0.0))
     cfg.swi_type = swi_type
     cfg.fsi = 1 if fsi else 0

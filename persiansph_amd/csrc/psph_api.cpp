@@ -433,6 +433,7 @@ static int allocate(psph_handle *h, size_t n)
         AL(a.fsi, sizeof(double4)*N);
         AL(a.fsk, sizeof(double)*N);
         AL(a.xflag, sizeof(int32_t)*N);
+        AL(a.swz, sizeof(double2)*N);
     }
     AL(a.shep, sizeof(double2)*N);
     if (P.any_solid || P.any_sr) AL(a.sr, sizeof(double)*6*N);
@@ -926,8 +927,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         return fail(PSPH_ERR_UNSUPPORTED, "solid and soil particles in one run (no interaction type between them)");
     if (P.multi && mats[3] && (P.swi_type < 0 || P.swi_type > 3))
         return fail(PSPH_ERR_INVALID, "Soil-Water Interaction Type No %d is out of range", P.swi_type);
-    if (P.multi && mats[3] && P.swi_type == 1)
-        return fail(PSPH_ERR_UNSUPPORTED, "the surface-erosion soil-water interaction (SWIType 1) is outside this path");
+    P.swi1 = (P.multi && mats[3] && P.swi_type == 1) ? 1 : 0;
     P.any_soil_fluid = P.multi && mats[3];
     P.any_solid_fluid = P.multi && mats[2];
     P.uniform_h = 1;

@@ -91,6 +91,7 @@ struct Params {
     // pairs of different materials (fluid with solid, or fluid with soil): CalcForce12/13 in
     // k_cross, the FSI smoothing (fsi) and the saturation of soil (swi_type != 2)
     int multi, fsi, swi_type, any_soil_fluid, any_solid_fluid;
+    int swi1;                          // SWIType 1 with soil and fluid: S and ZWab go to swz
     int nc[3], ncells;
     int n, nfree, nfixed, nclasses;
     int dim, kernel_type, visc_eq, gradient_type, scheme, precision;
@@ -189,6 +190,7 @@ struct Dev {
     double4 *xsp;      // VXSPH x,y,z, ZWab
     double4 *fsi;      // FSINSv x,y,z, FSIPressure (free solid particles next to fluid; zero otherwise)
     double *fsk;       // FSISumKernel
+    double2 *swz;      // SWIType 1: S and ZWab of the last force pass (fixed: 0 and 0/1)
     int32_t *xflag;    // free target has pairs with particles of another material (per target)
     double2 *shep;     // SumDen, SumKernel
     double *sr;        // StrainRate (6)

@@ -496,6 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     decode_key(P, key_i, kx, ky, kz);
     double SK = 0.0, Pw = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0;
     double sg[6] = {0, 0, 0, 0, 0, 0};
+    int npair = 0;
     const double *G = P.gravity;
     const double sup = P.cellfac*P.hmax;
     const double sup2m = sup*sup*(1.0 + 1e-9);
@@ -525,6 +526,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
             if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
             double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
             if (!((rij/h) <= P.cellfac)) continue;      // outside the reference's pair set
+            ++npair;
             // orientation of the reference's pair (P1 = smaller key / larger index); xij is
             // x_P1 - x_P2.  Multiplying by -1 is an exact negation (rij unchanged).  The sign is
             // formed without short-circuit branches: hipcc (ROCm 7.2, -O3) dropped the same-cell
@@ -562,6 +564,8 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     vi.w = Pw;
     D.vel_b[i] = vi;
     D.shep[i] = make_double2(0.0, SK);
+    // a fixed particle in a pair gets ZWab = 1 and keeps S = 0 (Interaction.cpp:158, 350)
+    if (P.swi1) D.swz[i] = make_double2(0.0, npair > 0 ? 1.0 : 0.0);
     // wall density and sound speed as the free partner sees them (CalcForce11 :41-61, :69-70)
     double dw = density_solid(P.eos_eq_m[mat], P.eos_cs_m[mat], P.eos_p0_m[mat], Pw, P.eos_rho0_m[mat]);
     double Cw = sound_speed(P.eos_eq_m[mat], P.eos_cs_m[mat], dw, P.eos_rho0_m[mat]);
@@ -848,6 +852,7 @@ template <class T>
 struct AccT {
     T a0, a1, a2, dd, ddc;
     T vx0, vx1, vx2, zw, sd;
+    T sw;                 // S of the surface-erosion soil-water interaction (SWIType 1)
     T sr[6];
     T rr[3];
 };
@@ -948,6 +953,7 @@ __device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *
         A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
         A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
     }
+    if (P.swi1) A.sw = A.sw + mj/dj*b0*x1*-GK;         // Interaction.cpp:154
     if (shep_on) {
         A.zw += mj/dj* K;
         A.sd += mj*K;
@@ -1061,6 +1067,7 @@ __device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams
     }
     T fq = mj/dj;
     A.zw += mj/dj* K;
+    if (P.swi1) A.sw = A.sw + mj/dj*b0*x1*-GK;         // Interaction.cpp:347
     A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
     A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
     A.rr[0] = A.rr[0] + r01*fq; A.rr[1] = A.rr[1] + r02*fq; A.rr[2] = A.rr[2] + r12*fq;
@@ -1178,7 +1185,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     AccT<T> A;
     A.a0 = F32 ? (T)0 : (T)P.afree[0]; A.a1 = F32 ? (T)0 : (T)P.afree[1]; A.a2 = F32 ? (T)0 : (T)P.afree[2];
     A.dd = 0.0; A.ddc = 0.0;
-    A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0;
+    A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0; A.sw = 0.0;
     for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
     A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
     double sgi[6] = {0, 0, 0, 0, 0, 0}, tri[6] = {0, 0, 0, 0, 0, 0};
@@ -1270,6 +1277,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
         }
         D.acc[i] = make_double4(a0, a1, a2, dd);
         if (P.any_xsph || zw_on) D.xsp[i] = make_double4(A.vx0, A.vx1, A.vx2, A.zw);
+        if (P.swi1) D.swz[i] = make_double2((double)A.sw, (double)A.zw);
         if (shep_on) D.shep[i] = make_double2(A.sd, 0.0);
         if (SOLID || P.any_sr) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
         if (SOLID) for (int k = 0; k < 3; ++k) D.rr[3*i+k] = A.rr[k];
@@ -1471,9 +1479,21 @@ __device__ inline void pair13(const Dev &D, const Params &P, const ClassParams *
     const double4 vw = D.vel_b[w], vs = D.vel_b[so];
     const double v0 = vw.x - vs.x, v1 = vw.y - vs.y, v2 = vw.z - vs.z;
     const double nv = sqrt(v0*v0 + v1*v1 + v2*v2);
-    double SF1, SF2;
-    seepage(Cs.SeepageType, Cs.k, Cs.k2, Cw.MuRef, Cw.RefDensity, SF1, SF2);
-    double s0 = (SF1*v0 + SF2*nv*v0) *K, s1v = (SF1*v1 + SF2*nv*v1) *K, s2 = (SF1*v2 + SF2*nv*v2) *K;
+    double s0, s1v, s2;
+    const double2 zs = P.swi1 ? D.swz[so] : make_double2(0.0, 1.0);   // soil's S, ZWab
+    if (P.swi1 && zs.y < 0.25) {
+        // surface erosion: drag and lift on a soil particle with few soil neighbours
+        const double s_w = D.swz[w].x;                 // the water particle's S
+        const double Cd = 24.0*(Cw.MuRef/Cw.RefDensity)/(Cs.d*nv+0.01*h*h) + 2.0;
+        s0 = (3.0/(4.0*Cs.d)*Cw.RefDensity*(1.0-Cs.n0)*Cd*nv*v0) *K;
+        s1v = (3.0/(4.0*Cs.d)*Cw.RefDensity*(1.0-Cs.n0)*Cd*nv*v1) *K;
+        s2 = (3.0/(4.0*Cs.d)*Cw.RefDensity*(1.0-Cs.n0)*Cd*nv*v2) *K;
+        s1v += (Cw.RefDensity*(1.0-Cs.n0)*nv*fabs(s_w-zs.x)) *K;
+    } else {
+        double SF1, SF2;
+        seepage(Cs.SeepageType, Cs.k, Cs.k2, Cw.MuRef, Cw.RefDensity, SF1, SF2);
+        s0 = (SF1*v0 + SF2*nv*v0) *K; s1v = (SF1*v1 + SF2*nv*v1) *K; s2 = (SF1*v2 + SF2*nv*v2) *K;
+    }
     if (P.dim == 2) s2 = 0.0;
     if (s1) {
         const double mw = D.mmu_b[w].x;

@@ -2,15 +2,17 @@
 CalcForce13 with saturation (Interaction.cpp:379-653, Domain.cpp:1060-1234).
 
 Cases (persiansph_amd.cases): a water column collapsing onto a saturating soil bed with the
-seepage force (SWIType 0) or the seepage force and pore pressure (SWIType 2), under modified
-Verlet and Leapfrog; a water column against a free elastic block with FSI on.  Against the
+seepage force (SWIType 0), with surface erosion (SWIType 1: drag and lift on soil particles
+with few soil neighbours) or with the seepage force and pore pressure (SWIType 2), under
+modified Verlet and Leapfrog; a water column against a free elastic block with FSI on.  Against the
 oracle on the same inputs:
   * the first force pass: a and dDensity of the free particles to 1e-12 of (|a| + the sum of
     |pair contributions|), the FSI pressure and velocity of the solid particles to 1e-12 of
     their kernel-weighted sums, the saturation state and the saturated masses and densities;
   * trajectories: positions within 1e-11 m, densities within 1e-11 relative, over 20 steps
-    (soil-water; the bed yields after that and runs 1 ulp apart part) or the one step the
-    reference's FSI division quirk leaves finite (FSI).
+    (soil-water; the bed yields after that and runs 1 ulp apart part), 4 steps (the stirred
+    erosion case, whose stress-free grains part after that) or the one step the reference's
+    FSI division quirk leaves finite (FSI).
 """
 import numpy as np
 import pytest
@@ -35,6 +37,7 @@ CASES = {
     "seepage_leapfrog": lambda: cases.soil_water_2d(swi_type=0, scheme=1),
     "pore_pressure": lambda: cases.soil_water_2d(swi_type=2, scheme=0),
     "forchheimer": lambda: cases.soil_water_2d(swi_type=0, scheme=0, seepage_type=2),
+    "erosion": lambda: cases.soil_water_2d(swi_type=1, scheme=0, grains=7, stir=0.2),
     "fsi": lambda: cases.fsi_2d(scheme=0),
     "fsi_leapfrog": lambda: cases.fsi_2d(scheme=1),
 }
@@ -71,6 +74,9 @@ def test_multimaterial_parity(case):
         assert np.nanmax(np.abs(O["FSIPressure"])) > 0.0
     elif case != "pore_pressure":
         assert O["IsSat"].sum() > 0
+    if case == "erosion":
+        s = (O["Material"] == 3) & fr
+        assert (O["ZWab"][s] < 0.25).any() and np.abs(O["S"]).max() > 0.0   # the erosion branch runs
     # The reference divides a solid particle's FSI pressure and velocity by the kernel sum
     # once per FSI pair (FreeFSIParticles holds a particle once per pair, Domain.cpp:1117,
     # 1131, 1150-1155), so they vanish or overflow; in the FSI case a few particles leave the
@@ -78,7 +84,8 @@ def test_multimaterial_parity(case):
     # as the reference's would.  Its trajectory is compared over that one step.
     # The soil bed yields between steps 20 and 30: two oracle runs 1 ulp apart part there by
     # ~1e-9 m (tests/test_oracle_multimaterial.py), so the soil-water window is 20 steps.
-    steps = 1 if "fsi" in case else 20
+    # The stirred erosion case's stress-free grains part after 4 steps in the same way.
+    steps = 1 if "fsi" in case else (4 if case == "erosion" else 20)
     sg, so = g.run(steps), o.run(steps)
     assert sg.steps == so.steps == steps
     G, O = g.particles(), o.particles()

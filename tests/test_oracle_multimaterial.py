@@ -255,3 +255,20 @@ def test_soil_water_window_of_the_gpu_comparison():
     a.run(10)
     b.run(10)
     assert np.abs(a.particles()["x"] - b.particles()["x"]).max() > 1e-12
+
+
+def test_erosion_window_of_the_gpu_comparison():
+    """The stirred erosion case (suspended stress-free grains, SWIType 1) is compared over 4
+    steps: runs 1 ulp apart agree to 1e-15 m over them and part after."""
+    from persiansph_amd import cases
+    cfg, p, dt = cases.soil_water_2d(swi_type=1, scheme=0, grains=7, stir=0.2)
+    q = p.copy()
+    fl = np.where(q["IsFree"] == 1)[0]
+    q["x"][fl, 0] = np.nextafter(q["x"][fl, 0], np.inf)
+    a, b = Oracle(cfg, p, dt), Oracle(cfg, q, dt)
+    a.run(4)
+    b.run(4)
+    assert np.abs(a.particles()["x"] - b.particles()["x"]).max() <= 1e-15
+    a.run(6)
+    b.run(6)
+    assert np.abs(a.particles()["x"] - b.particles()["x"]).max() > 1e-12
