@@ -53,7 +53,7 @@ typedef struct psph_config {
     int32_t scheme;          /* Domain::Scheme: 0 modified Verlet, 1 Leapfrog */
     int32_t precision;       /* PSPH_FP64 or PSPH_FP32_MIXED */
     int32_t periodic[3];     /* Boundary::Periodic (z only in 3-D; needs 3+ cells across a period) */
-    int32_t in_out_flow;     /* Boundary::InOutFlow (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    int32_t in_out_flow;     /* Boundary::InOutFlow: 1 inflow, 2 outflow, 3 both (x direction) */
     int32_t swi_type;        /* Domain::SWIType (only the single-material meaning is used) */
     int32_t fsi;             /* Domain::FSI (nonzero -> PSPH_ERR_UNSUPPORTED) */
     double  cellfac;         /* Domain::Cellfac (2 for kernels 0/1, 3 for kernel 2) */
@@ -67,6 +67,14 @@ typedef struct psph_config {
     double  R;               /* Domain::R (periodic margin only) */
     double  mu_max;          /* Domain::MuMax on entry (the reference never initialises it; pass 0) */
     double  cs_max;          /* Domain::CsMax on entry (likewise; pass 0) */
+    /* in/out-flow boundaries (Boundary_Condition.cpp:25-42; Domain.cpp:1327-1635), with the
+     * default InCon/OutCon (constant velocity and density) */
+    double  in_v[3], out_v[3];            /* Boundary::inv, Boundary::outv */
+    double  in_density, out_density;      /* Boundary::inDensity, Boundary::outDensity */
+    double  bc_cellfac;                   /* Boundary::cellfac (default 3.9): zone width / hmax */
+    double  initial_dist;                 /* Domain::InitialDist: recycling offset */
+    int32_t mass_conservation;            /* Boundary::MassConservation */
+    int32_t reserved_io;
 } psph_config;
 
 /* One particle: the fields of SPH::Particle (Source/Particle.h:29-149) that the path reads

@@ -713,6 +713,11 @@ struct psph_odomain {
     int64_t last_pairs;
     double DomSize[3];          /* periodic length per direction, 0 where not periodic */
     double body[3];             /* body-force hook: added to every free particle's a */
+    /* in/out-flow boundaries: Boundary::InPart, InFlowLoc1-3, OutFlowLoc, inoutcounter */
+    int *inpart;
+    size_t ninpart, capinpart;
+    double loc1, loc2, loc3, outloc;
+    int inoutcounter;
 };
 
 #define HOCI(d,i,j,k) ((d)->HOC[((size_t)(i)*(d)->CellNo[1] + (size_t)(j))*(d)->CellNo[2] + (size_t)(k)])
@@ -1507,6 +1512,194 @@ static int calc_force13(psph_odomain *d, OP *o1, OP *o2, int64_t *cnt)
     return 0;
 }
 
+
+/* ------------------------------------------------------------------------------------------
+ * in/out-flow boundaries (Domain.cpp:1327-1635) with the default InCon/OutCon (Domain.cpp:
+ * 35-45: the constant velocity and density of the boundary)
+ * ---------------------------------------------------------------------------------------- */
+static void inpart_push(psph_odomain *d, int a)
+{
+    if (d->ninpart == d->capinpart) {
+        d->capinpart = d->capinpart ? 2*d->capinpart : 256;
+        d->inpart = (int*)realloc(d->inpart, d->capinpart*sizeof(int));
+    }
+    d->inpart[d->ninpart++] = a;
+}
+
+/* the cells x-indices [q1a, q1b) of every y, z row in the reference's loop order (q2 outer,
+ * q3 middle, q1 inner), each chain from its head (the largest particle index first) */
+#define IO_SCAN(d, q1a, q1b, BODY)                                                            \
+    for (int q2 = 0; (d)->cfg.periodic[1] ? (q2 < (d)->CellNo[1]-2) : (q2 < (d)->CellNo[1]); q2++) \
+    for (int q3 = 0; (d)->cfg.periodic[2] ? (q3 < (d)->CellNo[2]-2) : (q3 < (d)->CellNo[2]); q3++) \
+    for (int q1 = (q1a); q1 < (q1b) && q1 < (d)->CellNo[0]; q1++)                                \
+        for (int t = HOCI(d, q1, q2, q3); t != -1; t = (d)->P[t]->LL) { BODY }
+
+/* InFlowBCFresh, Domain.cpp:1498-1635 */
+static int inflow_fresh(psph_odomain *d)
+{
+    const psph_config *c = &d->cfg;
+    const int io = c->in_out_flow;
+    if (d->inoutcounter == 0) {
+        if (io == 1 || io == 3) {
+            if (!(norm3(c->in_v) > 0.0) || !(c->in_density > 0.0))
+                return set_err(PSPH_ERR_INVALID, "BC.inv or BC.inDensity are not defined");
+            d->ninpart = 0;
+            d->loc1 = d->BLPF[0] + c->bc_cellfac*d->hmax;
+            int t1 = (int)(floor((d->loc1 - d->BLPF[0]) / d->CellSize[0]));
+            IO_SCAN(d, 0, t1 + 1, {
+                psph_particle *q = &d->P[t]->p;
+                if (q->IsFree && (q->x[0] <= d->loc1)) { inpart_push(d, t); q->InOut = 1; }
+            })
+            if (d->ninpart == 0) return set_err(PSPH_ERR_INVALID, "no free particle in the inflow zone");
+            d->loc2 = d->loc3 = d->P[d->inpart[0]]->p.x[0];
+            for (size_t i = 0; i < d->ninpart; i++) {
+                double x0 = d->P[d->inpart[i]]->p.x[0];
+                if (x0 < d->loc2) d->loc2 = x0;
+                if (x0 > d->loc3) d->loc3 = x0;
+            }
+        }
+        if (io == 2 || io == 3) d->outloc = d->TRPR[0] - c->bc_cellfac*d->hmax;
+        d->inoutcounter = 2;
+    }
+    if (d->inoutcounter == 1) {
+        d->ninpart = 0;
+        int t1 = (int)(floor((d->loc1 - d->BLPF[0]) / d->CellSize[0]));
+        IO_SCAN(d, 0, t1 + 1, {
+            psph_particle *q = &d->P[t]->p;
+            if (q->IsFree && (q->x[0] <= d->loc1) && q->InOut == 1) inpart_push(d, t);
+        })
+        d->inoutcounter = 2;
+    }
+    int *outp = NULL;
+    size_t nout = 0, capout = 0;
+    if (io == 2 || io == 3) {
+        int t1 = (int)(floor((d->outloc - d->BLPF[0]) / d->CellSize[0]));
+        IO_SCAN(d, t1, d->CellNo[0], {
+            psph_particle *q = &d->P[t]->p;
+            if (q->IsFree && (q->x[0] >= d->outloc)) {
+                if (nout == capout) { capout = capout ? 2*capout : 256; outp = (int*)realloc(outp, capout*sizeof(int)); }
+                outp[nout++] = t;
+                q->InOut = 2;
+            }
+        })
+    }
+    for (size_t i = 0; i < d->ninpart; i++) {
+        psph_particle *q = &d->P[d->inpart[i]]->p;
+        for (int k = 0; k < 3; ++k) { q->v[k] = c->in_v[k]; q->vb[k] = c->in_v[k]; q->va[k] = c->in_v[k]; }
+        q->Density = c->in_density; q->Densityb = c->in_density; q->Densitya = c->in_density;
+        q->Pressure = psph_o_eos(q->PresEq, q->Cs, q->P0, q->Density, q->RefDensity);
+    }
+    double temp11 = c->mass_conservation ? d->ninpart*1.0/(nout*1.0) : 1.0;
+    for (size_t i = 0; i < nout; i++) {
+        psph_particle *q = &d->P[outp[i]]->p;
+        if (norm3(c->out_v) > 0.0 || temp11 != 1.0)
+            for (int k = 0; k < 3; ++k) { double w = temp11*c->out_v[k]; q->v[k] = w; q->vb[k] = w; q->va[k] = w; }
+        if (c->out_density > 0.0) {
+            q->Density = c->out_density; q->Densityb = c->out_density; q->Densitya = c->out_density;
+            q->Pressure = psph_o_eos(q->PresEq, q->Cs, q->P0, q->Density, q->RefDensity);
+        }
+    }
+    free(outp);
+    return 0;
+}
+
+/* the fields an inflow particle passes to the one recycled or created from it (Domain.cpp:
+ * 1366-1401) */
+static void inflow_copy(psph_particle *a, const psph_particle *b, const double *x)
+{
+    for (int k = 0; k < 3; ++k) a->x[k] = x[k];
+    a->Material = 1; a->InOut = 1; a->FirstStep = 0;
+    a->P0 = b->P0; a->PresEq = b->PresEq; a->Cs = b->Cs;
+    a->Alpha = b->Alpha; a->Beta = b->Beta; a->Mu = b->Mu; a->MuRef = b->MuRef; a->T0 = 0.0;
+    a->Mass = b->Mass; a->h = b->h; a->ID = b->ID; a->TI = 0.0; a->RefDensity = b->RefDensity;
+    a->ct = b->ct; a->Shepard = b->Shepard; a->ShepardStep = b->ShepardStep;
+    a->ShepardCounter = b->ShepardCounter; a->LES = b->LES; a->CSmag = b->CSmag;
+}
+
+/* InFlowBCLeave, Domain.cpp:1327-1496: particles that left [BLPF, TRPR] are recycled as the new
+ * inflow particles that the inflow particles past InFlowLoc1 leave behind; extra ones are
+ * created, missing ones deleted (the inflow list is then rebuilt from the cells) */
+static int inflow_leave(psph_odomain *d, int64_t *ndel)
+{
+    const psph_config *c = &d->cfg;
+    size_t n = d->n, ndp = 0, nadd = 0;
+    int *delp = (int*)malloc((n + 1)*sizeof(int));
+    int *addb = (int*)malloc((d->ninpart + 1)*sizeof(int));
+    double *addx = (double*)malloc(3*(d->ninpart + 1)*sizeof(double));
+    for (size_t i = 0; i < n; i++) {
+        psph_particle *q = &d->P[i]->p;
+        if ((q->x[0] > d->TRPR[0]) || (q->x[1] > d->TRPR[1]) || (q->x[2] > d->TRPR[2]) ||
+            (q->x[0] < d->BLPF[0]) || (q->x[1] < d->BLPF[1]) || (q->x[2] < d->BLPF[2])) {
+            q->InOut = 0;
+            delp[ndp++] = (int)i;
+        }
+    }
+    if (c->in_out_flow == 1 || c->in_out_flow == 3) {
+        size_t k = 0;
+        for (size_t i = 0; i < d->ninpart; i++) {
+            psph_particle *q = &d->P[d->inpart[i]]->p;
+            if (q->x[0] > d->loc1) {
+                double *t = addx + 3*nadd;
+                for (int m = 0; m < 3; ++m) t[m] = q->x[m];
+                t[0] -= (d->loc3 - d->loc2 + c->initial_dist);
+                q->InOut = 0; q->FirstStep = 0; q->ShepardCounter = 0;
+                addb[nadd++] = d->inpart[i];
+            } else {
+                d->inpart[k++] = d->inpart[i];      /* InPart.DelItems, order kept */
+            }
+        }
+        d->ninpart = k;
+    }
+    if (nadd >= ndp) {
+        for (size_t i = 0; i < ndp; i++) {
+            inflow_copy(&d->P[delp[i]]->p, &d->P[addb[i]]->p, addx + 3*i);
+            inpart_push(d, delp[i]);
+        }
+        if (nadd > ndp) {
+            d->P = (OP**)realloc(d->P, (n + nadd - ndp)*sizeof(OP*));
+            d->fixed = (size_t*)realloc(d->fixed, (n + nadd - ndp)*sizeof(size_t));
+            for (size_t i = ndp; i < nadd; i++) {
+                const psph_particle *b = &d->P[addb[i]]->p;
+                OP *o = (OP*)calloc(1, sizeof(OP));
+                psph_particle *a = &o->p;
+                /* Particle::Particle (Particle.cpp:25-117) with the inflow particle's ID, v, Mass,
+                 * RefDensity and h */
+                for (int m = 0; m < 3; ++m) a->v[m] = b->v[m];
+                a->TIn = 4.0; a->Density = b->RefDensity; a->RefDensity = b->RefDensity; a->Mass = b->Mass;
+                a->FPMassC = 1.0; a->IsFree = 1; a->h = b->h; a->ID = b->ID; a->m = 300.0;
+                a->FirstStep = 1; a->V = a->Mass/a->RefDensity; a->ShepardStep = 40; a->CSmag = 0.17;
+                inflow_copy(a, b, addx + 3*i);
+                o->LL = -1;
+                omp_init_lock(&o->lock);
+                d->P[d->n] = o;
+                inpart_push(d, (int)d->n);
+                d->n++;
+            }
+        }
+    } else {
+        for (size_t i = 0; i < nadd; i++) {
+            inflow_copy(&d->P[delp[i]]->p, &d->P[addb[i]]->p, addx + 3*i);
+            inpart_push(d, delp[i]);
+        }
+        /* Particles.DelItems of the rest, order kept (External/array.h:726-738) */
+        size_t k = 0, j = nadd;
+        for (size_t i = 0; i < d->n; i++) {
+            if (j < ndp && (int)i == delp[j]) {
+                omp_destroy_lock(&d->P[i]->lock);
+                free(d->P[i]);
+                j++;
+                (*ndel)++;
+            } else {
+                d->P[k++] = d->P[i];
+            }
+        }
+        d->n = k;
+        d->inoutcounter = 1;
+    }
+    free(delp); free(addb); free(addx);
+    return 0;
+}
+
 /* LastComputeAcceleration, Domain.cpp:1238-1297 */
 static int last_compute_acceleration(psph_odomain *d)
 {
@@ -1587,6 +1780,11 @@ static int domain_move(psph_odomain *d, double dt)
     for (size_t i = 0; i < d->n; i++) {
         psph_particle *p = &d->P[i]->p;
         if (p->IsFree) {
+            /* in/out-flow particles move with the imposed velocity (Domain.cpp:1305-1320) */
+            if (p->InOut > 0) {
+                memset(p->a, 0, sizeof p->a);
+                if (p->InOut == 1 || d->cfg.out_density > 0.0) { p->dDensity = 0.0; p->ZWab = 0.0; }
+            }
             int rc = psph_o_particle_move(p, dt, d->cfg.scheme, d->I22);
             if (rc) {
                 #pragma omp critical
@@ -1664,8 +1862,10 @@ int psph_o_create(const psph_config *cfg, const psph_particle *p, size_t n, doub
                   int nthreads, int track_scales, psph_odomain **out)
 {
     if (!cfg || !p || !out || n == 0) return set_err(PSPH_ERR_INVALID, "null argument or no particles");
-    if (cfg->in_out_flow)
-        return set_err(PSPH_ERR_UNSUPPORTED, "in-out-flow boundaries are outside this path");
+    if (cfg->in_out_flow < 0 || cfg->in_out_flow > 3)
+        return set_err(PSPH_ERR_INVALID, "InOutFlow %d out of range", cfg->in_out_flow);
+    if (cfg->in_out_flow > 0 && cfg->periodic[0])
+        return set_err(PSPH_ERR_INVALID, "Periodic BC in the X direction cannot be used with In/Out-Flow BC simultaneously");
     if (cfg->kernel_type < 0 || cfg->kernel_type > 2) return set_err(PSPH_ERR_INVALID, "Kernel Type No is out of range");
     if (cfg->visc_eq < 0 || cfg->visc_eq > 3) return set_err(PSPH_ERR_INVALID, "Viscosity Equation No is out of range");
     if (nthreads < 1) nthreads = 1;
@@ -1719,6 +1919,10 @@ void psph_o_set_body_force(psph_odomain *d, const double accel[3])
 static int force_pass(psph_odomain *d)
 {
     start_acceleration(d);
+    if (d->cfg.in_out_flow) {
+        int rc = inflow_fresh(d);
+        if (rc) return rc;
+    }
     main_neighbour_search(d);
     body_force_hook(d);
     primary_compute_acceleration(d);
@@ -1732,7 +1936,12 @@ static int finish_step(psph_odomain *d, psph_status *st)
     rc = domain_move(d, d->deltat);
     if (rc) return rc;
     d->Time += d->deltat;
-    st->deleted += check_particle_leave(d);
+    if (d->cfg.in_out_flow) {
+        rc = inflow_leave(d, &st->deleted);
+        if (rc) return rc;
+    } else {
+        st->deleted += check_particle_leave(d);
+    }
     cell_reset(d);
     list_generate(d);
     return 0;
@@ -1829,7 +2038,7 @@ void psph_o_destroy(psph_odomain *d)
     if (!d) return;
     for (size_t i = 0; i < d->n; ++i) { omp_destroy_lock(&d->P[i]->lock); free(d->P[i]); }
     for (int k = 0; k < d->nthreads; ++k) { free(d->SM[k].v); free(d->FSM[k].v); free(d->NSM[k].v); }
-    free(d->SM); free(d->FSM); free(d->NSM); free(d->nsm);
+    free(d->SM); free(d->FSM); free(d->NSM); free(d->nsm); free(d->inpart);
     free(d->P); free(d->fixed); free(d->HOC);
     free(d);
 }

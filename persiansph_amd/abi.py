@@ -69,6 +69,10 @@ class Config(ctypes.Structure):
         ("dom_max", ctypes.c_double * 3), ("dom_min", ctypes.c_double * 3),
         ("trpr", ctypes.c_double * 3), ("blpf", ctypes.c_double * 3),
         ("R", ctypes.c_double), ("mu_max", ctypes.c_double), ("cs_max", ctypes.c_double),
+        ("in_v", ctypes.c_double * 3), ("out_v", ctypes.c_double * 3),
+        ("in_density", ctypes.c_double), ("out_density", ctypes.c_double),
+        ("bc_cellfac", ctypes.c_double), ("initial_dist", ctypes.c_double),
+        ("mass_conservation", ctypes.c_int32), ("reserved_io", ctypes.c_int32),
     ]
 
 
@@ -124,6 +128,7 @@ def default_config(dimension=2, kernel_type=0, visc_eq=0, gradient_type=0, schem
         c.dom_min[k] = 100000000000.0
     c.xsph = xsph
     c.sqrt_h_a = sqrt_h_a
+    c.bc_cellfac = 3.9                                     # Boundary::Boundary, Boundary_Condition.cpp:39
     return c
 
 

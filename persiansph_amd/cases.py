@@ -519,3 +519,36 @@ def fsi_2d(dx=0.01, scheme=0, E=1.0e6, rho_s=1200.0):
     cfg.fsi = 1
     dt = 0.2*1.3*dx/max(cs_w, cs_s)
     return cfg, p, dt
+
+
+def channel_inflow_2d(dx=0.01, L=0.5, H=0.12, v_in=1.0, cs=20.0, mu=1e-3, in_out_flow=3, scheme=0,
+                      out_v=0.0, out_density=0.0, mass_conservation=False):
+    """A 2-D channel with in/out-flow boundaries (Boundary::InOutFlow): fluid filling a channel
+    L x H between three layers of fixed no-slip walls above and below, the inflow zone at the
+    left imposing v = (v_in, 0, 0) and the reference density 1000, the outflow zone at the right
+    (out_v, out_density; 0 keeps the particles' own), particles that leave recycled into the
+    inflow.  Cubic spline, h = 1.3 dx, linear EOS with Cs, Morris viscosity mu."""
+    lay = 3
+    p = add_box_length(2, (0.0, -lay*dx, 0.0), L + dx/10.0, H + 2*lay*dx + dx/10.0, 0.0, dx/2.0, 1000.0,
+                       1.3*dx, 1, 0, False)
+    y = p["x"][:, 1]
+    wall = (y < 0.0) | (y > H)
+    p["IsFree"] = np.where(wall, 0, 1)
+    p["NoSlip"] = np.where(wall, 1, 0)
+    p["ID"] = np.where(wall, 2, 1)
+    p["Material"] = 1
+    p["Cs"] = cs
+    p["PresEq"] = 0
+    p["Mu"] = mu
+    p["MuRef"] = mu
+    p["v"][~wall, 0] = v_in
+    cfg = default_config(dimension=2, kernel_type=0, scheme=scheme)
+    cfg.in_out_flow = in_out_flow
+    cfg.in_v[0] = v_in
+    cfg.in_density = 1000.0
+    cfg.out_v[0] = out_v
+    cfg.out_density = out_density
+    cfg.mass_conservation = 1 if mass_conservation else 0
+    cfg.initial_dist = dx
+    dt = 0.2*1.3*dx/cs
+    return cfg, p, dt

@@ -421,6 +421,9 @@ public:
             cfg.trpr[c] = TRPR(c); cfg.blpf[c] = BLPF(c);
         }
         cfg.in_out_flow = BC.InOutFlow; cfg.swi_type = (int)SWIType; cfg.fsi = FSI ? 1 : 0;
+        for (int c = 0; c < 3; ++c) { cfg.in_v[c] = BC.inv(c); cfg.out_v[c] = BC.outv(c); }
+        cfg.in_density = BC.inDensity; cfg.out_density = BC.outDensity; cfg.bc_cellfac = BC.cellfac;
+        cfg.initial_dist = InitialDist; cfg.mass_conservation = BC.MassConservation ? 1 : 0;
         cfg.cellfac = Cellfac; cfg.xsph = XSPH; cfg.sqrt_h_a = sqrt_h_a; cfg.R = R;
         cfg.mu_max = MuMax; cfg.cs_max = CsMax;
         WholeVelocity();           // before the upload: its EOS pressure is the upload's (Domain.cpp:1656)
