@@ -18,9 +18,12 @@
 // User hooks: GeneralBefore may be SPH::BodyForceHook (a constant acceleration, Domain::BodyForce,
 // on every free particle, as 1-Poiseuille.cpp's UserAcc), which Solve applies on the GPU.
 //
-// Not available on the GPU path (Solve throws Fatal): in/out-flow boundaries, FSI and
-// multi-material pairs, and other GeneralBefore/GeneralAfter hooks than the default no-op and
-// the body-force hook.
+// In/out-flow boundaries (BC.InOutFlow with the default InCon/OutCon) run on the GPU; particles
+// the recycling creates are appended to Particles at the next download.
+//
+// Not available on the GPU path (Solve throws Fatal): in/out-flow boundaries with particles other
+// than fluid or with the Shepard filter, and other GeneralBefore/GeneralAfter hooks than the
+// default no-op and the body-force hook.
 #ifndef PERSIANSPH_AMD_DOMAIN_H
 #define PERSIANSPH_AMD_DOMAIN_H
 
@@ -686,17 +689,25 @@ private:
     }
 
     // Copies the device state back; particles deleted by the leave check are removed from
-    // Particles in order (Array::DelItems semantics).
+    // Particles in order (Array::DelItems semantics), particles the in/out-flow recycling created
+    // (Index past the uploaded ones, numbered in creation order) are appended as new objects.
     void download(psph_handle *h)
     {
         size_t n = 0;
         check(psph_download(h, nullptr, 0, &n));
         std::vector<psph_particle> buf(n);
         check(psph_download(h, buf.data(), n, &n));
+        for (size_t k = 0; k < n; ++k) {
+            const size_t ix = (size_t)buf[k].Index;
+            while (up_.size() <= ix) {
+                up_.push_back(new Particle(0, Vec3_t(0, 0, 0), Vec3_t(0, 0, 0), 0.0, 0.0, 0.0));
+                alive_.push_back(1);
+            }
+        }
         std::vector<char> keep(up_.size(), 0);
         Array<Particle *> next;
         for (size_t k = 0; k < n; ++k) {
-            Particle *p = up_[buf[k].Index];        // Index: position at upload
+            Particle *p = up_[buf[k].Index];        // Index: position at upload or creation
             keep[buf[k].Index] = 1;
             from_abi(buf[k], *p);
             next.Push(p);

@@ -93,6 +93,12 @@ struct psph_handle {
     psph_grid grid{};
     bool trans_ok = false;             // transient columns match the current order
     double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
+    // in/out-flow: Boundary::InPart (host indices, the reference's order), the zone limits,
+    // inoutcounter; the next upload index for particles the recycling creates
+    std::vector<int32_t> inpart;
+    double loc2 = 0.0, loc3 = 0.0;
+    int inoutcounter = 0;
+    int32_t next_index = 0;
     // timing
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -286,7 +292,7 @@ static ClassParams class_of(const psph_particle &p)
     c.G = p.G; c.K = p.K; c.Sigmay = p.Sigmay; c.c = p.c; c.phi = p.phi; c.psi = p.psi;
     c.IsFree = p.IsFree ? 1 : 0; c.NoSlip = p.NoSlip ? 1 : 0; c.Shepard = p.Shepard ? 1 : 0;
     c.LES = p.LES ? 1 : 0; c.Material = p.Material; c.PresEq = p.PresEq; c.VisM = p.VisM;
-    c.Fail = p.Fail; c.ID = p.ID; c.ShepardStep = p.ShepardStep; c.InOut = p.InOut;
+    c.Fail = p.Fail; c.ID = p.ID; c.ShepardStep = p.ShepardStep; c.InOut = 0;   // InOut: flag bits
     c.V = p.V; c.RhoF = p.RhoF; c.k = p.k; c.k2 = p.k2; c.d = p.d; c.n0 = p.n0;
     c.SeepageType = p.SeepageType;
     // a saturated soil particle belongs to the class of its unsaturated RefDensity
@@ -509,7 +515,15 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
         const psph_particle &q = p[i];
         ClassParams c = class_of(q);
         auto it = idx.find(c);
-        if (it == idx.end()) return fail(PSPH_ERR_STATE, "particle %zu has an unknown class", i);
+        if (it == idx.end()) {
+            // a parameter set new since upload (a particle the in/out-flow recycling created)
+            if ((int)h->classes.size() >= kMaxClasses)
+                return fail(PSPH_ERR_CAPACITY, "more than %d distinct particle parameter sets", kMaxClasses);
+            h->classes.push_back(c);
+            HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
+            P.nclasses = (int)h->classes.size();
+            it = idx.emplace(c, (int)h->classes.size() - 1).first;
+        }
         int ci = it->second;
         pos[i] = make_double4(q.x[0], q.x[1], q.x[2], q.Density);
         vel[i] = make_double4(q.v[0], q.v[1], q.v[2], q.Pressure);
@@ -519,6 +533,7 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
         mmu[i] = make_double2(q.Mass, q.Mu);
         flags[i] = (uint32_t)ci | (q.IsFree ? kFree : 0u) | (q.NoSlip ? kNoSlip : 0u) |
                    ((uint32_t)(q.Material & 3) << kMatShift);
+        flags[i] |= ((uint32_t)(q.InOut & 3)) << kInOutShift;
         if (q.Material == 3 && q.IsSat) flags[i] |= kSat;
         orig[i] = (int32_t)i;
         if (P.any_solid) {
@@ -608,6 +623,7 @@ static int read_state(psph_handle *h)
         if (P.scheme == 1 && fr) { q.va[0] = vh[i].x; q.va[1] = vh[i].y; q.va[2] = vh[i].z; q.Densitya = vh[i].w; }
         else { q.vb[0] = vh[i].x; q.vb[1] = vh[i].y; q.vb[2] = vh[i].z; q.Densityb = vh[i].w; }
         q.Mass = mmu[i].x; q.Mu = mmu[i].y;
+        q.InOut = flag_inout(flags[i]);
         if (P.multi) {
             const ClassParams &cl = h->classes[flag_cls(flags[i])];
             q.IsSat = (flags[i] & kSat) ? 1 : 0;
@@ -723,6 +739,187 @@ static int prime_first_step(psph_handle *h)
 
 // Order-preserving deletion of the particles outside [BLPF, TRPR] (CheckParticleLeave with
 // Array::DelItems, External/array.h:726-738); the survivors are re-uploaded.
+
+// ------------------------------------------------------------------------------------------
+// in/out-flow boundaries (Domain.cpp:1327-1635) on the host mirror: the inflow list and its
+// order, and the recycling of particles that left the box
+// ------------------------------------------------------------------------------------------
+
+// ListGenerate's cell of a particle (Domain.cpp:662-784), in the reference's CellNo grid
+static void ref_cell(const psph_handle *h, const psph_particle &q, int ijk[3])
+{
+    const psph_grid &g = h->grid;
+    ijk[0] = ijk[1] = ijk[2] = 0;
+    for (int c = 0; c < h->cfg.dimension; ++c) {
+        int i = (int)(std::floor((q.x[c] - g.blpf[c]) / g.cell_size[c]));
+        if (i < 0) { if ((g.blpf[c] - q.x[c]) <= g.hmax) i = 0; }
+        if (i >= g.cell_no[c]) { if ((q.x[c] - g.trpr[c]) <= g.hmax) i = g.cell_no[c]-1; }
+        ijk[c] = std::min(std::max(i, 0), g.cell_no[c]-1);
+    }
+}
+
+// The particles the reference's inflow scan meets (Domain.cpp:1514-1528, 1549-1563), in its
+// order: cell rows y outer, z middle, x from 0 to x1 inner, each cell's chain from the largest
+// index down
+static std::vector<int32_t> inflow_scan(const psph_handle *h, int x1, bool need_flag)
+{
+    struct E { int q2, q3, q1, idx; };
+    std::vector<E> e;
+    const psph_grid &g = h->grid;
+    const double loc1 = h->P.loc1;
+    for (size_t k = 0; k < h->host.size(); ++k) {
+        const psph_particle &q = h->host[k];
+        if (!q.IsFree || !(q.x[0] <= loc1) || (need_flag && q.InOut != 1)) continue;
+        int ijk[3];
+        ref_cell(h, q, ijk);
+        if (ijk[0] > x1) continue;
+        if (h->cfg.periodic[1] && ijk[1] >= g.cell_no[1]-2) continue;
+        if (h->cfg.periodic[2] && ijk[2] >= g.cell_no[2]-2) continue;
+        e.push_back({ijk[1], ijk[2], ijk[0], (int)k});
+    }
+    std::sort(e.begin(), e.end(), [](const E &a, const E &b) {
+        if (a.q2 != b.q2) return a.q2 < b.q2;
+        if (a.q3 != b.q3) return a.q3 < b.q3;
+        if (a.q1 != b.q1) return a.q1 < b.q1;
+        return a.idx > b.idx;
+    });
+    std::vector<int32_t> out;
+    out.reserve(e.size());
+    for (auto &x : e) out.push_back(x.idx);
+    return out;
+}
+
+// InFlowBCFresh's first call (Domain.cpp:1505-1542): the inflow zone and list, the outflow line
+static int inflow_init(psph_handle *h)
+{
+    Params &P = h->P;
+    const psph_config &c = h->cfg;
+    const psph_grid &g = h->grid;
+    h->inpart.clear();
+    if (P.inout == 1 || P.inout == 3) {
+        if (!(std::sqrt(c.in_v[0]*c.in_v[0] + c.in_v[1]*c.in_v[1] + c.in_v[2]*c.in_v[2]) > 0.0) || !(c.in_density > 0.0))
+            return fail(PSPH_ERR_INVALID, "BC.inv or BC.inDensity are not defined");
+        P.loc1 = g.blpf[0] + c.bc_cellfac*g.hmax;
+        int x1 = (int)(std::floor((P.loc1 - g.blpf[0]) / g.cell_size[0]));
+        h->inpart = inflow_scan(h, x1, false);
+        if (h->inpart.empty()) return fail(PSPH_ERR_INVALID, "no free particle in the inflow zone");
+        for (int32_t k : h->inpart) h->host[k].InOut = 1;
+        h->loc2 = h->loc3 = h->host[h->inpart[0]].x[0];
+        for (int32_t k : h->inpart) {
+            double x0 = h->host[k].x[0];
+            if (x0 < h->loc2) h->loc2 = x0;
+            if (x0 > h->loc3) h->loc3 = x0;
+        }
+    }
+    if (P.inout == 2 || P.inout == 3) P.out_loc = g.trpr[0] - c.bc_cellfac*g.hmax;
+    h->inoutcounter = 2;
+    P.n_in = (int)h->inpart.size();
+    return 0;
+}
+
+// The fields an inflow particle passes on (Domain.cpp:1366-1401)
+static void inflow_copy(psph_particle &a, const psph_particle &b, const double *x)
+{
+    for (int k = 0; k < 3; ++k) a.x[k] = x[k];
+    a.Material = 1; a.InOut = 1; a.FirstStep = 0;
+    a.P0 = b.P0; a.PresEq = b.PresEq; a.Cs = b.Cs;
+    a.Alpha = b.Alpha; a.Beta = b.Beta; a.Mu = b.Mu; a.MuRef = b.MuRef; a.T0 = 0.0;
+    a.Mass = b.Mass; a.h = b.h; a.ID = b.ID; a.TI = 0.0; a.RefDensity = b.RefDensity;
+    a.ct = b.ct; a.Shepard = b.Shepard; a.ShepardStep = b.ShepardStep;
+    a.ShepardCounter = b.ShepardCounter; a.LES = b.LES; a.CSmag = b.CSmag;
+}
+
+static int write_back(psph_handle *h);
+
+// InFlowBCLeave (Domain.cpp:1327-1496) at a halt of the device: particles that left the box are
+// recycled into the inflow positions the inflow particles past InFlowLoc1 leave behind
+static int inflow_leave(psph_handle *h, long long *ndel)
+{
+    int rc = read_state(h);
+    if (rc) return rc;
+    const double *T = h->grid.trpr, *B = h->grid.blpf;
+    const Params &P = h->P;
+    std::vector<psph_particle> &H = h->host;
+    std::vector<int32_t> delp, addb;
+    std::vector<double> addx;
+    for (size_t k = 0; k < H.size(); ++k) {
+        psph_particle &q = H[k];
+        if ((q.x[0] > T[0]) || (q.x[1] > T[1]) || (q.x[2] > T[2]) ||
+            (q.x[0] < B[0]) || (q.x[1] < B[1]) || (q.x[2] < B[2])) {
+            q.InOut = 0;
+            delp.push_back((int32_t)k);
+        }
+    }
+    if (P.inout == 1 || P.inout == 3) {
+        std::vector<int32_t> keep;
+        for (int32_t k : h->inpart) {
+            psph_particle &q = H[k];
+            if (q.x[0] > P.loc1) {
+                double t[3] = {q.x[0], q.x[1], q.x[2]};
+                t[0] -= (h->loc3 - h->loc2 + h->cfg.initial_dist);
+                q.InOut = 0; q.FirstStep = 0; q.ShepardCounter = 0;
+                addx.insert(addx.end(), t, t + 3);
+                addb.push_back(k);
+            } else {
+                keep.push_back(k);
+            }
+        }
+        h->inpart.swap(keep);
+    }
+    const size_t nd = delp.size(), na = addb.size();
+    if (na >= nd) {
+        for (size_t i = 0; i < nd; ++i) {
+            inflow_copy(H[delp[i]], H[addb[i]], &addx[3*i]);
+            h->inpart.push_back(delp[i]);
+        }
+        for (size_t i = nd; i < na; ++i) {
+            // Particle::Particle (Particle.cpp:25-117) with the inflow particle's ID, v, Mass,
+            // RefDensity and h
+            const psph_particle b = H[addb[i]];
+            psph_particle a;
+            std::memset(&a, 0, sizeof a);
+            for (int k = 0; k < 3; ++k) a.v[k] = b.v[k];
+            a.TIn = 4.0; a.Density = b.RefDensity; a.RefDensity = b.RefDensity; a.Mass = b.Mass;
+            a.FPMassC = 1.0; a.IsFree = 1; a.h = b.h; a.ID = b.ID; a.m = 300.0;
+            a.FirstStep = 1; a.V = a.Mass/a.RefDensity; a.ShepardStep = 40; a.CSmag = 0.17;
+            inflow_copy(a, b, &addx[3*i]);
+            H.push_back(a);
+            h->upload_index.push_back(h->next_index++);
+            h->inpart.push_back((int32_t)H.size() - 1);
+        }
+    } else {
+        for (size_t i = 0; i < na; ++i) {
+            inflow_copy(H[delp[i]], H[addb[i]], &addx[3*i]);
+            h->inpart.push_back(delp[i]);
+        }
+        // Particles.DelItems of the rest, order kept (External/array.h:726-738)
+        std::vector<char> gone(H.size(), 0);
+        for (size_t i = na; i < nd; ++i) gone[delp[i]] = 1;
+        std::vector<psph_particle> keep;
+        std::vector<int32_t> keep_idx;
+        for (size_t k = 0; k < H.size(); ++k) {
+            if (gone[k]) { ++*ndel; continue; }
+            keep.push_back(H[k]);
+            keep_idx.push_back(h->upload_index[k]);
+        }
+        H.swap(keep);
+        h->upload_index.swap(keep_idx);
+        h->inoutcounter = 1;
+    }
+    if (H.size() > h->cap)
+        return fail(PSPH_ERR_CAPACITY, "the in/out-flow recycling needs %zu particles, the buffers hold %zu",
+                    H.size(), h->cap);
+    if (h->inoutcounter == 1) {
+        // InFlowBCFresh's rebuild of the inflow list (Domain.cpp:1545-1566), from the cells of
+        // these positions, which the next step starts from
+        int x1 = (int)(std::floor((h->P.loc1 - h->grid.blpf[0]) / h->grid.cell_size[0]));
+        h->inpart = inflow_scan(h, x1, true);
+        h->inoutcounter = 2;
+    }
+    h->P.n_in = (int)h->inpart.size();
+    return write_back(h);
+}
+
 static int delete_leavers(psph_handle *h, long long *ndel)
 {
     int rc = read_state(h);
@@ -741,13 +938,25 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     }
     h->host.swap(keep);
     h->upload_index.swap(keep_idx);
+    *ndel = del;
+    return write_back(h);
+}
+
+// The host mirror replaces the device state (after deletions or recycling)
+static int write_back(psph_handle *h)
+{
+    int rc = 0;
     Params &P = h->P;
     P.n = (int)h->host.size();
     int nf = 0;
     for (auto &q : h->host) nf += q.IsFree ? 1 : 0;
+    const bool grew = nf > P.nfree;
     P.nfree = nf;
     P.nfixed = P.n - nf;
-    // the compacted state replaces the current one
+    if (grew && P.nl_cap > 0) {                 // lists are per free target
+        rc = nl_alloc(h, P.nl_cap);
+        if (rc) return rc;
+    }
     rc = write_state(h, h->host.data(), h->host.size(), h->ctl_host->side);
     if (rc) return rc;
     HIPCHECK(hipMemsetAsync(h->D.ccount, 0, sizeof(int32_t)*(P.ncells + 1), h->stream));
@@ -755,10 +964,7 @@ static int delete_leavers(psph_handle *h, long long *ndel)
     h->ctl_host->leave = 0;
     h->ctl_host->halt = kRun;
     h->trans_ok = false;
-    rc = prime_first_step(h);
-    if (rc) return rc;
-    *ndel = del;
-    return 0;
+    return prime_first_step(h);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -789,7 +995,10 @@ int psph_create(const psph_config *cfg, psph_handle **out)
         return fail(PSPH_ERR_INVALID, "precision must be PSPH_FP64 or PSPH_FP32_MIXED");
     if (cfg->dimension == 2 && cfg->periodic[2])
         return fail(PSPH_ERR_UNSUPPORTED, "a periodic z direction in a 2-D run is outside this path");
-    if (cfg->in_out_flow) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries are outside this path");
+    if (cfg->in_out_flow < 0 || cfg->in_out_flow > 3)
+        return fail(PSPH_ERR_INVALID, "InOutFlow %d out of range", cfg->in_out_flow);
+    if (cfg->in_out_flow > 0 && cfg->periodic[0])
+        return fail(PSPH_ERR_INVALID, "Periodic BC in the X direction cannot be used with In/Out-Flow BC simultaneously");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PSPH_ERR_NO_DEVICE, "no HIP device visible");
     psph_handle *h = new psph_handle();
@@ -853,7 +1062,6 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         if (q.Material < 1 || q.Material > 3)
             return fail(PSPH_ERR_INVALID, "Material type %d out of range (1 fluid, 2 solid, 3 soil)", q.Material);
         mats[q.Material] = true;
-        if (q.InOut != 0) return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow particles are outside this path");
         if (q.PresEq < 0 || q.PresEq > 2) return fail(PSPH_ERR_INVALID, "Pressure Equation No %d out of range", q.PresEq);
         if (q.Material == 3 && q.Fail > 1 && q.Fail != 2 && q.Fail != 3) return fail(PSPH_ERR_INVALID, "Failure Type %d out of range", q.Fail);
         if (q.Material == 2 && q.Fail > 1) return fail(PSPH_ERR_INVALID, "Undefined failure criteria for solids");
@@ -929,6 +1137,20 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         return fail(PSPH_ERR_INVALID, "Soil-Water Interaction Type No %d is out of range", P.swi_type);
     P.swi1 = (P.multi && mats[3] && P.swi_type == 1) ? 1 : 0;
     P.any_soil_fluid = P.multi && mats[3];
+    // in/out-flow boundaries with the default InCon/OutCon
+    P.inout = h->cfg.in_out_flow;
+    if (P.inout) {
+        const psph_config &c = h->cfg;
+        for (int k = 0; k < 3; ++k) { P.in_v[k] = c.in_v[k]; P.out_v[k] = c.out_v[k]; }
+        P.in_density = c.in_density;
+        P.out_density = c.out_density;
+        P.mass_cons = c.mass_conservation ? 1 : 0;
+        if (P.multi || mat0 != 1)
+            return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries with particles other than fluid");
+        for (auto &c : h->classes)
+            if (c.IsFree && c.Shepard && P.scheme == 0)
+                return fail(PSPH_ERR_UNSUPPORTED, "in/out-flow boundaries with the Shepard filter (per-particle counters)");
+    }
     P.any_solid_fluid = P.multi && mats[2];
     P.uniform_h = 1;
     P.h_uniform = h->classes[0].h;
@@ -947,7 +1169,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         double mu0 = have_free ? ref.Mu : 0.0;
         for (size_t i = 0; i < n; ++i)
             if (h->host[i].IsFree && h->host[i].Mu != mu0) { mu_uniform = false; break; }
-        bool ok = mat0 == 1 && !P.multi && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
+        bool ok = mat0 == 1 && !P.multi && !P.inout && P.kernel_type == 0 && P.visc_eq == 0 && P.gradient_type == 0 &&
                   !P.any_ti && !P.any_sr && !P.any_xsph && !P.any_shepard && P.uniform_h &&
                   mu_uniform && P.nclasses <= 8 && have_free && !P.any_periodic &&
                   !std::getenv("PSPH_GENERAL_KERNELS");
@@ -1007,8 +1229,14 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.verlet_force = std::getenv("PSPH_REBUILD_EVERY_STEP") ? 1 : 0;
         }
     }
-    rc = allocate(h, n);
+    // in/out-flow runs may grow by recycling: room for half as many particles again
+    rc = allocate(h, P.inout ? n + n/2 + 256 : n);
     if (rc) return rc;
+    h->next_index = (int32_t)n;
+    if (P.inout) {
+        rc = inflow_init(h);
+        if (rc) return rc;
+    }
     // neighbour lists for the fast fluid path (entries pack the index in 27 bits, 25 with reuse)
     P.nl_cap = 0;
     P.nl_codes = n < ((size_t)1 << 27) ? 1 : 0;
@@ -1116,7 +1344,7 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
         }
         if (C.halt == kHaltLeave) {
             long long nd = 0;
-            rc = delete_leavers(h, &nd);
+            rc = h->P.inout ? inflow_leave(h, &nd) : delete_leavers(h, &nd);
             if (rc) { h->broken = true; return rc; }
             s.deleted += nd;
             if (C.steps_done >= max_steps || !(C.time < tf)) {

@@ -36,6 +36,8 @@ constexpr uint32_t kNoSlip = 1u << 17;
 constexpr int kMatShift = 18;      // 2 bits: Material 1..3
 constexpr uint32_t kSat = 1u << 20;       // IsSat: saturated soil (density and RefDensity less RhoF)
 constexpr uint32_t kSatCheck = 1u << 21;  // SatCheck of the last force pass
+constexpr int kInOutShift = 22;           // 2 bits: InOut (1 inflow, 2 outflow)
+__host__ __device__ inline int flag_inout(uint32_t f) { return (int)((f >> kInOutShift) & 3u); }
 
 __host__ __device__ inline int flag_cls(uint32_t f) { return (int)(f & kClsMask); }
 __host__ __device__ inline bool flag_free(uint32_t f) { return (f & kFree) != 0; }
@@ -73,7 +75,7 @@ struct Ctl {
     int side;                           // buffer of the persistent columns at the step's start
     int nwact;                          // wall targets with skin neighbours at the last build
     long long rebuilds;                 // list builds (sorts) since upload
-    int ct, shep_counter, first_step, pad;
+    int ct, shep_counter, first_step, n_out;   // n_out: outflow particles of this step
     long long reorders;                 // completed reorders (buffer parity)
 };
 
@@ -92,6 +94,13 @@ struct Params {
     // k_cross, the FSI smoothing (fsi) and the saturation of soil (swi_type != 2)
     int multi, fsi, swi_type, any_soil_fluid, any_solid_fluid;
     int swi1;                          // SWIType 1 with soil and fluid: S and ZWab go to swz
+    // in/out-flow boundaries (Boundary::InOutFlow 1, 2, 3) with the constant default conditions:
+    // inflow particles (InOut 1) carry in_v and in_density, outflow particles (x >= out_loc,
+    // InOut 2) out_v scaled by the mass-conservation factor n_in/n_out and out_density if
+    // positive; an inflow particle past loc1 or a particle outside the box halts the step for
+    // the host's recycling
+    int inout, mass_cons, n_in;
+    double in_v[3], out_v[3], in_density, out_density, loc1, out_loc;
     int nc[3], ncells;
     int n, nfree, nfixed, nclasses;
     int dim, kernel_type, visc_eq, gradient_type, scheme, precision;

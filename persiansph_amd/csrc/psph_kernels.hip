@@ -1298,6 +1298,66 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
 
 
 // ------------------------------------------------------------------------------------------
+// in/out-flow boundaries: InFlowBCFresh's imposition (Domain.cpp:1588-1635) with the default
+// InCon/OutCon (the boundary's constant velocity and density); the inflow set is the particles
+// flagged InOut 1 (the host keeps the reference's list order for the recycling)
+// ------------------------------------------------------------------------------------------
+// Outflow particles of this step: free and at or past OutFlowLoc; flagged InOut 2 and counted.
+__global__ void __launch_bounds__(kBlock) k_io_mark(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t f = D.flags_b[i];
+    if (!flag_free(f) || !(P.inout == 2 || P.inout == 3)) return;
+    if (D.pos_b[i].x >= P.out_loc) {
+        D.flags_b[i] = (f & ~(3u << kInOutShift)) | (2u << kInOutShift);
+        atomicAdd(&D.ctl->n_out, 1);
+    }
+}
+
+// The imposed state: velocity (v, va, vb), density (Density, Densitya, Densityb) and its EOS
+// pressure; outflow velocity scaled by InPart/OutPart with mass conservation, set only if the
+// boundary has one or the scale differs from 1; outflow density only if positive.
+__global__ void __launch_bounds__(kBlock) k_io_impose(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t f = D.flags_b[i];
+    const int io = flag_inout(f);
+    if (!flag_free(f) || io == 0) return;
+    double4 p = D.pos_b[i], v = D.vel_b[i], vh = D.vh_b[i];
+    bool set_v = false, set_d = false;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0, den = 0.0;
+    if (io == 1) {
+        set_v = set_d = true;
+        w0 = P.in_v[0]; w1 = P.in_v[1]; w2 = P.in_v[2]; den = P.in_density;
+    } else if (p.x >= P.out_loc) {
+        const double t11 = P.mass_cons ? P.n_in*1.0/(D.ctl->n_out*1.0) : 1.0;
+        const double nv = sqrt(P.out_v[0]*P.out_v[0] + P.out_v[1]*P.out_v[1] + P.out_v[2]*P.out_v[2]);
+        set_v = (nv > 0.0 || t11 != 1.0);
+        w0 = t11*P.out_v[0]; w1 = t11*P.out_v[1]; w2 = t11*P.out_v[2];
+        set_d = P.out_density > 0.0;
+        den = P.out_density;
+    }
+    if (!set_v && !set_d) return;
+    const ClassParams &c = D.cls[flag_cls(f)];
+    if (set_v) { v.x = w0; v.y = w1; v.z = w2; vh.x = w0; vh.y = w1; vh.z = w2; }
+    if (set_d) { p.w = den; vh.w = den; v.w = eos(c.PresEq, c.Cs, c.P0, den, c.RefDensity); }
+    D.pos_b[i] = p;
+    D.vel_b[i] = v;
+    D.vh_b[i] = vh;
+    // the pair formulas' per-particle values (k_reorder's sound speed and P/d^2)
+    double4 a = D.aux[i];
+    a.z = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
+    a.w = v.w/(p.w*p.w);
+    D.aux[i] = a;
+}
+
+// ------------------------------------------------------------------------------------------
 // pairs of different materials (LastComputeAcceleration's NSM loop, Domain.cpp:1256-1271)
 // ------------------------------------------------------------------------------------------
 
@@ -2188,6 +2248,12 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
         double4 v = D.vel_b[i];
         double4 vh = D.vh_b[i];
         double4 a = D.acc[i];
+        if (P.inout && flag_inout(f) > 0) {
+            // in/out-flow particles move with the imposed velocity (Domain.cpp:1305-1320)
+            a.x = 0.0; a.y = 0.0; a.z = 0.0;
+            if (flag_inout(f) == 1 || P.out_density > 0.0) a.w = 0.0;
+            D.acc[i] = a;
+        }
         double rho = p.w;
         double P_ = v.w;
         if (P.scheme == 1) {
@@ -2353,6 +2419,10 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
         double d0 = p.x - r.x, d1 = p.y - r.y, d2 = p.z - r.z;
         if (d0*d0 + d1*d1 + d2*d2 > P.skin_trig2) ctl->need_rebuild = 1;
     }
+    // an inflow particle past InFlowLoc1 halts the step for the host's recycling
+    // (InFlowBCLeave, Domain.cpp:1344-1357)
+    if ((P.inout == 1 || P.inout == 3) && flag_free(f) && flag_inout(f) == 1 && p.x > P.loc1)
+        atomicAdd(&ctl->leave, 1);
     // CheckParticleLeave test against [BLPF, TRPR]
     if ((p.x > P.trpr[0]) || (p.y > P.trpr[1]) || (p.z > P.trpr[2]) ||
         (p.x < P.blpf[0]) || (p.y < P.blpf[1]) || (p.z < P.blpf[2])) {
@@ -2386,6 +2456,7 @@ __global__ void k_tail(Dev D, Params P)
     c->steps_done += 1;
     c->resume = 0;
     c->mid_step = 0;
+    c->n_out = 0;
     // the step's working copy is the next step's state
     c->side ^= c->rebuild;
     if (P.verlet) {
@@ -2555,6 +2626,11 @@ static void launch_wall_pass(const Dev &D, const Params &P, hipStream_t s);
 // in the order of PrimaryComputeAcceleration (Domain.cpp:1020-1234).
 void launch_wall(const Dev &D, const Params &P, hipStream_t s)
 {
+    if (P.inout) {
+        // InFlowBCFresh runs before the neighbour search (Domain.cpp:1763-1766)
+        hipLaunchKernelGGL(k_io_mark, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+        hipLaunchKernelGGL(k_io_impose, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+    }
     launch_wall_pass(D, P, s);
     if (P.multi) {
         hipLaunchKernelGGL(k_fsi, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
