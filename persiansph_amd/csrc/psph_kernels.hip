@@ -859,7 +859,7 @@ struct AccT {
 using Acc = AccT<double>;
 
 // CalcForce11 (Interaction.cpp:25-184) contribution of neighbour j to free particle i as P1.
-__device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+__device__ __forceinline__ void pair11(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                               uint32_t fi, double4 pi, double4 vi, double4 ai, Acc &A, int shep_on)
 {
     uint32_t fj = D.flags_b[j];
@@ -964,7 +964,7 @@ __device__ inline void pair11(const Dev &D, const Params &P, const ClassParams *
 // with the pair arithmetic and sums in T (double: the parity tier; float: the FP32-mixed tier,
 // positions differenced in FP64 and rounded, Kahan-compensated continuity sum).
 template <class T>
-__device__ inline void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+__device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                                 uint32_t fi, double4 pi, double4 vi, double4 ai, const double *sgi,
                                 const double *tri, AccT<T> &A, int shep_on)
 {
@@ -1195,7 +1195,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     // a pair of the target's own material: CalcForce11 for fluid targets, CalcForce2233 for
     // solid and soil ones (LastComputeAcceleration, Domain.cpp:1244-1254); runs with fluid and
     // soil or solid together take the SOLID kernel
-    auto pair_same = [&](int jj) {
+    auto pair_same = [&](int jj) __attribute__((always_inline)) {
         if constexpr (SOLID) {
             if constexpr (sizeof(T) == 8) {
                 if (mat == 1) { pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on); return; }
@@ -1470,7 +1470,7 @@ struct XAcc { double a0, a1, a2, dd; double sr[6]; };
 // the two fluid and the other solid.  The fluid particle's EOS turns the solid particle's FSI
 // pressure into its density; both masses are the fluid particle's; no artificial viscosity
 // (the reference zeroes Alpha and Beta before testing them).
-__device__ inline void pair12(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+__device__ __forceinline__ void pair12(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                               uint32_t fi, uint32_t fj, double4 pi, double4 pj, double x0, double x1,
                               double x2, double rij, double h, XAcc &A)
 {
@@ -1518,7 +1518,7 @@ __device__ inline void pair12(const Dev &D, const Params &P, const ClassParams *
 }
 
 // Seepage (Functions.cpp:266-298): Darcy (0, 1) or Forchheimer (2, 3) coefficients
-__device__ inline void seepage(int st, double k, double k2, double mu, double rho, double &SF1, double &SF2)
+__device__ __forceinline__ void seepage(int st, double k, double k2, double mu, double rho, double &SF1, double &SF2)
 {
     SF1 = mu/k;
     SF2 = (st >= 2) ? k2*rho : 0.0;
@@ -1527,7 +1527,7 @@ __device__ inline void seepage(int st, double k, double k2, double mu, double rh
 // CalcForce13 (Interaction.cpp:508-653) contribution to free particle i as P1 from j, one of
 // the two soil and the other water: the seepage force (SWIType 0 and 2), with SWIType 2 the
 // water pressure on the soil particle, nothing with SWIType 3.  K is over both densities.
-__device__ inline void pair13(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
+__device__ __forceinline__ void pair13(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                               uint32_t fi, uint32_t fj, double4 pi, double4 pj, double x0, double x1,
                               double x2, double rij, double h, XAcc &A)
 {

@@ -16,8 +16,8 @@ template <> struct K_<float> { static constexpr float PI = 3.1415926535897932384
 // Integer powers: the reference calls pow(x, n.0); for the FP64 path pow() from the device
 // math library is used (correctly rounded to within its documented ulp bound); for FP32 a
 // multiplication chain.
-__device__ inline double ipow(double x, double n) { return pow(x, n); }
-__device__ inline float ipow(float x, float n)
+__device__ __forceinline__ double ipow(double x, double n) { return pow(x, n); }
+__device__ __forceinline__ float ipow(float x, float n)
 {
     int k = (int)n;
     float r = 1.0f;
@@ -27,7 +27,7 @@ __device__ inline float ipow(float x, float n)
 
 // Kernel, Functions.cpp:25-63.  C is passed precomputed when h is known per class pair.
 template <class T>
-__device__ inline T kernel_w(int dim, int kt, T q, T h)
+__device__ __forceinline__ T kernel_w(int dim, int kt, T q, T h)
 {
     const T PI = K_<T>::PI;
     T C;
@@ -52,7 +52,7 @@ __device__ inline T kernel_w(int dim, int kt, T q, T h)
 
 // GradKernel, Functions.cpp:65-106: (1/r) dW/dr.
 template <class T>
-__device__ inline T grad_kernel(int dim, int kt, T q, T h)
+__device__ __forceinline__ T grad_kernel(int dim, int kt, T q, T h)
 {
     const T PI = K_<T>::PI;
     T C;
@@ -80,7 +80,7 @@ __device__ inline T grad_kernel(int dim, int kt, T q, T h)
 
 // LaplaceKernel, Functions.cpp:108-146.
 template <class T>
-__device__ inline T laplace_kernel(int dim, int kt, T q, T h)
+__device__ __forceinline__ T laplace_kernel(int dim, int kt, T q, T h)
 {
     const T PI = K_<T>::PI;
     T C;
@@ -106,7 +106,7 @@ __device__ inline T laplace_kernel(int dim, int kt, T q, T h)
 
 // SecDerivativeKernel, Functions.cpp:148-186.
 template <class T>
-__device__ inline T secderiv_kernel(int dim, int kt, T q, T h)
+__device__ __forceinline__ T secderiv_kernel(int dim, int kt, T q, T h)
 {
     const T PI = K_<T>::PI;
     T C;
@@ -130,7 +130,7 @@ __device__ inline T secderiv_kernel(int dim, int kt, T q, T h)
 }
 
 // EOS, Functions.cpp:188-212.
-__device__ inline double eos(int eq, double Cs0, double P00, double Density, double Density0)
+__device__ __forceinline__ double eos(int eq, double Cs0, double P00, double Density, double Density0)
 {
     switch (eq) {
     case 0: return P00+(Cs0*Cs0)*(Density-Density0);
@@ -140,14 +140,14 @@ __device__ inline double eos(int eq, double Cs0, double P00, double Density, dou
 }
 
 // SoundSpeed, Functions.cpp:214-238.
-__device__ inline double sound_speed(int eq, double Cs0, double Density, double Density0)
+__device__ __forceinline__ double sound_speed(int eq, double Cs0, double Density, double Density0)
 {
     if (eq == 1) return sqrt((Cs0*Cs0)*pow(Density/Density0,6.0));
     return Cs0;
 }
 
 // DensitySolid, Functions.cpp:240-264.
-__device__ inline double density_solid(int eq, double Cs0, double P00, double Pressure, double Density0)
+__device__ __forceinline__ double density_solid(int eq, double Cs0, double P00, double Pressure, double Density0)
 {
     switch (eq) {
     case 0: return (Pressure-P00)/(Cs0*Cs0) + Density0;
@@ -158,7 +158,7 @@ __device__ inline double density_solid(int eq, double Cs0, double P00, double Pr
 
 // Viscous_Force, Functions.cpp:300-332 (VI returned through v0..v2).
 template <class T>
-__device__ inline void viscous_force(int viseq, T &v0, T &v1, T &v2, T Mu, T di, T dj, T GK,
+__device__ __forceinline__ void viscous_force(int viseq, T &v0, T &v1, T &v2, T Mu, T di, T dj, T GK,
                                      T b0, T b1, T b2, int dim, int kt, T rij, T h,
                                      T x0, T x1, T x2, T u0, T u1, T u2)
 {
@@ -193,20 +193,20 @@ __device__ inline void viscous_force(int viseq, T &v0, T &v1, T &v2, T Mu, T di,
 
 // Symmetric 3x3 tensors stored as 6 components (xx, xy, xz, yy, yz, zz).
 template <class T>
-__device__ inline void sym_to_full(const T s[6], T M[9])
+__device__ __forceinline__ void sym_to_full(const T s[6], T M[9])
 {
     M[0] = s[0]; M[1] = s[1]; M[2] = s[2];
     M[3] = s[1]; M[4] = s[3]; M[5] = s[4];
     M[6] = s[2]; M[7] = s[4]; M[8] = s[5];
 }
 
-__device__ inline void full_to_sym(const double M[9], double s[6])
+__device__ __forceinline__ void full_to_sym(const double M[9], double s[6])
 {
     s[0] = M[0]; s[1] = M[1]; s[2] = M[2]; s[3] = M[4]; s[4] = M[5]; s[5] = M[8];
 }
 
 // Mult(Mat3,Mat3), External/matvec.h:822-828.
-__device__ inline void mat_mult(const double *A, const double *B, double *M)
+__device__ __forceinline__ void mat_mult(const double *A, const double *B, double *M)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -218,7 +218,7 @@ __device__ inline void mat_mult(const double *A, const double *B, double *M)
 // Symmetric eigen decomposition by cyclic Jacobi rotations, eigenvalues ascending and
 // eigenvectors as columns, for the artificial-stress rotation (Functions.cpp:334-348, whose
 // GSL gsl_eigen_symmv call reads the lower triangle).
-__device__ inline void eig_sym3(const double m[9], double vals[3], double vecs[9])
+__device__ __forceinline__ void eig_sym3(const double m[9], double vals[3], double vecs[9])
 {
     double a[3][3], v[3][3];
 #pragma unroll
@@ -275,7 +275,7 @@ __device__ inline void eig_sym3(const double m[9], double vals[3], double vecs[9
 
 // Artificial-stress tensor TIR (Domain.cpp:955-987, :1171-1204), symmetric output.
 // For 2-D only the xx, xy, yy entries are written (the reference leaves the rest untouched).
-__device__ inline void tir_tensor(int dim, const double sg[6], double TI, double Density, double out[6])
+__device__ __forceinline__ void tir_tensor(int dim, const double sg[6], double TI, double Density, double out[6])
 {
     if (dim == 2) {
         double teta, Sigmaxx, Sigmayy, C, S;
