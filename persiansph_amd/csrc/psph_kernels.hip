@@ -1132,21 +1132,35 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
         } else if (!(rs & 1)) {
             row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
         }
-        for (int j = beg; j < end; ++j) {
-            if (j == i) continue;
-            uint32_t fj = D.flags_b[j];
-            if (flag_mat(fj) != mat) {
-                // pairs of different materials go to k_cross; none between solid and soil
-                const int prod = mat*flag_mat(fj);
-                if (P.multi && (prod == 2 || prod == 3)) xc = 1; else xmat = 1;
-                continue;
+        // candidates in groups of 4: the group's loads are issued together (slots past the
+        // range re-read `beg`, which is in bounds, and are not tested), then tested in walk order
+        for (int j0 = beg; j0 < end; j0 += 4) {
+            uint32_t fg[4];
+            double4 pg[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = (j0 + u < end) ? j0 + u : beg;
+                fg[u] = D.flags_b[jj];
+                pg[u] = D.pos_b[jj];
             }
-            double4 pj = D.pos_b[j];
-            double h = P.uniform_h ? P.h_uniform : (hcls[ci] + hcls[flag_cls(fj)])/2;
-            double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
-            if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
-            double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-            if ((rij/h) <= P.cellfac) out.push(j);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j >= end || j == i) continue;
+                const uint32_t fj = fg[u];
+                if (flag_mat(fj) != mat) {
+                    // pairs of different materials go to k_cross; none between solid and soil
+                    const int prod = mat*flag_mat(fj);
+                    if (P.multi && (prod == 2 || prod == 3)) xc = 1; else xmat = 1;
+                    continue;
+                }
+                const double4 pj = pg[u];
+                double h = P.uniform_h ? P.h_uniform : (hcls[ci] + hcls[flag_cls(fj)])/2;
+                double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
+                double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
+                if ((rij/h) <= P.cellfac) out.push(j);
+            }
         }
     }
     out.flush();
