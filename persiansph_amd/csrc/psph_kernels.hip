@@ -303,6 +303,15 @@ __device__ inline void cell_origin(const Params &P, int key, double &o0, double 
 // Domain.cpp:951-988).  With list reuse, a step that does not rebuild keeps the order: the
 // columns are copied as they stand (the double buffer flips every step either way).
 // ------------------------------------------------------------------------------------------
+// One particle's 6-component tensor from slot s of src to slot i of dst
+__device__ __forceinline__ void copy_tensor6(const double *src, double *dst, int s, int i)
+{
+    const double2 *a = reinterpret_cast<const double2 *>(src + 6*(size_t)s);
+    double2 *b = reinterpret_cast<double2 *>(dst + 6*(size_t)i);
+    const double2 t0 = a[0], t1 = a[1], t2 = a[2];
+    b[0] = t0; b[1] = t1; b[2] = t2;
+}
+
 __global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -355,23 +364,25 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
                 D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
             }
             if (P.any_solid) {
+                // 48-B tensor records (16-B aligned) moved as three 16-B loads and stores
                 double sg[6];
-                for (int q = 0; q < 6; ++q) {
-                    sg[q] = D.sig[6*s+q];
-                    if (sorted) {
-                        D.sig_b[6*i+q] = sg[q];
-                        D.sigh_b[6*i+q] = D.sigh[6*s+q];
-                        D.tau_b[6*i+q] = D.tau[6*s+q];
-                        D.tauh_b[6*i+q] = D.tauh[6*s+q];
-                        D.eps_b[6*i+q] = D.eps[6*s+q];
-                        D.epsh_b[6*i+q] = D.epsh[6*s+q];
-                    }
+                const double2 *sg2 = reinterpret_cast<const double2 *>(D.sig + 6*(size_t)s);
+                for (int q = 0; q < 3; ++q) { const double2 t = sg2[q]; sg[2*q] = t.x; sg[2*q+1] = t.y; }
+                if (sorted) {
+                    double2 *d2 = reinterpret_cast<double2 *>(D.sig_b + 6*(size_t)i);
+                    for (int q = 0; q < 3; ++q) d2[q] = make_double2(sg[2*q], sg[2*q+1]);
+                    copy_tensor6(D.sigh, D.sigh_b, s, i);
+                    copy_tensor6(D.tau, D.tau_b, s, i);
+                    copy_tensor6(D.tauh, D.tauh_b, s, i);
+                    copy_tensor6(D.eps, D.eps_b, s, i);
+                    copy_tensor6(D.epsh, D.epsh_b, s, i);
                 }
                 if (flag_free(f)) {
                     // TIR is rebuilt every step; entries the 2-D formula leaves alone are zero
                     double t[6] = {0, 0, 0, 0, 0, 0};
                     if (c.Material > 1 && c.TI > 0.0) tir_tensor(P.dim, sg, c.TI, p.w, t);
-                    for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t[q];
+                    double2 *t2 = reinterpret_cast<double2 *>(D.tir + 6*(size_t)i);
+                    for (int q = 0; q < 3; ++q) t2[q] = make_double2(t[2*q], t[2*q+1]);
                 }
             }
         }
