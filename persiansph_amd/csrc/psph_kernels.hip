@@ -303,6 +303,19 @@ __device__ inline void cell_origin(const Params &P, int key, double &o0, double 
 // Domain.cpp:951-988).  With list reuse, a step that does not rebuild keeps the order: the
 // columns are copied as they stand (the double buffer flips every step either way).
 // ------------------------------------------------------------------------------------------
+// One particle's 6-component tensor (a 16-B aligned 48-B record) as three 16-B loads / stores
+__device__ __forceinline__ void ld6(const double *a, int i, double *v)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(a + 6*(size_t)i);
+    const double2 t0 = p[0], t1 = p[1], t2 = p[2];
+    v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y; v[4] = t2.x; v[5] = t2.y;
+}
+__device__ __forceinline__ void st6(double *a, int i, const double *v)
+{
+    double2 *p = reinterpret_cast<double2 *>(a + 6*(size_t)i);
+    p[0] = make_double2(v[0], v[1]); p[1] = make_double2(v[2], v[3]); p[2] = make_double2(v[4], v[5]);
+}
+
 // One particle's 6-component tensor from slot s of src to slot i of dst
 __device__ __forceinline__ void copy_tensor6(const double *src, double *dst, int s, int i)
 {
@@ -1021,7 +1034,9 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     if (Ci_.TI > 0.0 || Cj_.TI > 0.0) {
         T sT = pow((K/kernel_w<T>(P.dim, P.kernel_type, (T)((Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*(double)h)), h)),
                    (T)((Ci_.TIn+Cj_.TIn)/2.0));
-        for (int k = 0; k < 6; ++k) TI6[k] = (T)(tri[k]+D.tir[6*j+k])*sT;
+        double trj[6];
+        ld6(D.tir, j, trj);
+        for (int k = 0; k < 6; ++k) TI6[k] = (T)(tri[k]+trj[k])*sT;
     }
     T b0, b1, b2;
     if (freej) { b0 = u0; b1 = u1; b2 = u2; }
@@ -1042,7 +1057,9 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     // M = Sigma_i/di^2 + Sigma_j/dj^2 + PI I + TI, then temp = (GK xij)^T M (matvec.h:806-811)
     T Mf[9];
     T sgiT[6], sgj[6];
-    for (int k = 0; k < 6; ++k) { sgiT[k] = (T)sgi[k]; sgj[k] = (T)D.sig_b[6*j+k]; }
+    double sgjd[6];
+    ld6(D.sig_b, j, sgjd);
+    for (int k = 0; k < 6; ++k) { sgiT[k] = (T)sgi[k]; sgj[k] = (T)sgjd[k]; }
     T Mi[9], Mj[9], T9[9];
     sym_to_full(sgiT, Mi);
     sym_to_full(sgj, Mj);
@@ -1215,7 +1232,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
     double sgi[6] = {0, 0, 0, 0, 0, 0}, tri[6] = {0, 0, 0, 0, 0, 0};
     if (SOLID) {
-        for (int k = 0; k < 6; ++k) { sgi[k] = D.sig_b[6*i+k]; tri[k] = D.tir[6*i+k]; }
+        ld6(D.sig_b, i, sgi);
+        ld6(D.tir, i, tri);
     }
     // a pair of the target's own material: CalcForce11 for fluid targets, CalcForce2233 for
     // solid and soil ones (LastComputeAcceleration, Domain.cpp:1244-1254); runs with fluid and
@@ -1304,7 +1322,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
         if (P.any_xsph || zw_on) D.xsp[i] = make_double4(A.vx0, A.vx1, A.vx2, A.zw);
         if (P.swi1) D.swz[i] = make_double2((double)A.sw, (double)A.zw);
         if (shep_on) D.shep[i] = make_double2(A.sd, 0.0);
-        if (SOLID || P.any_sr) for (int k = 0; k < 6; ++k) D.sr[6*i+k] = A.sr[k];
+        if (SOLID || P.any_sr) { double srv[6]; for (int k = 0; k < 6; ++k) srv[k] = A.sr[k]; st6(D.sr, i, srv); }
         if (SOLID) for (int k = 0; k < 3; ++k) D.rr[3*i+k] = A.rr[k];
         // dt control: min over free particles of sqrt_h_a*sqrt(h/|a|) (Domain.cpp:1281-1296);
         // targets with pairs of another material take theirs after k_cross
@@ -2343,14 +2361,12 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
             }
         } else {
             double sr[6], rr[3], R[6];
-            for (int k = 0; k < 6; ++k) sr[k] = D.sr[6*i+k];
+            ld6(D.sr, i, sr);
             for (int k = 0; k < 3; ++k) rr[k] = D.rr[3*i+k];
             double sg[6], sh[6], tau[6], th[6], ep[6], eh[6];
-            for (int k = 0; k < 6; ++k) {
-                sg[k] = D.sig_b[6*i+k]; sh[k] = D.sigh_b[6*i+k];
-                tau[k] = D.tau_b[6*i+k]; th[k] = D.tauh_b[6*i+k];
-                ep[k] = D.eps_b[6*i+k]; eh[k] = D.epsh_b[6*i+k];
-            }
+            ld6(D.sig_b, i, sg); ld6(D.sigh_b, i, sh);
+            ld6(D.tau_b, i, tau); ld6(D.tauh_b, i, th);
+            ld6(D.eps_b, i, ep); ld6(D.epsh_b, i, eh);
             double i22 = (P.dim == 2) ? 0.0 : 1.0;
             const double I6[6] = {1, 0, 0, 1, 0, 1};
             if (c.Material == 2) {
@@ -2419,11 +2435,9 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
                     for (int k = 0; k < 6; ++k) eh[k] = st[k];
                 }
             }
-            for (int k = 0; k < 6; ++k) {
-                D.sig_b[6*i+k] = sg[k]; D.sigh_b[6*i+k] = sh[k];
-                D.tau_b[6*i+k] = tau[k]; D.tauh_b[6*i+k] = th[k];
-                D.eps_b[6*i+k] = ep[k]; D.epsh_b[6*i+k] = eh[k];
-            }
+            st6(D.sig_b, i, sg); st6(D.sigh_b, i, sh);
+            st6(D.tau_b, i, tau); st6(D.tauh_b, i, th);
+            st6(D.eps_b, i, ep); st6(D.epsh_b, i, eh);
         }
         // periodic position update (Particle.cpp:127-142)
         if (P.any_periodic) {
