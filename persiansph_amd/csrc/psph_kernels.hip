@@ -566,8 +566,11 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
                 double4 vj = D.vel_b[j];
                 Pw += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
             }
-            if (SOLID && mat > 1)
-                for (int q = 0; q < 6; ++q) sg[q] = sg[q] + D.sig_b[6*j+q]*K;
+            if (SOLID && mat > 1) {
+                double sj[6];
+                ld6(D.sig_b, j, sj);
+                for (int q = 0; q < 6; ++q) sg[q] = sg[q] + sj[q]*K;
+            }
             if (C.NoSlip) {
                 double4 vj = D.vel_b[j];
                 n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K;
@@ -597,12 +600,12 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     D.aux[i] = make_double4(mm.x, mm.y, Cw, Pw/(dw*dw));
     D.wal[i] = make_double4(n0, n1, n2, dw);
     if (SOLID) {
-        if (mat > 1) for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = sg[q];
-        else for (int q = 0; q < 6; ++q) D.sig_b[6*i+q] = 0.0;
-        for (int q = 0; q < 6; ++q) D.tau_b[6*i+q] = 0.0;
+        const double z6[6] = {0, 0, 0, 0, 0, 0};
+        st6(D.sig_b, i, mat > 1 ? sg : z6);
+        st6(D.tau_b, i, z6);
         double t6[6] = {0, 0, 0, 0, 0, 0};
         if (mat > 1 && C.TI > 0.0 && SK != 0.0) tir_tensor(P.dim, sg, C.TI, pi.w, t6);
-        for (int q = 0; q < 6; ++q) D.tir[6*i+q] = t6[q];
+        st6(D.tir, i, t6);
     }
 }
 
