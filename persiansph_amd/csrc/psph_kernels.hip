@@ -541,10 +541,24 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
         ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
         if (ex*ex + ey*ey + ez*ez > sup2m) continue;       // the cell lies outside the support
         int beg = D.cstart[cell], end = D.cstart[cell + 1];
-        for (int j = beg; j < end; ++j) {
-            uint32_t fj = D.flags_b[j];
+        // candidates in groups of 4: the group's loads are issued together (slots past the
+        // range re-read `beg`, which is in bounds, and are skipped), then processed in order
+        for (int j0 = beg; j0 < end; j0 += 4) {
+          uint32_t fg[4];
+          double4 pg[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int jj = (j0 + u < end) ? j0 + u : beg;
+            fg[u] = D.flags_b[jj];
+            pg[u] = D.pos_b[jj];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u;
+            if (j >= end) continue;
+            uint32_t fj = fg[u];
             if (!flag_free(fj) || flag_mat(fj) != mat) continue;
-            double4 pj = D.pos_b[j];
+            double4 pj = pg[u];
             double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
             double h = pair_h(cp, P, ci, flag_cls(fj));
             if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
@@ -575,6 +589,7 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
                 double4 vj = D.vel_b[j];
                 n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K;
             }
+        }
         }
     }
     // StartAcceleration resets fixed P, NSv, Sigma (Domain.cpp:990-996); then divide by SumKernel
