@@ -121,3 +121,29 @@ def test_couette_dropin_user_output(tmp_path):
         assert not d["Property2"].any() and not d["Property3"].any()
     assert np.abs(states[3]["ShearRate"]).max() > 0.0
     s.close()
+
+
+@pytest.mark.gpu
+def test_channel_inflow_dropin_matches_binding(tmp_path):
+    """In/out-flow boundaries through the drop-in (BC.InOutFlow = 3 with an outflow velocity
+    scaled by mass conservation): the recycling creates particles, which Domain appends to
+    Particles; the count, positions and velocities equal the same run through the Python
+    binding of the C ABI, bit for bit."""
+    from persiansph_amd import Solver, cases
+    exe = os.path.join(ROOT, "persiansph_amd", "cpp", "examples", "channel_inflow")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    cfg, p, dt = cases.channel_inflow_2d(out_v=0.8, out_density=1000.5, mass_conservation=True)
+    tf, dt_out = 0.03, 0.01
+    out = tmp_path / "out.bin"
+    print(f"[dropin] channel_inflow tf={tf}", flush=True)
+    r = subprocess.run([exe, repr(tf), repr(dt_out), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _read(out)
+    s = Solver(cfg, p, dt)
+    st = _solve_like_dropin(s, tf, dt_out, 1000)
+    q = s.particles()
+    assert len(q) == len(got) > len(p)            # particles were created
+    np.testing.assert_array_equal(q["x"], got[:, :3])
+    np.testing.assert_array_equal(q["v"][:, 0], got[:, 3])
+    assert f"{st.time:.9e}" in r.stdout
+    s.close()
