@@ -11,7 +11,9 @@ import numpy as np
 from persiansph_amd.abi import PARTICLE_DTYPE, Config, Grid, Status
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpsph_oracle.so")
+# PSPH_ORACLE_LIB points the binding at another build of the oracle (the mutant builds of
+# tests/test_oracle_mutants.py); by default it loads the library `make -C oracle` builds
+LIB_PATH = os.environ.get("PSPH_ORACLE_LIB") or os.path.join(_HERE, "libpsph_oracle.so")
 
 _lib = None
 

@@ -326,6 +326,52 @@ def granular_column(dim=3, dx=0.004, size=(0.2, 0.4, 0.2), floor_half=0.5, rho=2
     return cfg, p, dt
 
 
+def elastic_block(dim=2, dx=0.01, size=(0.06, 0.1, 0.06), floor_half=0.08, rho=1200.0, E=1.0e6,
+                  nu=0.3, v0=-0.5, fail=0, sigmay=5.0e3, ti=0.3, alpha=0.1, beta=0.0, scheme=1,
+                  layers=3, g=9.81):
+    """An elastic solid block (Material 2) dropped with velocity v0 (y) onto a rigid floor of
+    `layers` fixed Material 2 layers: the impact loads the block well beyond its weight.
+
+    Linear EOS (PresEq 0) with Cs = sqrt(K/rho), K = E/(3(1-2nu)), G = E/(2(1+nu)); Fail 1 adds
+    the von Mises scale-back at Sigmay (Particle.cpp:293-300, 613-620).  Artificial stress TI
+    with TIn = 2.55 and TIInitDist = dx, artificial viscosity alpha/beta, dt = 0.2 h/Cs."""
+    h = 1.3*dx
+    r = dx/2.0
+    K = E/(3.0*(1.0-2.0*nu))
+    G = E/(2.0*(1.0+nu))
+    cs = math.sqrt(K/rho)
+    Lx, Ly, Lz = size
+    nx, ny, nz = int(round(Lx/dx)), int(round(Ly/dx)), int(round(Lz/dx))
+    block = _lattice(dim, (0.0, 0.0, 0.0), (nx, ny, nz), r, rho, h)
+    nf = int(round(2*floor_half/dx))
+    fx0 = Lx/2.0 - nf*dx/2.0
+    if dim == 3:
+        fz0 = Lz/2.0 - nf*dx/2.0
+        fl = _lattice(3, (fx0, -layers*dx, fz0), (nf, layers, nf), r, rho, h)
+    else:
+        fl = _lattice(2, (fx0, -layers*dx, 0.0), (nf, layers, 1), r, rho, h)
+    fl["IsFree"] = 0
+    fl["ID"] = 2
+    block["v"][:, 1] = v0
+    p = np.concatenate([fl, block])
+    p["Material"] = 2
+    p["Fail"] = fail
+    p["Sigmay"] = sigmay
+    p["PresEq"] = 0
+    p["K"] = K
+    p["G"] = G
+    p["Cs"] = cs
+    p["Alpha"] = alpha
+    p["Beta"] = beta
+    p["TI"] = ti
+    p["TIn"] = 2.55
+    p["TIInitDist"] = dx
+    cfg = default_config(dimension=dim, kernel_type=0, visc_eq=0, gradient_type=0,
+                         scheme=scheme, gravity=(0.0, -g, 0.0))
+    dt = 0.2*h/cs
+    return cfg, p, dt
+
+
 def runout(p):
     """Runout of a granular column: the largest x reached by a free particle."""
     free = p["IsFree"] == 1
