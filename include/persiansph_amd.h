@@ -192,6 +192,16 @@ int psph_output_end(psph_handle *h, psph_output_f32 *out);
  * at least one free, same material.  pairs holds 2*cap int64 values. */
 int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs);
 
+/* psph_neighbour_pairs without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and state changes as
+ * psph_neighbour_pairs, but the pair list is staged in a static 32 MiB device array (one per
+ * GPU, held until the process ends) instead of a buffer allocated for the call, so at most
+ * 2,097,152 pairs (2^21) can be returned: with more, *npairs is set and the call fails with
+ * PSPH_ERR_CAPACITY.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several threads or handles) race. */
+int psph_neighbour_pairs_noalloc(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs);
+
 /* The grid computed at upload. */
 int psph_get_grid(psph_handle *h, psph_grid *g);
 

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,39 @@ int psph_internal_fail(int code, const char *msg) { return fail(code, "%s", msg)
         if (e_ != hipSuccess) return fail(PSPH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+// ------------------------------------------------------------------------------------------
+// Staging of psph_neighbour_pairs_noalloc: a file-scope device array (one copy per GPU, held
+// until the process ends) in place of the per-call allocation of psph_neighbour_pairs.  Host
+// code never takes the array's name as an address: device_scratch() looks up the current
+// device's copy with hipGetSymbolAddress, once per device, and every use goes through that
+// pointer.  All calls on one GPU share its copy, so pair queries that overlap on one GPU race.
+// ------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(32) << 20;                     // 32 MiB
+constexpr size_t kScratchPairs = kScratchBytes/(2*sizeof(long long));  // 2 Mi pairs
+static __device__ __align__(256) long long scratch[kScratchBytes/sizeof(long long)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+long long *g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+// The current device's copy of `scratch` (the caller has made `device` current).
+static int device_scratch(int device, long long **out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(PSPH_ERR_INVALID, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PSPH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_dev[device] = static_cast<long long *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return 0;
+}
+
 namespace {
 
 enum KernelId { KHead, KKeys, KScan, KScatter, KCellsort, KReorder, KWall, KForce, KIntegrate, KTail, KCount };
@@ -105,9 +139,12 @@ struct psph_handle {
     std::vector<hipEvent_t> event_pool;
     double t_ms[KCount] = {0};
     long long t_n[KCount] = {0};
-    // scratch for the pair query
+    // scratch for the pair query; the device's copy of the static staging array of
+    // psph_neighbour_pairs_noalloc (looked up by psph_create) or the lookup's error
     int *pair_cnt = nullptr;
     long long *pair_off = nullptr;
+    long long *pair_scratch = nullptr;
+    std::string pair_scratch_err;
     // neighbour lists of the fast fluid path (capacity in Params::nl_cap; skin lists with list
     // reuse); skin sub-lists of the wall targets (capacity Params::wl_cap)
     int32_t *nl = nullptr;
@@ -1004,6 +1041,7 @@ int psph_create(const psph_config *cfg, psph_handle **out)
     psph_handle *h = new psph_handle();
     h->cfg = *cfg;
     hipGetDevice(&h->device);
+    if (device_scratch(h->device, &h->pair_scratch)) h->pair_scratch_err = g_err;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(PSPH_ERR_HIP, "hipStreamCreate failed");
@@ -1477,7 +1515,10 @@ int psph_output_end(psph_handle *h, psph_output_f32 *out)
     return 0;
 }
 
-int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs)
+// The pair query of psph_neighbour_pairs / psph_neighbour_pairs_noalloc.  staging: device
+// memory for 2*staging_pairs values to stage the list in, or null to allocate it for the call.
+static int pair_query(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs,
+                      long long *staging, size_t staging_pairs)
 {
     if (!h || !npairs) return fail(PSPH_ERR_INVALID, "null argument");
     if (!h->uploaded) return fail(PSPH_ERR_STATE, "pairs before upload");
@@ -1506,14 +1547,17 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     long long tot = 0;
     for (int i = 0; i < P.n; ++i) { off[i] = tot; tot += cnt[i]; }
     *npairs = (size_t)tot;
-    long long *out_dev = nullptr;
-    if (pairs && (size_t)tot <= cap && tot > 0) {
-        HIPCHECK(hipMemcpy(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice));
-        HIPCHECK(hipMalloc((void **)&out_dev, sizeof(long long)*2*tot));
+    const bool fits_staging = !staging || (size_t)tot <= staging_pairs;
+    if (pairs && (size_t)tot <= cap && tot > 0 && fits_staging) {
+        long long *out_dev = staging;
+        HIPCHECK(hipMemcpyAsync(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice, h->stream));
+        if (!staging) HIPCHECK(hipMalloc((void **)&out_dev, sizeof(long long)*2*tot));
+        // k_pairs writes every one of the 2*tot values: nothing left from an earlier call is read
         launch_pairs(D, P, h->stream, 1, h->pair_off, nullptr, out_dev);
-        HIPCHECK(hipStreamSynchronize(h->stream));
-        HIPCHECK(hipMemcpy(pairs, out_dev, sizeof(long long)*2*tot, hipMemcpyDeviceToHost));
-        hipFree(out_dev);
+        hipError_t e = hipMemcpyAsync(pairs, out_dev, sizeof(long long)*2*tot, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (!staging) hipFree(out_dev);
+        if (e != hipSuccess) return fail(PSPH_ERR_HIP, "pair copy: %s", hipGetErrorString(e));
     }
     // restore the control block; the sorted copy is now the state, and the lists are stale
     rc = sync_ctl(h);
@@ -1528,7 +1572,23 @@ int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npa
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->trans_ok = false;
     if (pairs && (size_t)tot > cap) return fail(PSPH_ERR_CAPACITY, "need room for %lld pairs", tot);
+    if (pairs && !fits_staging)
+        return fail(PSPH_ERR_CAPACITY, "%lld pairs exceed the %zu that the no-allocation pair query "
+                    "stages (a %zu MiB device array); use psph_neighbour_pairs", tot, staging_pairs,
+                    kScratchBytes >> 20);
     return 0;
+}
+
+int psph_neighbour_pairs(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs)
+{
+    return pair_query(h, pairs, cap, npairs, nullptr, 0);
+}
+
+int psph_neighbour_pairs_noalloc(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (!h->pair_scratch) return fail(PSPH_ERR_HIP, "%s", h->pair_scratch_err.c_str());
+    return pair_query(h, pairs, cap, npairs, h->pair_scratch, kScratchPairs);
 }
 
 int psph_get_grid(psph_handle *h, psph_grid *g)

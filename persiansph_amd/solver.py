@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("PSPH_LIB") or os.path.join(_HERE, "lib", "libpersians
 
 EXPORTS = [
     "psph_create", "psph_upload", "psph_run", "psph_download", "psph_neighbour_pairs",
-    "psph_get_grid", "psph_set_timing", "psph_kernel_stats", "psph_stream",
+    "psph_neighbour_pairs_noalloc", "psph_get_grid", "psph_set_timing", "psph_kernel_stats", "psph_stream",
     "psph_traffic_model", "psph_destroy", "psph_last_error", "psph_version",
     "psph_output_begin", "psph_output_end", "psph_set_body_force",
 ]
@@ -38,6 +38,7 @@ def lib():
         L.psph_run.argtypes = [p, i64, d, d, ctypes.POINTER(Status)]
         L.psph_download.argtypes = [p, p, sz, ctypes.POINTER(sz)]
         L.psph_neighbour_pairs.argtypes = [p, p, sz, ctypes.POINTER(sz)]
+        L.psph_neighbour_pairs_noalloc.argtypes = [p, p, sz, ctypes.POINTER(sz)]
         L.psph_get_grid.argtypes = [p, ctypes.POINTER(Grid)]
         L.psph_set_timing.argtypes = [p, ctypes.c_int]
         L.psph_kernel_stats.argtypes = [p, ctypes.c_char_p, sz, p, p, sz, ctypes.POINTER(sz)]
@@ -127,11 +128,14 @@ class Solver:
         self.output_begin()
         return self.output_end()
 
-    def pairs(self):
+    def pairs(self, noalloc=False):
+        """The oriented pair set of the current positions (psph_neighbour_pairs, or with
+        noalloc=True psph_neighbour_pairs_noalloc, which allocates no device memory)."""
+        f = lib().psph_neighbour_pairs_noalloc if noalloc else lib().psph_neighbour_pairs
         n = ctypes.c_size_t()
-        _check(lib().psph_neighbour_pairs(self._h, None, 0, ctypes.byref(n)))
+        _check(f(self._h, None, 0, ctypes.byref(n)))
         out = np.zeros((max(n.value, 1), 2), dtype=np.int64)
-        _check(lib().psph_neighbour_pairs(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        _check(f(self._h, _ptr(out), n.value, ctypes.byref(n)))
         return out[:n.value]
 
     def grid(self):
