@@ -205,8 +205,12 @@ int psph_neighbour_pairs_noalloc(psph_handle *h, int64_t *pairs, size_t cap, siz
 /* The grid computed at upload. */
 int psph_get_grid(psph_handle *h, psph_grid *g);
 
-/* Profiling: when enabled, every kernel launch is bracketed by HIP events on the solver's
- * stream; psph_kernel_stats returns per-kernel launch counts and mean durations (ms). */
+/* Profiling: with PSPH_TIMING_ALL every kernel launch is bracketed by HIP events on the solver's
+ * stream, with PSPH_TIMING_FORCE only the force pass (two events per step instead of ~20, for
+ * timed runs); psph_kernel_stats returns per-kernel launch counts and mean durations (ms). */
+#define PSPH_TIMING_OFF   0
+#define PSPH_TIMING_ALL   1
+#define PSPH_TIMING_FORCE 2
 int psph_set_timing(psph_handle *h, int enable);
 int psph_kernel_stats(psph_handle *h, char *names, size_t names_len,
                       double *mean_ms, int64_t *launches, size_t cap, size_t *nkernels);
@@ -214,8 +218,9 @@ int psph_kernel_stats(psph_handle *h, char *names, size_t names_len,
 /* The HIP stream the solver launches on (hipStream_t as an opaque pointer). */
 int psph_stream(psph_handle *h, void **stream);
 
-/* Bytes of algorithmic (compulsory) HBM traffic per step for the current particle set,
- * split per kernel class, as used for roofline figures. */
+/* Bytes of algorithmic (compulsory) HBM traffic per step for the current particle set and
+ * precision tier (FP32-mixed: FP32 neighbour records), and of one force pass, as used for
+ * roofline figures: every array a kernel needs is read once and written once. */
 int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_bytes);
 
 int psph_destroy(psph_handle *h);

@@ -135,6 +135,7 @@ struct psph_handle {
     int32_t next_index = 0;
     // timing
     bool timing = false;
+    bool timing_force_only = false;   // events around the force pass only (PSPH_TIMING_FORCE)
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
     double t_ms[KCount] = {0};
@@ -275,7 +276,7 @@ static hipEvent_t get_event(psph_handle *h)
 template <class F>
 static void timed(psph_handle *h, int id, F &&f)
 {
-    if (!h->timing) { f(); return; }
+    if (!h->timing || (h->timing_force_only && id != KForce)) { f(); return; }
     hipEvent_t a = get_event(h), b = get_event(h);
     (void)hipEventRecord(a, h->stream);
     f();
@@ -1602,7 +1603,9 @@ int psph_get_grid(psph_handle *h, psph_grid *g)
 int psph_set_timing(psph_handle *h, int enable)
 {
     if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (enable < 0 || enable > PSPH_TIMING_FORCE) return fail(PSPH_ERR_INVALID, "timing mode %d", enable);
     h->timing = enable != 0;
+    h->timing_force_only = enable == PSPH_TIMING_FORCE;
     for (int k = 0; k < KCount; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
     return 0;
 }
@@ -1639,11 +1642,25 @@ int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_byt
 {
     if (!h || !h->uploaded) return fail(PSPH_ERR_STATE, "traffic model before upload");
     double N = h->P.n, Nf = h->P.nfree, Nx = h->P.nfixed, C = h->P.ncells;
-    // force gather: read x, v, rho, P, m, flags of every particle once (80 B) and NSv of the
-    // walls (24 B); write a and dDensity of the free particles (32 B)
-    double force = 80.0*N + 24.0*Nx + 32.0*Nf;
-    double wall = 64.0*N + 32.0*Nx;
-    double total = 32.0*N + 16.0*N + 8.0*C + 224.0*N + wall + force + 192.0*Nf;
+    double force, wall, views;
+    if (h->P.precision == PSPH_FP32_MIXED) {
+        // FP32-mixed: the gather reads FP32 x, v (24 B), rho, P, m (12 B) and the flags (4 B) of
+        // every particle once and the FP32 NSv of the walls (12 B), and writes a and dDensity
+        // of the free particles in FP64 (32 B); the wall pass reads the FP32 x, v, rho, P
+        // (32 B); the reorder writes those FP32 copies (40 B) besides the FP64 state
+        force = 40.0*N + 12.0*Nx + 32.0*Nf;
+        wall = 32.0*N + 32.0*Nx;
+        views = 40.0*N;
+    } else {
+        // FP64: the gather reads x, v, rho, P, m, flags of every particle once (80 B) and NSv
+        // of the walls (24 B), and writes a and dDensity of the free particles (32 B)
+        force = 80.0*N + 24.0*Nx + 32.0*Nf;
+        wall = 64.0*N + 32.0*Nx;
+        views = 0.0;
+    }
+    // keys 32 B, sort 16 B, cell table 8 B per cell, reorder of the persistent state 2 x 112 B,
+    // integration of the free particles 192 B
+    double total = 32.0*N + 16.0*N + 8.0*C + 224.0*N + views + wall + force + 192.0*Nf;
     if (bytes_per_step) *bytes_per_step = total;
     if (force_bytes) *force_bytes = force;
     return 0;

@@ -143,8 +143,9 @@ class Solver:
         _check(lib().psph_get_grid(self._h, ctypes.byref(g)))
         return g
 
-    def set_timing(self, on):
-        _check(lib().psph_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on, force_only=False):
+        """Per-kernel HIP events on the solver's stream (force_only: around the force pass only)."""
+        _check(lib().psph_set_timing(self._h, (2 if force_only else 1) if on else 0))
 
     def kernel_stats(self):
         names = ctypes.create_string_buffer(512)
