@@ -43,6 +43,13 @@ extern "C" {
 #define PSPH_STOP_OUTPUT  2  /* Time >= tout after the force pass (Domain.cpp:1773): the state is
                                 mid-step (forces of this step present, Move pending); the next
                                 psph_run call finishes the step first */
+#define PSPH_STOP_HOOK_BEFORE 3  /* host hook point of GeneralBefore (Domain.cpp:1767), enabled by
+                                psph_set_hooks: after StartAcceleration, the in/out-flow
+                                imposition and the neighbour search, before the wall smoothing
+                                and the pair sums (mid_step = 2) */
+#define PSPH_STOP_HOOK_AFTER  4  /* host hook point of GeneralAfter (Domain.cpp:1770): after the
+                                pair sums, before AdaptiveTimeStep and Move (mid_step = 1); at an
+                                output time the stop is PSPH_STOP_OUTPUT instead, at the same point */
 
 /* Solver-wide settings: the Domain fields read inside Solve (Domain.h:89-164). */
 typedef struct psph_config {
@@ -116,7 +123,8 @@ typedef struct psph_status {
     int64_t deleted;         /* particles removed by the leave check during this call */
     size_t  n;               /* particle count after the call */
     int32_t stop_reason;     /* PSPH_STOP_* */
-    int32_t mid_step;        /* 1 if stopped after the force pass with Move pending */
+    int32_t mid_step;        /* 1 if stopped after the force pass with Move pending, 2 if stopped
+                                before the wall smoothing (PSPH_STOP_HOOK_BEFORE) */
     int64_t list_builds;     /* steps of this call that sorted the particles and built the
                                 neighbour lists (all of them unless the lists are reused) */
 } psph_status;
@@ -156,8 +164,29 @@ int psph_download(psph_handle *h, psph_particle *p, size_t cap, size_t *n);
  * adds a constant to the acceleration of every free particle after StartAcceleration and
  * before the pair sums, Domain.cpp:1766-1770): from the next force pass on, every free
  * particle's acceleration starts at Gravity + accel.  Kept across psph_upload; zero by default.
- * Arbitrary GeneralBefore/GeneralAfter hooks need host state mid-step and are not offered. */
+ * Arbitrary GeneralBefore/GeneralAfter hooks take the host-synchronised path (psph_set_hooks). */
 int psph_set_body_force(psph_handle *h, const double accel[3]);
+
+/* Host-synchronised user hooks: arbitrary GeneralBefore / GeneralAfter functions of the reference
+ * (Domain.h:133-134, called at Domain.cpp:1767 and :1770 of every step), the slow path beside the
+ * device form psph_set_body_force.  With before (after) nonzero every step stops at
+ * PSPH_STOP_HOOK_BEFORE (PSPH_STOP_HOOK_AFTER); the caller downloads the state (psph_download:
+ * at the stop before the wall pass it shows what StartAcceleration leaves: a = Gravity, or
+ * Gravity + the body force for free particles, the other sums zero, fixed particles' Pressure,
+ * NSv, Sigma and ShearStress zero), runs its hook, writes back what the hook changed with
+ * psph_hook_accel and calls psph_run again, which finishes the step.  Kept across psph_upload;
+ * off by default.  PSPH_ERR_STATE while a step is pending. */
+int psph_set_hooks(psph_handle *h, int before, int after);
+
+/* At a stop inside a step (PSPH_STOP_HOOK_BEFORE, PSPH_STOP_HOOK_AFTER or PSPH_STOP_OUTPUT):
+ * accelerations a (n x 3) and, if not NULL, density rates ddensity (n) of every particle, in the
+ * order of psph_download.  Before the wall pass they are the values the step's pair sums start
+ * from (the reference's a and dDensity after GeneralBefore, Domain.cpp:1767-1769; without
+ * ddensity the sums start from zero); after the pair sums they replace the sums Move uses
+ * (GeneralAfter, :1770).  Only free particles' values are used: fixed particles' a and dDensity
+ * feed nothing.  Other fields a hook changes are not taken back; the drop-in refuses hooks that
+ * change them.  n must be the current particle count. */
+int psph_hook_accel(psph_handle *h, const double *a, const double *ddensity, size_t n);
 
 /* The fields Domain::WriteXDMF (Domain.cpp:1869-1964) converts to float for its HDF5 file, in
  * reference particle order, laid out as its datasets are: Position, Velocity, Acceleration

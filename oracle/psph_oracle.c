@@ -713,6 +713,8 @@ struct psph_odomain {
     int64_t last_pairs;
     double DomSize[3];          /* periodic length per direction, 0 where not periodic */
     double body[3];             /* body-force hook: added to every free particle's a */
+    psph_o_hook hook_before, hook_after;   /* GeneralBefore / GeneralAfter */
+    void *hook_user;
     /* in/out-flow boundaries: Boundary::InPart, InFlowLoc1-3, OutFlowLoc, inoutcounter */
     int *inpart;
     size_t ninpart, capinpart;
@@ -1916,6 +1918,27 @@ void psph_o_set_body_force(psph_odomain *d, const double accel[3])
     for (int c = 0; c < 3; ++c) d->body[c] = accel[c];
 }
 
+void psph_o_set_hooks(psph_odomain *d, psph_o_hook before, psph_o_hook after, void *user)
+{
+    d->hook_before = before;
+    d->hook_after = after;
+    d->hook_user = user;
+}
+
+/* A user hook on the particles: GeneralBefore(*this) / GeneralAfter(*this) */
+static int call_hook(psph_odomain *d, psph_o_hook f)
+{
+    if (!f) return 0;
+    psph_particle *tmp = (psph_particle*)malloc(d->n*sizeof *tmp);
+    if (!tmp && d->n) return set_err(PSPH_ERR_INVALID, "out of memory for a hook call");
+    for (size_t i = 0; i < d->n; i++) tmp[i] = d->P[i]->p;
+    f(tmp, d->n, d->hook_user);
+    for (size_t i = 0; i < d->n; i++) d->P[i]->p = tmp[i];
+    free(tmp);
+    return 0;
+}
+
+/* Solve's loop body up to the output check (Domain.cpp:1764-1771) */
 static int force_pass(psph_odomain *d)
 {
     start_acceleration(d);
@@ -1925,8 +1948,12 @@ static int force_pass(psph_odomain *d)
     }
     main_neighbour_search(d);
     body_force_hook(d);
+    int rc = call_hook(d, d->hook_before);
+    if (rc) return rc;
     primary_compute_acceleration(d);
-    return last_compute_acceleration(d);
+    rc = last_compute_acceleration(d);
+    if (rc) return rc;
+    return call_hook(d, d->hook_after);
 }
 
 static int finish_step(psph_odomain *d, psph_status *st)

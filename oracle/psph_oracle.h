@@ -57,6 +57,11 @@ int  psph_o_grid(psph_odomain *d, psph_grid *g);
 int  psph_o_pairs(psph_odomain *d, int64_t *pairs, size_t cap, size_t *npairs);
 /* body-force hook: accel is added to every free particle's a after StartAcceleration */
 void psph_o_set_body_force(psph_odomain *d, const double accel[3]);
+/* User hooks GeneralBefore / GeneralAfter (Domain.cpp:1767, 1770): called with the particles
+ * (reference order, a contiguous copy that is copied back afterwards, so a hook may change any
+ * field), after the neighbour search and after the pair sums of every step.  NULL: none. */
+typedef void (*psph_o_hook)(psph_particle *p, size_t n, void *user);
+void psph_o_set_hooks(psph_odomain *d, psph_o_hook before, psph_o_hook after, void *user);
 /* Per-particle |contribution| sums of the last force pass, 32 doubles per particle:
  * a[3], dDensity, Pressure(wall), NSv[3], SumKernel, StrainRate[6], RotationRate[3], (pad),
  * VXSPH[3], ZWab, SumDen, FSIPressure, FSINSv[3], S, (pad x3) -- the wall and FSI

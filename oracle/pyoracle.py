@@ -17,6 +17,9 @@ LIB_PATH = os.environ.get("PSPH_ORACLE_LIB") or os.path.join(_HERE, "libpsph_ora
 
 _lib = None
 
+# void (*)(psph_particle *p, size_t n, void *user)
+HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p)
+
 
 def lib():
     global _lib
@@ -57,6 +60,8 @@ def lib():
         L.psph_o_scales.argtypes = [p, p, ctypes.c_size_t]
         L.psph_o_set_body_force.restype = None
         L.psph_o_set_body_force.argtypes = [p, p]
+        L.psph_o_set_hooks.restype = None
+        L.psph_o_set_hooks.argtypes = [p, HOOK, HOOK, p]
         L.psph_o_destroy.restype = None
         L.psph_o_destroy.argtypes = [p]
         L.psph_o_last_error.restype = ctypes.c_char_p
@@ -106,6 +111,19 @@ class Oracle:
     def set_body_force(self, accel):
         a = np.ascontiguousarray(accel, dtype=np.float64)
         lib().psph_o_set_body_force(self._h, _ptr(a))
+
+    def set_hooks(self, before=None, after=None):
+        """GeneralBefore / GeneralAfter as Python functions of a structured array of the
+        particles (reference order), which they may change in place."""
+        def wrap(f):
+            if f is None:
+                return HOOK()
+            def cb(ptr, n, user):
+                buf = (ctypes.c_char*(n*PARTICLE_DTYPE.itemsize)).from_address(ptr)
+                f(np.frombuffer(buf, dtype=PARTICLE_DTYPE))
+            return HOOK(cb)
+        self._hooks = (wrap(before), wrap(after))      # kept alive with the oracle
+        lib().psph_o_set_hooks(self._h, self._hooks[0], self._hooks[1], None)
 
     def grid(self):
         g = Grid()

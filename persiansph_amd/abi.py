@@ -47,7 +47,7 @@ assert PARTICLE_DTYPE.itemsize == 174 * 8 + 20 * 4, PARTICLE_DTYPE.itemsize
 FP64 = 0
 FP32_MIXED = 1
 
-STOP_STEPS, STOP_TEND, STOP_OUTPUT = 0, 1, 2
+STOP_STEPS, STOP_TEND, STOP_OUTPUT, STOP_HOOK_BEFORE, STOP_HOOK_AFTER = 0, 1, 2, 3, 4
 
 ERRORS = {
     0: "PSPH_OK", -1: "PSPH_ERR_INVALID", -2: "PSPH_ERR_UNSUPPORTED", -3: "PSPH_ERR_HIP",

@@ -15,15 +15,19 @@
 // on the GPU (psph_output_begin/end) and written by a background thread while the next steps
 // run; a user UserOutput gets the full state downloaded into Particles first, as it reads them.
 //
-// User hooks: GeneralBefore may be SPH::BodyForceHook (a constant acceleration, Domain::BodyForce,
-// on every free particle, as 1-Poiseuille.cpp's UserAcc), which Solve applies on the GPU.
+// User hooks: GeneralBefore = SPH::BodyForceHook (a constant acceleration, Domain::BodyForce, on
+// every free particle, as 1-Poiseuille.cpp's UserAcc) runs on the GPU.  Any other GeneralBefore
+// or GeneralAfter runs on the host at its point of every step (Domain.cpp:1767, 1770): Solve
+// downloads the state into Particles, calls the hook and takes back the accelerations and
+// density rates it leaves (psph_set_hooks / psph_hook_accel) -- a slow path, a full download
+// per hook call.
 //
 // In/out-flow boundaries (BC.InOutFlow with the default InCon/OutCon) run on the GPU; particles
 // the recycling creates are appended to Particles at the next download.
 //
 // Not available on the GPU path (Solve throws Fatal): in/out-flow boundaries with particles other
-// than fluid or with the Shepard filter, and other GeneralBefore/GeneralAfter hooks than the
-// default no-op and the body-force hook.
+// than fluid, with the Shepard filter or with user InCon/OutCon functions; hooks that change
+// fields other than a and dDensity, or the particle set.
 #ifndef PERSIANSPH_AMD_DOMAIN_H
 #define PERSIANSPH_AMD_DOMAIN_H
 
@@ -230,7 +234,10 @@ inline void BodyForceHook(Domain &dom);
 // default UserOutput (Domain.cpp:28-33)
 inline void OutPut(Particle *, double &Prop1, double &Prop2, double &Prop3) { Prop1 = 0.0; Prop2 = 0.0; Prop3 = 0.0; }
 
-// default AllCon (Domain.cpp:47-51)
+// default InCon, OutCon, AllCon (Domain.cpp:35-51): the constant velocity and density of the
+// boundary; the GPU imposes the InCon/OutCon defaults itself (a user InCon/OutCon is refused)
+inline void InFlowCon(Vec3_t &, Vec3_t &Vel, double &Den, Boundary &bdry) { Vel = bdry.inv; Den = bdry.inDensity; }
+inline void OutFlowCon(Vec3_t &, Vec3_t &Vel, double &Den, Boundary &bdry) { Vel = bdry.outv; Den = bdry.outDensity; }
 inline void AllFlowCon(Vec3_t &, Vec3_t &Vel, double &Den, Boundary &bdry) { Vel = bdry.allv; Den = bdry.allDensity; }
 
 class Domain
@@ -260,6 +267,8 @@ public:
     PtDom GeneralAfter = &General;
     PtOut UserOutput = &OutPut;
     Vec3_t BodyForce;                    // acceleration added by BodyForceHook (this build's addition)
+    PtVel InCon = &InFlowCon;
+    PtVel OutCon = &OutFlowCon;
     PtVel AllCon = &AllFlowCon;
     std::string OutputName[3] = {"Property1", "Property2", "Property3"};
     size_t Scheme = 0;
@@ -408,11 +417,15 @@ public:
     void Solve(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx)
     {
         std::cout << "\n--------------Solving---------------------------------------------------------------" << std::endl;
+        // hooks: SPH::BodyForceHook runs on the GPU; any other GeneralBefore/GeneralAfter runs on
+        // the host at its point of every step (Domain.cpp:1767, 1770), with the state downloaded
+        // into Particles and the accelerations and density rates it leaves taken back
         const bool body_hook = GeneralBefore == &BodyForceHook;
-        if ((GeneralBefore != &General && !body_hook) || GeneralAfter != &General)
-            throw new Fatal("Domain::Solve: user hooks GeneralBefore/GeneralAfter are not available on the GPU path "
-                            "(a constant body force on the free particles: GeneralBefore = &SPH::BodyForceHook "
-                            "with Domain::BodyForce)");
+        const bool hook_before = GeneralBefore != &General && !body_hook;
+        const bool hook_after = GeneralAfter != &General;
+        if (BC.InOutFlow > 0 && (InCon != &InFlowCon || OutCon != &OutFlowCon))
+            throw new Fatal("Domain::Solve: user InCon/OutCon functions are not available on the GPU path "
+                            "(it imposes the constant BC.inv/inDensity and BC.outv/outDensity of the defaults)");
         if (Particles.Size() == 0) throw new Fatal("Domain::Solve: no particles");
         psph_config cfg;
         std::memset(&cfg, 0, sizeof cfg);
@@ -445,6 +458,7 @@ public:
             double b[3] = {BodyForce(0), BodyForce(1), BodyForce(2)};
             check(psph_set_body_force(h, b));
         }
+        if (hook_before || hook_after) check(psph_set_hooks(h, hook_before ? 1 : 0, hook_after ? 1 : 0));
         psph_grid g;
         check(psph_get_grid(h, &g));
         for (int c = 0; c < 3; ++c) { TRPR(c) = g.trpr[c]; BLPF(c) = g.blpf[c]; CellSize(c) = g.cell_size[c]; CellNo[c] = g.cell_no[c]; }
@@ -464,7 +478,10 @@ public:
             if (!(Time < tf && idx_out <= maxidx)) break;
             check(psph_run(h, INT64_MAX, tf, tout, &st));
             Time = st.time; deltat = st.deltat; deltatmin = st.deltatmin;
+            if (st.stop_reason == PSPH_STOP_HOOK_BEFORE) { run_hook(h, GeneralBefore, "GeneralBefore"); continue; }
+            if (st.stop_reason == PSPH_STOP_HOOK_AFTER) { run_hook(h, GeneralAfter, "GeneralAfter"); continue; }
             if (st.stop_reason != PSPH_STOP_OUTPUT) break;
+            if (hook_after) run_hook(h, GeneralAfter, "GeneralAfter");     // before the output (:1770-1773)
             if (TheFileKey != NULL) {
                 char fn[4096];
                 std::snprintf(fn, sizeof fn, "%s_%04zu", TheFileKey, idx_out);
@@ -715,6 +732,54 @@ private:
         for (size_t i = 0; i < up_.size(); ++i)
             if (!keep[i] && alive_[i]) { delete up_[i]; alive_[i] = 0; }
         Particles.swap(next);
+    }
+
+    // A user hook at its point inside a step: the state is downloaded into Particles, the hook
+    // runs on them, and the accelerations and density rates it leaves go back to the device
+    // (psph_hook_accel).  A hook that changes any other field, or the particle set, is refused:
+    // the GPU path does not take those back in the middle of a step.
+    void run_hook(psph_handle *h, PtDom hook, const char *name)
+    {
+        download(h);
+        const size_t n = Particles.Size();
+        std::vector<psph_particle> pre(n), post(n);
+        std::vector<Particle *> ptr(n);
+        for (size_t k = 0; k < n; ++k) { to_abi(*Particles[k], pre[k]); ptr[k] = Particles[k]; }
+        hook(*this);
+        if (Particles.Size() != n)
+            throw new Fatal("Domain::Solve: the %s hook added or removed particles, which the GPU path does "
+                            "not take back in the middle of a step", name);
+        std::vector<double> a(3*n), dd(n);
+        for (size_t k = 0; k < n; ++k) {
+            if (Particles[k] != ptr[k])
+                throw new Fatal("Domain::Solve: the %s hook reordered Particles, which the GPU path does not "
+                                "take back in the middle of a step", name);
+            to_abi(*Particles[k], post[k]);
+            const char *f = changed_field(pre[k], post[k]);
+            if (f)
+                throw new Fatal("Domain::Solve: the %s hook changed %s of particle %zu; on the GPU path a hook may "
+                                "change only a and dDensity", name, f, k);
+            for (int c = 0; c < 3; ++c) a[3*k + c] = post[k].a[c];
+            dd[k] = post[k].dDensity;
+        }
+        check(psph_hook_accel(h, a.data(), dd.data(), n));
+    }
+
+    // The first field other than a and dDensity in which two particle records differ, or null.
+    static const char *changed_field(const psph_particle &x, const psph_particle &y)
+    {
+#define S_(f) if (std::strcmp(#f, "dDensity") != 0 && std::memcmp(&x.f, &y.f, sizeof x.f) != 0) return #f;
+#define V_(f) if (std::strcmp(#f, "a") != 0 && std::memcmp(x.f, y.f, sizeof x.f) != 0) return #f;
+#define M_(f) if (std::memcmp(x.f, y.f, sizeof x.f) != 0) return #f;
+#define I_(f) if (x.f != y.f) return #f;
+        PSPH_FIELDS_SCALAR(S_) PSPH_FIELDS_VEC(V_) PSPH_FIELDS_MAT(M_)
+        I_(IsFree) I_(NoSlip) I_(Shepard) I_(LES) I_(Material) I_(PresEq) I_(VisM) I_(Fail) I_(ID) I_(ct)
+        I_(FirstStep) I_(InOut) I_(ShepardCounter) I_(ShepardStep) I_(SeepageType) I_(IsSat) I_(SatCheck)
+#undef S_
+#undef V_
+#undef M_
+#undef I_
+        return nullptr;
     }
 
     // One output inside Solve.  With the default UserOutput the fields are packed on the GPU and

@@ -1,10 +1,11 @@
 // Plane Poiseuille flow through the SPH::Domain drop-in, set up as the reference's
 // 1-Poiseuille.cpp:36-80 (periodic in x, walls |y| >= 20 dx, quintic spline, Takeda viscosity,
 // modified Verlet).  The reference drives the flow with GeneralBefore = UserAcc (a += 0.002 x
-// on free particles); here the same hook is SPH::BodyForceHook with BodyForce = (0.002, 0, 0),
-// which Solve applies on the GPU.
+// on free particles).  With hook "body" (the default) the same hook is SPH::BodyForceHook with
+// BodyForce = (0.002, 0, 0), which Solve applies on the GPU; with hook "user" it is the
+// reference's UserAcc itself, which Solve runs on the host at its point of every step.
 //
-//   poiseuille [tf [dtOut [filekey [maxidx [out.bin]]]]]
+//   poiseuille [tf [dtOut [filekey [maxidx [out.bin [hook]]]]]]
 // defaults: tf = 20 s, dtOut = 0.05 s, filekey "test06", maxidx 250 (the reference's call).
 // out.bin receives n followed by x, y, z, vx of every particle (FP64).  filekey "-" writes no
 // output files.
@@ -18,13 +19,22 @@
 using std::cout;
 using std::endl;
 
+// the GeneralBefore hook of 1-Poiseuille.cpp:22-28
+void UserAcc(SPH::Domain &domi)
+{
+    for (size_t i = 0; i < domi.Particles.Size(); i++)
+        if (domi.Particles[i]->IsFree)
+            domi.Particles[i]->a += Vec3_t(0.002, 0.0, 0.0);
+}
+
 int main(int argc, char **argv) try
 {
     double tf = argc > 1 ? std::atof(argv[1]) : 20.0;
     double dtout = argc > 2 ? std::atof(argv[2]) : 0.05;
     const char *key = (argc > 3 && std::string(argv[3]) != "-") ? argv[3] : (argc > 3 ? nullptr : "test06");
     size_t maxidx = argc > 4 ? (size_t)std::atol(argv[4]) : 250;
-    const char *out = argc > 5 ? argv[5] : nullptr;
+    const char *out = (argc > 5 && std::string(argv[5]) != "-") ? argv[5] : nullptr;
+    const bool user_hook = argc > 6 && std::string(argv[6]) == "user";
 
     SPH::Domain dom;
     dom.Dimension = 2;
@@ -46,8 +56,12 @@ int main(int argc, char **argv) try
     cout << "Rho = " << Rho << endl;
     cout << "Mu  = " << Mu << endl;
 
-    dom.BodyForce = 0.002, 0.0, 0.0;
-    dom.GeneralBefore = &SPH::BodyForceHook;
+    if (user_hook) {
+        dom.GeneralBefore = &UserAcc;
+    } else {
+        dom.BodyForce = 0.002, 0.0, 0.0;
+        dom.GeneralBefore = &SPH::BodyForceHook;
+    }
     dom.InitialDist = dx;
 
     dom.AddBoxLength(1, Vec3_t(0.0, -24.0*dx, 0.0), 20.0*dx + dx/10.0, 48.0*dx + dx/10.0, 0, dx/2.0, Rho, h, 1, 0, false, false);

@@ -30,7 +30,8 @@ void launch_scan(const Dev &D, const Params &P, hipStream_t s);
 void launch_scatter(const Dev &D, const Params &P, hipStream_t s);
 void launch_cellsort(const Dev &D, const Params &P, hipStream_t s);
 void launch_reorder(const Dev &D, const Params &P, hipStream_t s);
-void launch_wall(const Dev &D, const Params &P, hipStream_t s);
+void launch_wall(const Dev &D, const Params &P, hipStream_t s, int after_hook);
+void launch_hook_accel(const Dev &D, const Params &P, const double *a, const double *dd, hipStream_t s);
 void launch_force(const Dev &D, const Params &P, hipStream_t s);
 void launch_integrate(const Dev &D, const Params &P, hipStream_t s);
 void launch_tail(const Dev &D, const Params &P, hipStream_t s);
@@ -127,6 +128,10 @@ struct psph_handle {
     psph_grid grid{};
     bool trans_ok = false;             // transient columns match the current order
     double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
+    bool hook_b = false, hook_a = false;   // host hook stops (psph_set_hooks)
+    int regrowths = 0;                 // device buffers re-created for more particles (in/out-flow)
+    double *hook_dev = nullptr;        // staging of psph_hook_accel's values (4 doubles per particle)
+    size_t hook_cap = 0;
     // in/out-flow: Boundary::InPart (host indices, the reference's order), the zone limits,
     // inoutcounter; the next upload index for particles the recycling creates
     std::vector<int32_t> inpart;
@@ -651,6 +656,12 @@ static int read_state(psph_handle *h)
         if (P.any_les_t0 && rd(shr, D.shr, 2)) return PSPH_ERR_HIP;
     }
     bool shep_on = P.any_shepard;
+    // stopped before the wall pass (GeneralBefore hook): the sums are those StartAcceleration
+    // leaves (Domain.cpp:947-1017), and a free particle's a is where its pair sums will start
+    const bool before_walls = h->ctl_host->mid_step == 2;
+    const bool acc_init = h->ctl_host->acc_init != 0;
+    if (before_walls && acc_init && !have_transients)
+        HIPCHECK(hipMemcpy(acc.data(), D.acc, sizeof(double4)*n, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) {
         int o = orig[i];
         if (o < 0 || (size_t)o >= h->host.size()) return fail(PSPH_ERR_STATE, "corrupt particle index %d", o);
@@ -712,6 +723,21 @@ static int read_state(psph_handle *h)
             q.FirstStep = h->ctl_host->first_step;
             if (P.scheme == 0) { q.ct = h->ctl_host->ct; q.ShepardCounter = h->ctl_host->shep_counter; }
         }
+        if (before_walls) {
+            for (int c = 0; c < 3; ++c) {
+                q.a[c] = fr ? (acc_init ? (&acc[i].x)[c] : P.afree[c]) : h->cfg.gravity[c];
+                q.VXSPH[c] = 0.0; q.FSINSv[c] = 0.0;
+            }
+            q.dDensity = (fr && acc_init) ? acc[i].w : 0.0;
+            q.ZWab = 0.0; q.SumDen = 0.0; q.SumKernel = 0.0; q.S = 0.0;
+            q.FSISumKernel = 0.0; q.FSIPressure = 0.0; q.SatCheck = 0;
+            for (int k = 0; k < 9; ++k) { q.StrainRate[k] = 0.0; q.RotationRate[k] = 0.0; }
+            if (!fr) {
+                q.Pressure = 0.0;
+                for (int c = 0; c < 3; ++c) q.NSv[c] = 0.0;
+                for (int k = 0; k < 9; ++k) { q.Sigma[k] = 0.0; q.ShearStress[k] = 0.0; }
+            }
+        }
     }
     return 0;
 }
@@ -719,11 +745,13 @@ static int read_state(psph_handle *h)
 // ------------------------------------------------------------------------------------------
 // step queueing
 // ------------------------------------------------------------------------------------------
-static void enqueue_step(psph_handle *h, bool finish_only)
+// Queues one step.  from = 0: a whole step; 1: the rest of a step stopped before the wall pass
+// (GeneralBefore hook stop); 2: the rest of a step stopped after the force pass (Move).
+static void enqueue_step(psph_handle *h, int from)
 {
     const Params &P = h->P;
     const Dev &D = h->D;
-    if (!finish_only) {
+    if (from == 0) {
         timed(h, KScan, [&] {
             if (P.verlet) launch_keys(D, P, h->stream);      // only on steps that rebuild
             launch_scan(D, P, h->stream);
@@ -731,7 +759,9 @@ static void enqueue_step(psph_handle *h, bool finish_only)
         timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
         timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
         timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
-        timed(h, KWall, [&] { launch_wall(D, P, h->stream); });
+    }
+    if (from <= 1) {
+        timed(h, KWall, [&] { launch_wall(D, P, h->stream, from); });
         timed(h, KForce, [&] { launch_force(D, P, h->stream); });
     }
     timed(h, KIntegrate, [&] { launch_integrate(D, P, h->stream); });
@@ -869,6 +899,32 @@ static void inflow_copy(psph_particle &a, const psph_particle &b, const double *
 
 static int write_back(psph_handle *h);
 
+// Re-creates every device buffer for `cap` particles, keeping the run's settings, list
+// capacities, class table and control block; the state itself is on the host (read back) and
+// the caller writes it (write_back).  The pinned buffers stay: an output's host copy may still
+// be in use by the caller.
+static int grow_buffers(psph_handle *h, size_t cap)
+{
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    const Ctl saved = *h->ctl_host;
+    const int nl_cap = h->P.nl_cap, wl_cap = h->P.wl_cap;
+    for (void *p : h->allocs) if (p) HIPCHECK(hipFree(p));
+    h->allocs.clear();
+    h->nl = h->wl = nullptr;
+    h->out_dev = nullptr;
+    h->hook_dev = nullptr;
+    h->hook_cap = 0;
+    int rc = allocate(h, cap);
+    if (rc) return rc;
+    h->P.nl_cap = 0;
+    h->P.wl_cap = 0;
+    if (nl_cap > 0 && (rc = nl_alloc(h, nl_cap))) return rc;
+    if (wl_cap > 0 && (rc = wl_alloc(h, wl_cap))) return rc;
+    HIPCHECK(hipMemcpy(h->cls_dev, h->classes.data(), sizeof(ClassParams)*h->classes.size(), hipMemcpyHostToDevice));
+    *h->ctl_host = saved;
+    return 0;
+}
+
 // InFlowBCLeave (Domain.cpp:1327-1496) at a halt of the device: particles that left the box are
 // recycled into the inflow positions the inflow particles past InFlowLoc1 leave behind
 static int inflow_leave(psph_handle *h, long long *ndel)
@@ -944,9 +1000,13 @@ static int inflow_leave(psph_handle *h, long long *ndel)
         h->upload_index.swap(keep_idx);
         h->inoutcounter = 1;
     }
-    if (H.size() > h->cap)
-        return fail(PSPH_ERR_CAPACITY, "the in/out-flow recycling needs %zu particles, the buffers hold %zu",
-                    H.size(), h->cap);
+    if (H.size() > h->cap) {
+        // the inflow outpaces the outflow: room for half as many particles again (the
+        // reference's Particles array simply grows)
+        rc = grow_buffers(h, H.size() + H.size()/2 + 256);
+        if (rc) return rc;
+        ++h->regrowths;
+    }
     if (h->inoutcounter == 1) {
         // InFlowBCFresh's rebuild of the inflow list (Domain.cpp:1545-1566), from the cells of
         // these positions, which the next step starts from
@@ -1140,6 +1200,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.any_periodic = 1;
         }
         P.afree[c] = P.gravity[c] + h->body[c];
+    P.hook_before = h->hook_b ? 1 : 0;
+    P.hook_after = h->hook_a ? 1 : 0;
     }
     P.ncells = P.nc[0]*P.nc[1]*P.nc[2];
     P.hmax = h->grid.hmax;
@@ -1268,8 +1330,11 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.verlet_force = std::getenv("PSPH_REBUILD_EVERY_STEP") ? 1 : 0;
         }
     }
-    // in/out-flow runs may grow by recycling: room for half as many particles again
-    rc = allocate(h, P.inout ? n + n/2 + 256 : n);
+    // in/out-flow runs may grow by recycling: room for half as many particles again (more is
+    // made when needed, grow_buffers); PSPH_INOUT_SLACK sets the extra room (test switch)
+    size_t slack = n/2 + 256;
+    if (const char *e = std::getenv("PSPH_INOUT_SLACK")) slack = (size_t)std::max(0, std::atoi(e));
+    rc = allocate(h, P.inout ? n + slack : n);
     if (rc) return rc;
     h->next_index = (int32_t)n;
     if (P.inout) {
@@ -1347,14 +1412,17 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     if (rc) return rc;
     launch_head(h->D, h->P, h->stream);
     long long queued = 0;
-    bool pending_finish = C.mid_step != 0 && max_steps > 0;
+    // a step stopped mid-way is finished first: from its wall pass (stop before the wall pass)
+    // or by its Move (stop after the force pass)
+    int pending_from = (C.mid_step != 0 && max_steps > 0) ? (C.mid_step == 2 ? 1 : 2) : 0;
+    bool pending_finish = pending_from != 0;
     const long long batch = 16;
     for (;;) {
         // queue a batch; every kernel is a no-op once the device has halted
         long long nq = std::min<long long>(batch, max_steps - queued);
         if (pending_finish && nq < 1) nq = 1;
         for (long long k = 0; k < nq; ++k) {
-            enqueue_step(h, pending_finish);     // a pending step is finished by its Move alone
+            enqueue_step(h, pending_finish ? pending_from : 0);
             pending_finish = false;
             ++queued;
         }
@@ -1403,7 +1471,9 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     s.n = (size_t)h->P.n;
     s.mid_step = C.mid_step;
     s.list_builds = C.rebuilds - builds0;
-    s.stop_reason = (C.halt == kHaltOutput) ? PSPH_STOP_OUTPUT : (C.halt == kHaltTend) ? PSPH_STOP_TEND : PSPH_STOP_STEPS;
+    s.stop_reason = (C.halt == kHaltOutput) ? PSPH_STOP_OUTPUT : (C.halt == kHaltTend) ? PSPH_STOP_TEND
+                  : (C.halt == kHaltHookBefore) ? PSPH_STOP_HOOK_BEFORE : (C.halt == kHaltHookAfter) ? PSPH_STOP_HOOK_AFTER
+                  : PSPH_STOP_STEPS;
     if (st) *st = s;
     if (C.halt == kHaltError) {
         h->broken = true;
@@ -1440,6 +1510,46 @@ int psph_set_body_force(psph_handle *h, const double accel[3])
     return 0;
 }
 
+int psph_set_hooks(psph_handle *h, int before, int after)
+{
+    if (!h) return fail(PSPH_ERR_INVALID, "null handle");
+    if (h->uploaded) {
+        int rc = sync_ctl(h);
+        if (rc) return rc;
+        if (h->ctl_host->mid_step) return fail(PSPH_ERR_STATE, "hooks changed while a step is pending");
+    }
+    h->hook_b = before != 0;
+    h->hook_a = after != 0;
+    h->P.hook_before = h->hook_b ? 1 : 0;
+    h->P.hook_after = h->hook_a ? 1 : 0;
+    return 0;
+}
+
+int psph_hook_accel(psph_handle *h, const double *a, const double *ddensity, size_t n)
+{
+    if (!h || !a) return fail(PSPH_ERR_INVALID, "null argument");
+    if (!h->uploaded) return fail(PSPH_ERR_STATE, "hook values before upload");
+    int rc = sync_ctl(h);
+    if (rc) return rc;
+    const int ms = h->ctl_host->mid_step;
+    if (ms != 1 && ms != 2) return fail(PSPH_ERR_STATE, "psph_hook_accel outside a stop inside a step");
+    if (n != (size_t)h->P.n) return fail(PSPH_ERR_INVALID, "%zu values for %d particles", n, h->P.n);
+    if (h->hook_cap < 4*n) {
+        if (h->hook_dev) { rc = dev_free(h, (void **)&h->hook_dev); if (rc) return rc; }
+        h->hook_cap = 0;
+        rc = dev_alloc(h, (void **)&h->hook_dev, sizeof(double)*4*n);
+        if (rc) return rc;
+        h->hook_cap = 4*n;
+    }
+    HIPCHECK(hipMemcpyAsync(h->hook_dev, a, sizeof(double)*3*n, hipMemcpyHostToDevice, h->stream));
+    if (ddensity)
+        HIPCHECK(hipMemcpyAsync(h->hook_dev + 3*n, ddensity, sizeof(double)*n, hipMemcpyHostToDevice, h->stream));
+    launch_hook_accel(h->D, h->P, h->hook_dev, ddensity ? h->hook_dev + 3*n : nullptr, h->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 int psph_output_begin(psph_handle *h)
 {
     if (!h) return fail(PSPH_ERR_INVALID, "null handle");
@@ -1447,6 +1557,7 @@ int psph_output_begin(psph_handle *h)
     if (h->out_pending) return fail(PSPH_ERR_STATE, "an output is pending (call psph_output_end first)");
     int rc = sync_ctl(h);
     if (rc) return rc;
+    if (h->ctl_host->mid_step == 2) return fail(PSPH_ERR_STATE, "output at a stop before the pair sums");
     const Params &P = h->P;
     const size_t n = (size_t)P.n, nf = (size_t)output_floats();
     if (n > h->out_cap) {
@@ -1457,6 +1568,10 @@ int psph_output_begin(psph_handle *h)
         if (rc) return rc;
         HIPCHECK(hipHostMalloc((void **)&h->out_host, sizeof(float)*nf*n, hipHostMallocDefault));
         h->out_cap = n;
+    } else if (!h->out_dev) {
+        // the device buffers were re-created for more particles; the pinned copy is kept
+        rc = dev_alloc(h, (void **)&h->out_dev, sizeof(float)*nf*h->out_cap);
+        if (rc) return rc;
     }
     if (!h->out_event) HIPCHECK(hipEventCreateWithFlags(&h->out_event, hipEventDisableTiming));
     const Dev D = with_state(h, state_buffer(h));

@@ -52,6 +52,8 @@ enum : int {
     kHaltOutput = 3,   // Time >= tout after the force pass (mid-step)
     kHaltLeave = 4,    // particles outside [BLPF, TRPR] after Move
     kHaltError = 5,    // see Ctl::error
+    kHaltHookBefore = 6,   // host hook before the wall smoothing and the pair sums (mid-step)
+    kHaltHookAfter = 7,    // host hook after the pair sums, before Move (mid-step)
 };
 
 // Solve-loop scalars, resident in device memory so steps run without host round trips.
@@ -62,7 +64,13 @@ struct Ctl {
     long long steps_done, max_steps;
     unsigned long long pairs_ff, pairs_fw;  // directed free-free and free-fixed in-support pairs
     unsigned long long last_pairs;        // unordered in-support pairs of the last force pass
+    // mid_step: 1 = stopped after the force pass (Move pending), 2 = stopped before the wall
+    // smoothing (host hook GeneralBefore); resume (set by the run head): 1 = the step continues
+    // past the hook stop before the wall pass, 2 = past the stop after the force pass
     int halt, error, mid_step, resume;
+    // the pair sums of this step start from acc (accelerations and density rates the host wrote
+    // at a stop before the wall pass) instead of Gravity plus the body force
+    int acc_init;
     int leave, xmat;
     int nl_over;                        // targets whose neighbours overflowed the lists (cumulative)
     int nl_q;                           // overflow queue length of the current force pass
@@ -143,6 +151,9 @@ struct Params {
     // a free particle's acceleration at the start of its force pass: Gravity plus the body force
     // of the body-force hook (GeneralBefore adding a constant to every free particle's a)
     double afree[3];
+    // host hooks: stop before the wall smoothing (GeneralBefore) and after the pair sums
+    // (GeneralAfter) of every step
+    int hook_before, hook_after;
 };
 
 // One set of the persistent columns.

@@ -888,6 +888,15 @@ __device__ inline void force_partials(const Dev &D, unsigned long long dtb, unsi
     }
 }
 
+// The acceleration (x, y, z) and density rate (w) a free target's pair sums start from:
+// StartAcceleration's Gravity, plus the body force of the body-force hook; after a host hook
+// before the pair sums, the values the host wrote into acc for this step (psph_hook_accel).
+__device__ inline double4 a_start(const Dev &D, const Params &P, int i)
+{
+    if (D.ctl->acc_init) return D.acc[i];
+    return make_double4(P.afree[0], P.afree[1], P.afree[2], 0.0);
+}
+
 // Per-target sums of a force pass.  T = float for the FP32-mixed soil tier, whose continuity
 // sum carries a Kahan compensation term ddc.
 template <class T>
@@ -1246,8 +1255,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     int zw_on = shep_on;
     constexpr bool F32 = sizeof(T) == 4;
     AccT<T> A;
-    A.a0 = F32 ? (T)0 : (T)P.afree[0]; A.a1 = F32 ? (T)0 : (T)P.afree[1]; A.a2 = F32 ? (T)0 : (T)P.afree[2];
-    A.dd = 0.0; A.ddc = 0.0;
+    const double4 a_init = act ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
+    A.a0 = F32 ? (T)0 : (T)a_init.x; A.a1 = F32 ? (T)0 : (T)a_init.y; A.a2 = F32 ? (T)0 : (T)a_init.z;
+    A.dd = F32 ? (T)0 : (T)a_init.w; A.ddc = 0.0;
     A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0; A.sw = 0.0;
     for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
     A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
@@ -1336,8 +1346,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     if (act) {
         double a0 = A.a0, a1 = A.a1, a2 = A.a2, dd = A.dd;
         if (F32) {
-            a0 = P.afree[0] + (double)A.a0; a1 = P.afree[1] + (double)A.a1; a2 = P.afree[2] + (double)A.a2;
-            dd = (double)A.dd + (double)(-A.ddc);
+            a0 = a_init.x + (double)A.a0; a1 = a_init.y + (double)A.a1; a2 = a_init.z + (double)A.a2;
+            dd = a_init.w + ((double)A.dd + (double)(-A.ddc));
         }
         D.acc[i] = make_double4(a0, a1, a2, dd);
         if (P.any_xsph || zw_on) D.xsp[i] = make_double4(A.vx0, A.vx1, A.vx2, A.zw);
@@ -1727,11 +1737,11 @@ struct Acc11 { double a0, a1, a2, dd; };
 // current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
 // target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
 template <int DIM, bool SKIN>
-__device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
-                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
-                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
+__device__ inline void pair11_rec(const Dev &D, const Params &P, const double *pa, const double *pb,
+                                  double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
+                                  double4 a0, double4 a1, double4 a2, double4 vw,
+                                  double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
-    double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
     const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
     double r2 = x0*x0 + x1*x1 + x2*x2;
@@ -1755,8 +1765,8 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
     int cj = freej ? (int)meta : (int)(-1.0 - meta);
-    double w0 = a1.x, w1 = a1.y, w2 = a1.z;              // viscous velocity of the neighbour
-    if (!freej) { double4 vw = D.gw[j]; w0 = vw.x; w1 = vw.y; w2 = vw.z; }
+    // viscous velocity of the neighbour: a free neighbour's is its v
+    double w0 = freej ? a1.x : vw.x, w1 = freej ? a1.y : vw.y, w2 = freej ? a1.z : vw.z;
     double alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
     double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     double dvx = u0*x0 + u1*x1 + u2*x2;
@@ -1777,6 +1787,20 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     double temp1 = (u0*x0 + u1*x1 + u2*x2)*GK;
     A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
     A.dd += mj*(di*invdj)*temp1;
+}
+
+// SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
+// current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
+// target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
+template <int DIM, bool SKIN>
+__device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
+                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
+                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
+{
+    double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
+    double4 vw = a1;
+    if (a2.y < 0.0) vw = D.gw[j];
+    pair11_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, a0, a1, a2, vw, mu_ff, mu_w, A, nff, nfw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1897,7 +1921,10 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
     int ci = (int)gi2.y;
     const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     Acc11 A;
-    A.a0 = P.afree[0]; A.a1 = P.afree[1]; A.a2 = P.afree[2]; A.dd = 0.0;
+    {
+        const double4 a_init = act ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
+        A.a0 = a_init.x; A.a1 = a_init.y; A.a2 = a_init.z; A.dd = a_init.w;
+    }
     unsigned nff = 0, nfw = 0;
     const int cap = P.nl_cap;
     const bool codes = P.nl_codes != 0;
@@ -2119,8 +2146,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
         act = false;
     }
     if (act) {
-        double a0 = P.afree[0] + (double)A.a0, a1 = P.afree[1] + (double)A.a1, a2 = P.afree[2] + (double)A.a2;
-        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
+        const double4 ai = a_start(D, P, i);
+        double a0 = ai.x + (double)A.a0, a1 = ai.y + (double)A.a1, a2 = ai.z + (double)A.a2;
+        D.acc[i] = make_double4(a0, a1, a2, ai.w + ((double)A.dd + (double)(-A.ddc)));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         dtb = dbits(cand);
@@ -2171,8 +2199,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
             }
             ++k;
         });
-        double a0 = P.afree[0] + (double)A.a0, a1 = P.afree[1] + (double)A.a1, a2 = P.afree[2] + (double)A.a2;
-        D.acc[i] = make_double4(a0, a1, a2, (double)A.dd + (double)(-A.ddc));
+        const double4 ai = a_start(D, P, i);
+        double a0 = ai.x + (double)A.a0, a1 = ai.y + (double)A.a1, a2 = ai.z + (double)A.a2;
+        D.acc[i] = make_double4(a0, a1, a2, ai.w + ((double)A.dd + (double)(-A.ddc)));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         atomicMin(&D.ctl->dtmin_bits, dbits(cand));
@@ -2294,7 +2323,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
     const Dev D = bind(D0);
     Ctl *ctl = D.ctl;
     if (ctl->halt) return;
-    if (!ctl->resume && ctl->time >= ctl->tout) return;       // output stop before Move
+    if (ctl->resume < 2 && (ctl->time >= ctl->tout || P.hook_after)) return;   // stop before Move
     double deltatmin = bitsd(ctl->dtmin_bits);
     double dt = adaptive_dt(ctl->deltat, ctl->deltatint, deltatmin);
     if (dt < (ctl->deltatint/1.0e5)) return;                  // Fatal: reported by k_tail
@@ -2508,7 +2537,13 @@ __global__ void k_tail(Dev D, Params P)
     double deltatmin = bitsd(c->dtmin_bits);
     c->deltatmin = deltatmin;
     if (c->xmat) { c->halt = kHaltError; c->error = -2; return; }
-    if (!c->resume && c->time >= c->tout) { c->halt = kHaltOutput; c->mid_step = 1; return; }
+    if (c->resume < 2 && (c->time >= c->tout || P.hook_after)) {
+        // output time (Domain.cpp:1773), or the GeneralAfter hook of every step (:1770): the
+        // host takes over with the pair sums formed and Move pending
+        c->halt = (c->time >= c->tout) ? kHaltOutput : kHaltHookAfter;
+        c->mid_step = 1;
+        return;
+    }
     double dt = adaptive_dt(c->deltat, c->deltatint, deltatmin);
     if (dt < (c->deltatint/1.0e5)) { c->halt = kHaltError; c->error = -4; c->deltat = dt; return; }
     c->deltat = dt;
@@ -2516,6 +2551,7 @@ __global__ void k_tail(Dev D, Params P)
     c->steps_done += 1;
     c->resume = 0;
     c->mid_step = 0;
+    c->acc_init = 0;
     c->n_out = 0;
     // the step's working copy is the next step's state
     c->side ^= c->rebuild;
@@ -2547,8 +2583,38 @@ __global__ void k_head(Dev D, Params P)
     c->leave = 0;
     c->xmat = 0;
     if (c->max_steps <= 0) { c->halt = kHaltSteps; return; }
-    if (c->mid_step) { c->resume = 1; return; }
+    if (c->mid_step) { c->resume = (c->mid_step == 2) ? 1 : 2; return; }
     if (!(c->time < c->tf)) { c->halt = kHaltTend; return; }
+}
+
+// The GeneralBefore hook of every step (Domain.cpp:1767): after StartAcceleration, the
+// in/out-flow imposition and the neighbour search, the host takes over before the wall
+// smoothing and the pair sums.  A step resumed past this stop does not launch it.
+__global__ void k_hook_stop(Dev D, Params P)
+{
+    Ctl *c = D.ctl;
+    if (c->halt || c->resume) return;
+    c->halt = kHaltHookBefore;
+    c->mid_step = 2;
+}
+
+// Accelerations and density rates from the host at a hook stop, in the host's particle order
+// (the reference order of psph_download): before the wall pass they are the values the pair
+// sums of this step start from (acc_init), after the force pass the values Move uses.  Fixed
+// particles keep theirs (their pair sums feed nothing).
+__global__ void __launch_bounds__(kBlock) k_hook_accel(Dev D0, Params P, const double *a, const double *dd)
+{
+    const Dev D = bind(D0);
+    const int i = blockIdx.x*kBlock + threadIdx.x;
+    const bool before = D.ctl->mid_step == 2;
+    if (i == 0 && before) D.ctl->acc_init = 1;
+    if (i >= P.n) return;
+    if (!flag_free(D.flags_b[i])) return;
+    const int o = D.orig_b[i];
+    double4 v = D.acc[i];
+    v.x = a[3*o]; v.y = a[3*o + 1]; v.z = a[3*o + 2];
+    v.w = dd ? dd[o] : (before ? 0.0 : v.w);     // StartAcceleration zeroes dDensity
+    D.acc[i] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2684,12 +2750,15 @@ static void launch_wall_pass(const Dev &D, const Params &P, hipStream_t s);
 
 // The wall pass, then with several materials the FSI smoothing and the saturation of soil,
 // in the order of PrimaryComputeAcceleration (Domain.cpp:1020-1234).
-void launch_wall(const Dev &D, const Params &P, hipStream_t s)
+void launch_wall(const Dev &D, const Params &P, hipStream_t s, int after_hook)
 {
-    if (P.inout) {
-        // InFlowBCFresh runs before the neighbour search (Domain.cpp:1763-1766)
-        hipLaunchKernelGGL(k_io_mark, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
-        hipLaunchKernelGGL(k_io_impose, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+    if (!after_hook) {
+        if (P.inout) {
+            // InFlowBCFresh runs before the neighbour search (Domain.cpp:1763-1766)
+            hipLaunchKernelGGL(k_io_mark, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+            hipLaunchKernelGGL(k_io_impose, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+        }
+        if (P.hook_before) hipLaunchKernelGGL(k_hook_stop, dim3(1), dim3(1), 0, s, D, P);
     }
     launch_wall_pass(D, P, s);
     if (P.multi) {
@@ -2777,6 +2846,9 @@ void launch_integrate(const Dev &D, const Params &P, hipStream_t s)
 
 void launch_tail(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, D, P); }
+
+void launch_hook_accel(const Dev &D, const Params &P, const double *a, const double *dd, hipStream_t s)
+{ hipLaunchKernelGGL(k_hook_accel, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P, a, dd); }
 
 void launch_pairs(const Dev &D, const Params &P, hipStream_t s, int pass, const long long *off,
                   int *cnt, long long *out)

@@ -18,7 +18,7 @@ EXPORTS = [
     "psph_create", "psph_upload", "psph_run", "psph_download", "psph_neighbour_pairs",
     "psph_neighbour_pairs_noalloc", "psph_get_grid", "psph_set_timing", "psph_kernel_stats", "psph_stream",
     "psph_traffic_model", "psph_destroy", "psph_last_error", "psph_version",
-    "psph_output_begin", "psph_output_end", "psph_set_body_force",
+    "psph_output_begin", "psph_output_end", "psph_set_body_force", "psph_set_hooks", "psph_hook_accel",
 ]
 
 _lib = None
@@ -47,6 +47,8 @@ def lib():
         L.psph_destroy.argtypes = [p]
         L.psph_output_begin.argtypes = [p]
         L.psph_set_body_force.argtypes = [p, p]
+        L.psph_set_hooks.argtypes = [p, ctypes.c_int, ctypes.c_int]
+        L.psph_hook_accel.argtypes = [p, p, p, sz]
         L.psph_output_end.argtypes = [p, ctypes.POINTER(OutputF32)]
         L.psph_last_error.argtypes = [ctypes.c_char_p, sz]
         for f in EXPORTS:
@@ -104,6 +106,17 @@ class Solver:
         body-force GeneralBefore hook, psph_set_body_force)."""
         a = np.ascontiguousarray(accel, dtype=np.float64)
         _check(lib().psph_set_body_force(self._h, _ptr(a)))
+
+    def set_hooks(self, before, after):
+        """Stop every step at the GeneralBefore / GeneralAfter hook points (psph_set_hooks)."""
+        _check(lib().psph_set_hooks(self._h, 1 if before else 0, 1 if after else 0))
+
+    def hook_accel(self, a, ddensity=None):
+        """At a stop inside a step: the accelerations (and density rates) of every particle in
+        download order, as a hook left them (psph_hook_accel)."""
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+        dd = None if ddensity is None else np.ascontiguousarray(ddensity, dtype=np.float64)
+        _check(lib().psph_hook_accel(self._h, _ptr(a), None if dd is None else _ptr(dd), len(a)))
 
     def output_begin(self):
         """Starts the packed FP32 output of the current state (psph_output_begin)."""

@@ -1,15 +1,17 @@
-"""Side-by-side run of the FP64 and FP32-mixed tiers on identical seeded inputs.
+"""The FP64 and FP32-mixed tiers on the 3-D dam break, to t = 1 s, on identical seeded inputs.
 
-    python -m persiansph_amd.tiers --dx 0.01 --t-end 1.0 --samples 20 --out tiers.json
+    python -m persiansph_amd.tiers --tier fp64 --t-end 1.0 --samples 20 --out fp64.jsonl
+    python -m persiansph_amd.tiers --tier fp32 --t-end 1.0 --samples 20 --out fp32.jsonl
+    python -m persiansph_amd.tiers --compare fp64.jsonl fp32.jsonl
 
-Both solvers stop at each sample time t_k (the first force pass with Time >= t_k, as an output
-time of Solve would).  At each stop the script records:
-  - the per-step error: one force pass of each tier on the same state (the FP64 run's), with
-    the relative L2 error of the FP32-mixed accelerations and continuity rates;
-  - the trajectory difference: density excess, acceleration and position of the two runs
-    (these grow with the chaotic divergence of the two trajectories, not with a per-step error);
-  - the surge front of each run and its relative difference;
-  - the wall time per step of each tier.
+A tier run stops at each sample time t_k (the first force pass with Time >= t_k, as an output
+time of Solve would) and appends one JSON line to --out (flushed, so a run cut short keeps its
+samples): the surge front, the plain maximum x of the fluid, the mean density excess, the
+steps, list builds and wall time so far.  The FP64 run also measures the per-step error of the
+FP32-mixed tier at each sample: one force pass of each tier on the FP64 run's state, with the
+relative L2 error of the FP32-mixed accelerations and continuity rates.  --compare joins the
+two runs' samples: the front difference relative to the front and to the distance travelled,
+and the speed-up of FP32-mixed over FP64 in wall time per simulated second.
 """
 import argparse
 import json
@@ -20,83 +22,105 @@ import numpy as np
 
 from . import FP32_MIXED, FP64, Solver, analysis, cases
 
+COLUMN_LENGTH = 1.2
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def run_tier(args):
+    prec = FP64 if args.tier == "fp64" else FP32_MIXED
+    cfg, p, dt = cases.dam_break_3d(dx=args.dx, scheme=1, precision=prec)
+    cfg32 = type(cfg).from_buffer_copy(cfg)
+    cfg32.precision = FP32_MIXED
+    cfg64 = type(cfg).from_buffer_copy(cfg)
+    cfg64.precision = FP64
+    n0 = len(p)
+    s = Solver(cfg, p, dt)
+    del p
+    out = open(args.out, "a") if args.out else None
+    wall = 0.0
+    steps = builds = 0
+    for k in range(1, args.samples + 1):
+        tk = args.t_end*k/args.samples
+        t0 = time.perf_counter()
+        st = s.run(10**9, tout=tk)
+        wall += time.perf_counter() - t0
+        steps += st.steps
+        builds += st.list_builds
+        q = s.particles()
+        fr = q["IsFree"] == 1
+        row = {
+            "tier": args.tier, "t": st.time, "t_sample": tk, "steps": steps, "list_builds": builds,
+            "wall_s": wall, "ms_per_step": 1e3*wall/max(steps, 1), "n": int(len(q)),
+            "front": analysis.front_position(q, 3), "max_x": float(q["x"][fr, 0].max()),
+            "mean_density_excess": float(np.mean(q["Density"][fr]/1000.0 - 1.0)),
+            "deltat": st.deltat,
+        }
+        if args.tier == "fp64" and not args.no_step_error:
+            # per-step error of FP32-mixed: both tiers' force pass on this state
+            one = {}
+            for name, c in (("fp64", cfg64), ("fp32", cfg32)):
+                s1 = Solver(c, q, st.deltat, time=st.time)
+                s1.run(1, tout=0.0)
+                one[name] = s1.particles()
+                s1.close()
+            f1 = one["fp64"]["IsFree"] == 1
+            row["step_rel_err_accel"] = analysis.relative_error(one["fp32"]["a"][f1], one["fp64"]["a"][f1])
+            row["step_rel_err_ddensity"] = analysis.relative_error(one["fp32"]["dDensity"][f1],
+                                                                   one["fp64"]["dDensity"][f1])
+        log(json.dumps(row))
+        if out:
+            out.write(json.dumps(row) + "\n")
+            out.flush()
+    s.close()
+    summary = {"tier": args.tier, "workload": f"3-D dam break dx={args.dx}, {n0} particles, Leapfrog",
+               "t_end": args.t_end, "steps": steps, "list_builds": builds, "wall_s": wall,
+               "ms_per_step": 1e3*wall/max(steps, 1), "particle_updates_per_s": n0*steps/max(wall, 1e-9)}
+    print(json.dumps(summary))
+
+
+def compare(path64, path32):
+    def load(p):
+        with open(p) as f:
+            return {round(r["t_sample"], 9): r for r in map(json.loads, f) if r}
+    a, b = load(path64), load(path32)
+    rows = []
+    for t in sorted(set(a) & set(b)):
+        r64, r32 = a[t], b[t]
+        f64, f32 = r64["front"], r32["front"]
+        rows.append({
+            "t_sample": t, "front_fp64": f64, "front_fp32": f32,
+            "front_rel_err": abs(f32 - f64)/f64,
+            "front_rel_err_of_travel": abs(f32 - f64)/max(f64 - COLUMN_LENGTH, 1e-9),
+            "steps_fp64": r64["steps"], "steps_fp32": r32["steps"],
+            "wall_fp64": r64["wall_s"], "wall_fp32": r32["wall_s"],
+            "step_rel_err_accel": r64.get("step_rel_err_accel"),
+            "step_rel_err_ddensity": r64.get("step_rel_err_ddensity"),
+        })
+    last = rows[-1] if rows else None
+    out = {"samples": rows}
+    if last:
+        out["speedup_fp32_over_fp64_wall"] = last["wall_fp64"]/last["wall_fp32"]
+        out["max_front_rel_err"] = max(r["front_rel_err"] for r in rows)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--tier", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--dx", type=float, default=0.01)
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--samples", type=int, default=20)
+    ap.add_argument("--no-step-error", action="store_true")
     ap.add_argument("--out", default="")
+    ap.add_argument("--compare", nargs=2, metavar=("FP64_JSONL", "FP32_JSONL"))
     args = ap.parse_args()
-
-    cfg, p, dt = cases.dam_break_3d(dx=args.dx, scheme=1, precision=FP64)
-    cfg32 = type(cfg).from_buffer_copy(cfg)
-    cfg32.precision = FP32_MIXED
-    n0 = len(p)
-    L0 = 1.2
-    solvers = {"fp64": Solver(cfg, p, dt), "fp32": Solver(cfg32, p, dt)}
-    del p
-    wall = {k: 0.0 for k in solvers}
-    steps = {k: 0 for k in solvers}
-    rows = []
-    for k in range(1, args.samples + 1):
-        tk = args.t_end*k/args.samples
-        states = {}
-        for name, s in solvers.items():
-            t0 = time.perf_counter()
-            st = s.run(10**9, tout=tk)
-            wall[name] += time.perf_counter() - t0
-            steps[name] += st.steps
-            q = s.particles()
-            states[name] = (st, q)
-        (s64, q64), (s32, q32) = states["fp64"], states["fp32"]
-        common = np.intersect1d(q64["Index"], q32["Index"])
-        i64 = np.searchsorted(q64["Index"], common)
-        i32 = np.searchsorted(q32["Index"], common)
-        fr = q64["IsFree"][i64] == 1
-        rho_err = analysis.relative_error(q32["Density"][i32][fr] - 1000.0, q64["Density"][i64][fr] - 1000.0)
-        a_err = analysis.relative_error(q32["a"][i32][fr], q64["a"][i64][fr])
-        x_err = float(np.abs(q32["x"][i32][fr] - q64["x"][i64][fr]).max())
-        # per-step error: both tiers' force pass on the FP64 run's state
-        one = {}
-        for name, c in (("fp64", cfg), ("fp32", cfg32)):
-            s1 = Solver(c, q64, s64.deltat, time=s64.time)
-            s1.run(1, tout=0.0)
-            one[name] = s1.particles()
-            s1.close()
-        f1 = one["fp64"]["IsFree"] == 1
-        step_a = analysis.relative_error(one["fp32"]["a"][f1], one["fp64"]["a"][f1])
-        step_dd = analysis.relative_error(one["fp32"]["dDensity"][f1], one["fp64"]["dDensity"][f1])
-        f64 = analysis.front_position(q64, 3)
-        f32 = analysis.front_position(q32, 3)
-        row = {
-            "t_fp64": s64.time, "t_fp32": s32.time, "steps_fp64": steps["fp64"], "steps_fp32": steps["fp32"],
-            "n_fp64": int(len(q64)), "n_fp32": int(len(q32)),
-            "step_rel_err_accel": step_a, "step_rel_err_ddensity": step_dd,
-            "rel_err_density_excess": rho_err, "rel_err_accel": a_err, "max_abs_dx": x_err,
-            "front_fp64": f64, "front_fp32": f32,
-            "front_rel_err": abs(f32 - f64)/f64, "front_rel_err_travel": abs(f32 - f64)/max(f64 - L0, 1e-9),
-            "ms_per_step_fp64": 1e3*wall["fp64"]/max(steps["fp64"], 1),
-            "ms_per_step_fp32": 1e3*wall["fp32"]/max(steps["fp32"], 1),
-        }
-        rows.append(row)
-        log(json.dumps(row))
-    out = {
-        "workload": f"3-D dam break dx={args.dx}, {n0} particles, Leapfrog, to t={args.t_end}",
-        "samples": rows,
-        "speedup_fp32_over_fp64": (wall["fp64"]/steps["fp64"])/(wall["fp32"]/steps["fp32"]),
-        "max_front_rel_err": max(r["front_rel_err"] for r in rows),
-        "max_step_rel_err_accel": max(r["step_rel_err_accel"] for r in rows),
-        "max_step_rel_err_ddensity": max(r["step_rel_err_ddensity"] for r in rows),
-    }
-    if args.out:
-        with open(args.out, "w") as f:
-            json.dump(out, f, indent=1)
-    print(json.dumps({k: v for k, v in out.items() if k != "samples"}))
+    if args.compare:
+        compare(*args.compare)
+    else:
+        run_tier(args)
 
 
 if __name__ == "__main__":

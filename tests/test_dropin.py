@@ -147,3 +147,45 @@ def test_channel_inflow_dropin_matches_binding(tmp_path):
     np.testing.assert_array_equal(q["v"][:, 0], got[:, 3])
     assert f"{st.time:.9e}" in r.stdout
     s.close()
+
+
+@pytest.mark.gpu
+def test_poiseuille_user_hook_equals_body_force_hook(tmp_path):
+    """1-Poiseuille.cpp's own GeneralBefore = UserAcc (a += (0.002, 0, 0) on free particles), run
+    by Solve on the host at its point of every step, gives the GPU body-force hook's result bit
+    for bit: a = Gravity + 0.002 is the same sum either way, and the pair sums start from it."""
+    exe = os.path.join(ROOT, "persiansph_amd", "cpp", "examples", "poiseuille")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    from persiansph_amd import cases
+    cfg, p, dt, body = cases.poiseuille_2d()
+    tf, dt_out, maxidx = 60*dt, 20*dt, 2
+    res = {}
+    for hook in ("body", "user"):
+        out = tmp_path / f"{hook}.bin"
+        print(f"[dropin] poiseuille hook={hook}", flush=True)
+        r = subprocess.run([exe, repr(tf), repr(dt_out), "-", str(maxidx), str(out), hook],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        res[hook] = (_read(out), r.stdout.splitlines()[-1])
+    np.testing.assert_array_equal(res["user"][0], res["body"][0])
+    assert res["user"][1] == res["body"][1]
+
+
+REF = "/root/reference"
+REF_EXAMPLES = ["1-Poiseuille.cpp", "2-Couette.cpp", "3-DamBreak.cpp", "4-DamBreak3D.cpp"]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="the reference tree is not on this machine")
+@pytest.mark.parametrize("name", REF_EXAMPLES)
+def test_reference_examples_compile_against_dropin(name):
+    """The reference's own example drivers compile unchanged against the drop-in header (their
+    GeneralBefore = UserAcc included): g++ -fsyntax-only with persiansph_amd/cpp/SPH first on
+    the include path.  Nothing is copied from the reference tree."""
+    import __graft_entry__ as g
+    cflags, _ = g.hdf5_flags()
+    src = os.path.join(REF, name)
+    if not os.path.exists(src):
+        pytest.skip(f"{name} not in the reference tree")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-fopenmp", "-I" + os.path.join(ROOT, "include"),
+                    "-I" + os.path.join(ROOT, "persiansph_amd", "cpp", "SPH"),
+                    "-I" + os.path.join(ROOT, "persiansph_amd", "cpp")] + cflags + [src], check=True)

@@ -56,3 +56,28 @@ def test_inflow_parity(case):
         assert deleted > 0                     # outflow without the outflow zone deletes
     assert recycled > 0 or created > 0         # the recycling ran
     g.close()
+
+
+def test_inflow_growth_past_the_buffers(monkeypatch):
+    """With no spare room in the device buffers (PSPH_INOUT_SLACK=0), the first recycling that
+    creates more particles than it deletes re-creates the buffers for more particles instead of
+    failing; the run stays on the oracle's, whose Particles array simply grows."""
+    monkeypatch.setenv("PSPH_INOUT_SLACK", "0")
+    cfg, p, dt = cases.channel_inflow_2d(in_out_flow=1)
+    print(f"[inflow growth] n={len(p)}", flush=True)
+    g = Solver(cfg, p, dt)
+    o = Oracle(cfg, p, dt)
+    n0, nmax = len(p), len(p)
+    for chunk in range(8):
+        sg, so = g.run(250), o.run(250)
+        assert sg.steps == so.steps == 250
+        G, O = g.particles(), o.particles()
+        assert len(G) == len(O)
+        nmax = max(nmax, len(O))
+        np.testing.assert_array_equal(G["InOut"], O["InOut"])
+        np.testing.assert_allclose(G["x"], O["x"], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(G["v"], O["v"], rtol=0, atol=1e-9)
+    assert nmax > n0                            # the buffers had to grow
+    out = g.output_f32()                        # outputs after the growth
+    np.testing.assert_array_equal(out["position"], G["x"].astype(np.float32))
+    g.close()
