@@ -200,6 +200,19 @@ static int nl_alloc(psph_handle *h, int cap)
     if (rc) return rc;
     P.nl_cap = cap;
     h->D.nl = h->nl;
+    if (P.tiles) {
+        // tiles of the FP64 fast path: at most nfree / kTileTargets + one per cell row
+        P.tile_max = (int)(nf/kTileTargets) + P.nc[1]*P.nc[2] + 1;
+        // slot lists: per target, groups of 4*kTileLanes entries (one 8-byte word per lane)
+        P.tl_stride = kTileTargets*((cap + 4*kTileLanes - 1)/(4*kTileLanes))*4*kTileLanes;
+        if ((rc = dev_free(h, (void **)&h->D.tile)) || (rc = dev_free(h, (void **)&h->D.items)) ||
+            (rc = dev_free(h, (void **)&h->D.tl)))
+            return rc;
+        if ((rc = dev_alloc(h, (void **)&h->D.tile, sizeof(int4)*(size_t)P.tile_max)) ||
+            (rc = dev_alloc(h, (void **)&h->D.items, sizeof(int32_t)*(size_t)P.tile_max*kTileItems)) ||
+            (rc = dev_alloc(h, (void **)&h->D.tl, sizeof(uint16_t)*(size_t)P.tile_max*P.tl_stride)))
+            return rc;
+    }
     return 0;
 }
 
@@ -913,6 +926,7 @@ static int grow_buffers(psph_handle *h, size_t cap)
     h->nl = h->wl = nullptr;
     h->out_dev = nullptr;
     h->hook_dev = nullptr;
+    h->D.tile = nullptr; h->D.items = nullptr; h->D.tl = nullptr;
     h->hook_cap = 0;
     int rc = allocate(h, cap);
     if (rc) return rc;
@@ -1341,6 +1355,12 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         rc = inflow_init(h);
         if (rc) return rc;
     }
+    // the FP64 fast path with list reuse gathers per tile with the neighbours in LDS
+    // (PSPH_NO_TILES: the per-target list gather instead)
+    P.tiles = (P.fast11 && !P.mixed32 && P.verlet && !std::getenv("PSPH_NO_TILES")) ? 1 : 0;
+    P.tile_items = kTileItems;
+    if (const char *e = std::getenv("PSPH_TILE_ITEMS")) P.tile_items = std::max(0, std::min(kTileItems, std::atoi(e)));
+    h->D.tile = nullptr; h->D.items = nullptr; h->D.tl = nullptr;
     // neighbour lists for the fast fluid path (entries pack the index in 27 bits, 25 with reuse)
     P.nl_cap = 0;
     P.nl_codes = n < ((size_t)1 << 27) ? 1 : 0;

@@ -85,6 +85,7 @@ struct Ctl {
     long long rebuilds;                 // list builds (sorts) since upload
     int ct, shep_counter, first_step, n_out;   // n_out: outflow particles of this step
     long long reorders;                 // completed reorders (buffer parity)
+    int ntiles;                         // tiles of the FP64 fast path at the last list build
 };
 
 // Run-wide constants.
@@ -154,6 +155,11 @@ struct Params {
     // host hooks: stop before the wall smoothing (GeneralBefore) and after the pair sums
     // (GeneralAfter) of every step
     int hook_before, hook_after;
+    // FP64 fast path with list reuse: the gather runs per tile with the neighbours' records in
+    // LDS (see Dev::tile); tile_max bounds the tile count (grid size), tl_stride is the list
+    // slots per tile (kTileTargets times the list capacity rounded up to 4 kTileLanes)
+    int tiles, tile_max, tl_stride;
+    int tile_items;                    // items a tile may stage (kTileItems; less: test switch)
 };
 
 // One set of the persistent columns.
@@ -246,6 +252,26 @@ struct Dev {
     int32_t *wl, *wcnt;
     int32_t *wact;     // wall targets with skin neighbours at the last build (Ctl::nwact of them)
     double4 *xref;
+    // tiles of the FP64 fast path with list reuse (Params::tiles): the free targets of one cell
+    // row in groups of at most kTileTargets, tile = (first target, targets, free items, wall
+    // items; -1 wall items: the tile's neighbours do not fit, it reads them from HBM).  A tile's
+    // items are the union of its targets' skin-list neighbours (free ones first, then walls, each
+    // in index order), and its targets' lists hold item slots (16 bits) at tl[tile*tl_stride +
+    // tl_index(target, e)]: lane e % kTileLanes of a target reads its slots four at a time
+    int4 *tile;
+    int32_t *items;
+    uint16_t *tl;
 };
+
+// tiles of the FP64 fast path: targets per tile, lanes per target, the items and wall items
+// whose gather records fit a workgroup's LDS, and the bitmap words of the union build
+constexpr int kTileTargets = 112;
+constexpr int kTileLanes = 8;
+constexpr int kTileItems = 1408;
+constexpr int kTileWalls = 640;
+constexpr int kTileWords = 8192;
+constexpr int kSlotWords = 6;          // slot words a lane holds in registers (4 slots each)
+constexpr int kUnionBlocks = 256;      // grid of the tile union (one block per CU)
+static_assert(kTileItems <= 2*kTileTargets*kTileLanes, "staging takes at most two items a thread");
 
 }  // namespace psph

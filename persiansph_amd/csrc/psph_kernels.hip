@@ -1736,16 +1736,23 @@ struct Acc11 { double a0, a1, a2, dd; };
 // SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
 // current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
 // target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
-template <int DIM, bool SKIN>
+// The pair of free target i (views gi0..gi2, class ci) with the neighbour whose records are
+// a0..a2 (and vw, its viscous velocity, read only for walls).  jf() gives the neighbour's index
+// in the current order; it is called only for the rare key-adjacency part of the skin test.
+template <int DIM, bool SKIN, class JF>
 __device__ inline void pair11_rec(const Dev &D, const Params &P, const double *pa, const double *pb,
-                                  double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
+                                  double4 gi0, double4 gi1, double4 gi2, int ci, int i, JF jf,
                                   double4 a0, double4 a1, double4 a2, double4 vw,
                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
     const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
     double r2 = x0*x0 + x1*x1 + x2*x2;
-    if (SKIN && !skin_member(D, P, i, j, r2)) return;
+    if (SKIN) {
+        bool in = in_support(P, r2);
+        if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[jf()]);
+        if (!in) return;
+    }
     double invr = rsq_nr(r2);
     double rij = r2 > 0.0 ? r2*invr : 0.0;
     double q = rij*P.invh;
@@ -1800,7 +1807,8 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
     double4 vw = a1;
     if (a2.y < 0.0) vw = D.gw[j];
-    pair11_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, a0, a1, a2, vw, mu_ff, mu_w, A, nff, nfw);
+    pair11_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
+                          mu_ff, mu_w, A, nff, nfw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1999,6 +2007,286 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params 
         if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
         if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// FP64 fast path with list reuse: the gather per tile, with the neighbours' records in LDS.
+//
+// The list-only gather re-reads each neighbour's three 32-B records from L2 once per target
+// that lists it (~80 times a step), and is bound by the address processing of those scattered
+// loads.  Here the free targets of one cell row are grouped in tiles of at most kTileTargets;
+// on the steps that build the skin lists, each tile forms the union of its targets' list
+// entries (its items) and rewrites the lists as item slots.  Every step a workgroup then reads
+// its tile's items once, with row-contiguous loads, into LDS and walks the slot lists there,
+// kTileLanes lanes per target (lane l takes entries l, l + kTileLanes, ...), whose partial sums
+// meet in a fixed order.  A tile whose items do not fit reads them from HBM instead.
+// ------------------------------------------------------------------------------------------
+
+// Block-wide exclusive scan of one int per thread (blockDim.x a multiple of 64, <= 1024);
+// returns the thread's offset and the total in *total.
+__device__ inline int block_scan(int v, int *sh, int *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < nw; ++k) { int t = sh[k]; sh[k] = run; run += t; }
+        sh[nw] = run;
+    }
+    __syncthreads();
+    int r = sh[w] + x - v;
+    *total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+// The XCD-aware order of xcd_block over the first nb blocks of a grid.
+__device__ inline int xcd_remap(int b, int nb)
+{
+    int q = nb >> 3, r = nb & 7;
+    int x = b & 7, k = b >> 3;
+    return (x < r) ? x*(q + 1) + k : r*(q + 1) + (x - r)*q + k;
+}
+
+// Tile table (list builds only): the free targets of each cell row (x fastest) split evenly
+// into ceil(count / kTileTargets) tiles.
+__global__ void __launch_bounds__(1024) k_tile_table(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt || !D.ctl->rebuild) return;
+    __shared__ int sh[17];
+    const int rows = P.nc[1]*P.nc[2];
+    int running = 0;
+    for (int r0 = 0; r0 < rows; r0 += 1024) {
+        const int r = r0 + (int)threadIdx.x;
+        int f0 = 0, cnt = 0, nt = 0;
+        if (r < rows) {
+            const int c0 = P.nc[0]*r;
+            f0 = D.fstart[c0];
+            cnt = D.fstart[c0 + P.nc[0]] - f0;
+            nt = (cnt + kTileTargets - 1)/kTileTargets;
+        }
+        int total = 0;
+        const int off = block_scan(nt, sh, &total);
+        for (int k = 0; k < nt; ++k) {
+            const int a = f0 + (int)((long long)k*cnt/nt), b = f0 + (int)((long long)(k + 1)*cnt/nt);
+            D.tile[running + off + k] = make_int4(a, b - a, 0, 0);
+        }
+        running += total;
+    }
+    if (threadIdx.x == 0) D.ctl->ntiles = running;
+}
+
+// Slot of list entry e of tile target tt in the tile's slot list: lane e % kTileLanes of the
+// target reads its entries e, e + kTileLanes, ... four at a time as one 8-byte word, so the
+// entries of group g (entry e = l + g kTileLanes) sit at word (g / 4, tt, l), half-word g % 4.
+__device__ inline int tl_index(int tt, int e)
+{
+    const int g = e / kTileLanes, l = e % kTileLanes;
+    return (((g >> 2)*kTileTargets + tt)*kTileLanes + l)*4 + (g & 3);
+}
+
+// Union of a tile's list entries (list builds only): two bitmaps over the tile's index window
+// (free neighbours and walls), their prefix popcounts give each neighbour its item slot (free
+// ones first), and the targets' lists are rewritten as slots.  The same neighbour met in several
+// lists writes the same item.
+template <int REACH>
+__global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt || !D.ctl->rebuild) return;
+    __shared__ uint32_t bmf[kTileWords], bmw[kTileWords];
+    __shared__ uint16_t pf[kTileWords], pw[kTileWords];
+    __shared__ int sh[17];
+    __shared__ int jlo, jhi;
+    const int ntiles = D.ctl->ntiles, cap = P.nl_cap;
+    const size_t gs = (size_t)P.nfree;
+    // a bounded grid walks the tiles, so steps without a list build cost one wave of blocks
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int4 ti = D.tile[tile];
+        const int t0 = ti.x, nt = ti.y;
+        __syncthreads();                       // the previous tile's reads of the shared arrays
+        if (threadIdx.x == 0) { jlo = 0x7fffffff; jhi = -1; }
+        __syncthreads();
+        auto entry = [&](int k, int &tt, int &e) -> bool {
+            tt = k / cap;
+            e = k - tt*cap;
+            const int c = D.ncnt[t0 + tt];
+            return e < (c < cap ? c : cap);
+        };
+        auto list_j = [&](int tt, int e) {
+            const int v = D.nl[(((size_t)(e >> 2))*gs + (size_t)(t0 + tt))*4 + (e & 3)];
+            return nl_index<REACH>(v);
+        };
+        int lo = 0x7fffffff, hi = -1;
+        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
+            int tt, e;
+            if (!entry(k, tt, e)) continue;
+            const int j = list_j(tt, e);
+            lo = j < lo ? j : lo;
+            hi = j > hi ? j : hi;
+        }
+        if (hi >= 0) { atomicMin(&jlo, lo); atomicMax(&jhi, hi); }
+        __syncthreads();
+        const int base = jlo;
+        const int nwords = (jhi >= jlo) ? ((jhi - jlo) >> 5) + 1 : 0;
+        if (nwords > kTileWords) {
+            if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, 0, -1);
+            continue;
+        }
+        for (int w = threadIdx.x; w < nwords; w += blockDim.x) { bmf[w] = 0u; bmw[w] = 0u; }
+        __syncthreads();
+        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
+            int tt, e;
+            if (!entry(k, tt, e)) continue;
+            const int j = list_j(tt, e), b = j - base;
+            atomicOr(flag_free(D.flags_b[j]) ? &bmf[b >> 5] : &bmw[b >> 5], 1u << (b & 31));
+        }
+        __syncthreads();
+        // prefix popcounts: each thread a contiguous run of words
+        const int per = (nwords + (int)blockDim.x - 1)/(int)blockDim.x;
+        const int w0 = (int)threadIdx.x*per, w1 = min(w0 + per, nwords);
+        int cf = 0, cw = 0;
+        for (int w = w0; w < w1; ++w) { cf += __popc(bmf[w]); cw += __popc(bmw[w]); }
+        int uf = 0, uw = 0;
+        int of = block_scan(cf, sh, &uf);
+        int ow = block_scan(cw, sh, &uw);
+        if (uf + uw > P.tile_items || uw > kTileWalls) {
+            if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, 0, -1);
+            continue;
+        }
+        for (int w = w0; w < w1; ++w) {
+            pf[w] = (uint16_t)of; of += __popc(bmf[w]);
+            pw[w] = (uint16_t)ow; ow += __popc(bmw[w]);
+        }
+        __syncthreads();
+        int32_t *items = D.items + (size_t)tile*kTileItems;
+        uint16_t *tl = D.tl + (size_t)tile*P.tl_stride;
+        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
+            int tt, e;
+            if (!entry(k, tt, e)) continue;
+            const int j = list_j(tt, e), b = j - base, w = b >> 5;
+            const uint32_t below = (1u << (b & 31)) - 1u;
+            const int slot = flag_free(D.flags_b[j]) ? pf[w] + __popc(bmf[w] & below)
+                                                     : uf + pw[w] + __popc(bmw[w] & below);
+            items[slot] = j;
+            tl[tl_index(tt, e)] = (uint16_t)slot;
+        }
+        if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, uf, uw);
+    }
+}
+
+// The FP64 gather per tile.  Sums: lane l of a target adds entries l, l + kTileLanes, ... in
+// list order, the first lane starting from the target's initial acceleration; the lanes' sums
+// meet as ((l0 + l1) + (l2 + l3)) + ((l4 + l5) + (l6 + l7)).  A target whose list overflowed
+// queues the result for k_force11_nl_over, which adds the entries past the capacity.
+template <int DIM>
+__global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int nb = D.ctl->ntiles;
+    if ((int)blockIdx.x >= nb) return;
+    __shared__ double4 s0[kTileItems], s1[kTileItems], s2[kTileItems], sw[kTileWalls];
+    __shared__ double pa[64], pb[64];
+    const int tile = xcd_remap((int)blockIdx.x, nb);
+    const int4 ti = D.tile[tile];
+    const int t0 = ti.x, nt = ti.y, uf = ti.z, uw = ti.w;
+    const bool staged = uw >= 0;
+    const int tid = threadIdx.x;
+    if (tid < 64) { pa[tid] = P.pair_alpha[tid]; pb[tid] = P.pair_beta[tid]; }
+    const int32_t *items = D.items + (size_t)tile*kTileItems;
+    // the target's own records first, so their loads are in flight during the staging
+    const int tt = tid / kTileLanes, l = tid % kTileLanes;
+    const bool act = tt < nt;
+    const int t = t0 + (act ? tt : 0);
+    const int i = D.free_list[t];
+    const double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i];
+    const int cap = P.nl_cap;
+    const int cnt = act ? D.ncnt[t] : 0;
+    const int m = cnt < cap ? cnt : cap;
+    const uint64_t *tq = reinterpret_cast<const uint64_t *>(D.tl + (size_t)tile*P.tl_stride) +
+                         tt*kTileLanes + l;
+    constexpr size_t kWordStride = (size_t)kTileTargets*kTileLanes;
+    // the lane's first kSlotWords slot words (its entries l + 8 g, g < 4 kSlotWords) are loaded
+    // here, in flight during the staging; longer lists read the rest in the loop
+    uint64_t qw[kSlotWords];
+#pragma unroll
+    for (int G = 0; G < kSlotWords; ++G)
+        qw[G] = (staged && l + 4*kTileLanes*G < m) ? tq[G*kWordStride] : 0ull;
+    if (staged) {
+        // at most two items per thread (kTileItems <= 2 blockDim.x): both rounds' loads are
+        // issued before the LDS stores
+        const int U = uf + uw, u0 = tid, u1 = tid + (int)blockDim.x;
+        const int j0 = u0 < U ? items[u0] : -1, j1 = u1 < U ? items[u1] : -1;
+        double4 r00, r01, r02, r0w, r10, r11, r12, r1w;
+        if (j0 >= 0) { r00 = D.g0[j0]; r01 = D.g1[j0]; r02 = D.g2[j0]; if (u0 >= uf) r0w = D.gw[j0]; }
+        if (j1 >= 0) { r10 = D.g0[j1]; r11 = D.g1[j1]; r12 = D.g2[j1]; if (u1 >= uf) r1w = D.gw[j1]; }
+        if (j0 >= 0) { s0[u0] = r00; s1[u0] = r01; s2[u0] = r02; if (u0 >= uf) sw[u0 - uf] = r0w; }
+        if (j1 >= 0) { s0[u1] = r10; s1[u1] = r11; s2[u1] = r12; if (u1 >= uf) sw[u1 - uf] = r1w; }
+    }
+    __syncthreads();
+    const int ci = (int)gi2.y;
+    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
+    Acc11 A;
+    {
+        const double4 a_init = (act && l == 0) ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
+        A.a0 = a_init.x; A.a1 = a_init.y; A.a2 = a_init.z; A.dd = a_init.w;
+    }
+    unsigned nff = 0, nfw = 0;
+    if (staged) {
+        uint64_t q = 0ull;
+        for (int e = l, g = 0; e < m; e += kTileLanes, ++g) {
+            if ((g & 3) == 0) {
+                // this lane's next four slots (g is the same in every lane of the wave)
+                const int G = g >> 2;
+                q = qw[0];
+#pragma unroll
+                for (int k = 1; k < kSlotWords; ++k) q = (G == k) ? qw[k] : q;
+                if (G >= kSlotWords) q = tq[(size_t)G*kWordStride];
+            }
+            const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
+            // all four records read at once; the wall record's slot is clamped for free ones
+            const double4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
+            const double4 w = sw[slot >= uf ? slot - uf : 0];
+            const double4 vw = slot >= uf ? w : a1;
+            pair11_rec<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
+                                  a0, a1, a2, vw, mu_ff, mu_w, A, nff, nfw);
+        }
+    } else {
+        const size_t gs = (size_t)P.nfree;
+        for (int e = l; e < m; e += kTileLanes) {
+            const int j = nl_index<2>(D.nl[(((size_t)(e >> 2))*gs + (size_t)t)*4 + (e & 3)]);
+            pair11_fast<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, mu_ff, mu_w, A, nff, nfw);
+        }
+    }
+    // the lanes' sums meet in a fixed order
+    for (int o = 1; o < kTileLanes; o <<= 1) {
+        A.a0 += __shfl_xor(A.a0, o, 64);
+        A.a1 += __shfl_xor(A.a1, o, 64);
+        A.a2 += __shfl_xor(A.a2, o, 64);
+        A.dd += __shfl_xor(A.dd, o, 64);
+    }
+    unsigned long long dtb = ~0ull;
+    if (act && l == 0) {
+        if (cnt > cap) {
+            int q = atomicAdd(&D.ctl->nl_q, 1);
+            D.ovq[q] = t;
+            D.ovs[2*q] = make_double4(A.a0, A.a1, A.a2, A.dd);
+        } else {
+            D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
+            double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
+            double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+            dtb = dbits(cand);
+        }
+    }
+    force_partials(D, dtb, nff, nfw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2805,6 +3093,14 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
 {
     const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
     hipLaunchKernelGGL((k_nlist<DIM, REACH>), gl, dim3(kBlock), 0, s, D, P);
+    if (REACH == 2 && P.tiles) {
+        const dim3 gt(P.tile_max);
+        hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, D, P);
+        hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
+        hipLaunchKernelGGL((k_force11_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+        hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        return;
+    }
     if (P.mixed32) {
         hipLaunchKernelGGL((k_force11_f32_nl<DIM, REACH>), gf, dim3(kForceBlock), 0, s, D, P);
         hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);

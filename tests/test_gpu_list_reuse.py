@@ -149,3 +149,27 @@ def test_deletion_with_reused_lists(monkeypatch):
     np.testing.assert_array_equal(G["ID"], O["ID"])
     np.testing.assert_allclose(G["x"], O["x"], rtol=0, atol=1e-10)
     g.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_tile_gather_matches_list_gather(monkeypatch, case):
+    """The FP64 gather per tile (neighbour records staged in LDS, eight lanes per target) against
+    the per-target list gather (PSPH_NO_TILES) and against tiles that mostly read their
+    neighbours from HBM (PSPH_TILE_ITEMS=64: few tiles fit): the same pairs, and states that
+    agree to the rounding of the regrouped sums."""
+    cfg, p, dt = CASES[case]()
+    steps = 120
+    res = {}
+    for name, env in (("tiles", {}), ("lists", {"PSPH_NO_TILES": "1"}), ("fallback", {"PSPH_TILE_ITEMS": "64"})):
+        for k in ("PSPH_NO_TILES", "PSPH_TILE_ITEMS"):
+            monkeypatch.delenv(k, raising=False)
+        print(f"[tiles] {case} {name}", flush=True)
+        g = _solver(monkeypatch, env, cfg, p, dt)
+        st, builds = _run(g, steps, chunk=40)
+        res[name] = (st, g.particles())
+        g.close()
+    st0, R = res["lists"]
+    for name in ("tiles", "fallback"):
+        st, G = res[name]
+        assert st.steps == st0.steps and st.pairs == st0.pairs, name
+        _close(G, R)
