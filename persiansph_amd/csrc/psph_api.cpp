@@ -1561,9 +1561,10 @@ int psph_hook_accel(psph_handle *h, const double *a, const double *ddensity, siz
         if (rc) return rc;
         h->hook_cap = 4*n;
     }
-    HIPCHECK(hipMemcpyAsync(h->hook_dev, a, sizeof(double)*3*n, hipMemcpyHostToDevice, h->stream));
+    // blocking copies from the caller's (pageable) arrays; the stream is idle at a stop
+    HIPCHECK(hipMemcpy(h->hook_dev, a, sizeof(double)*3*n, hipMemcpyHostToDevice));
     if (ddensity)
-        HIPCHECK(hipMemcpyAsync(h->hook_dev + 3*n, ddensity, sizeof(double)*n, hipMemcpyHostToDevice, h->stream));
+        HIPCHECK(hipMemcpy(h->hook_dev + 3*n, ddensity, sizeof(double)*n, hipMemcpyHostToDevice));
     launch_hook_accel(h->D, h->P, h->hook_dev, ddensity ? h->hook_dev + 3*n : nullptr, h->stream);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1686,7 +1687,8 @@ static int pair_query(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs
     const bool fits_staging = !staging || (size_t)tot <= staging_pairs;
     if (pairs && (size_t)tot <= cap && tot > 0 && fits_staging) {
         long long *out_dev = staging;
-        HIPCHECK(hipMemcpyAsync(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice, h->stream));
+        // blocking copy from the pageable offsets; the stream is idle here
+        HIPCHECK(hipMemcpy(h->pair_off, off.data(), sizeof(long long)*P.n, hipMemcpyHostToDevice));
         if (!staging) HIPCHECK(hipMalloc((void **)&out_dev, sizeof(long long)*2*tot));
         // k_pairs writes every one of the 2*tot values: nothing left from an earlier call is read
         launch_pairs(D, P, h->stream, 1, h->pair_off, nullptr, out_dev);

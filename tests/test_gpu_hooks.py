@@ -90,10 +90,11 @@ def test_hooks_against_oracle(which, scheme):
     np.testing.assert_allclose(G["x"], O_["x"], rtol=0, atol=1e-11)
     np.testing.assert_allclose(G["v"], O_["v"], rtol=0, atol=1e-9)
     np.testing.assert_allclose(G["Density"], O_["Density"], rtol=1e-11, atol=0)
-    # the hooks changed the run
+    # the hooks changed the run, by far more than the parity bar on positions above (30 steps
+    # of 35 us: the after hook's damping moves particles by some 1e-8 m)
     plain = Oracle(cfg, p, dt)
     plain.run(steps)
-    assert np.abs(plain.particles()["x"] - O_["x"]).max() > 1e-6
+    assert np.abs(plain.particles()["x"] - O_["x"]).max() > 100*1e-11
 
 
 def test_hooks_with_output_stops():
@@ -144,3 +145,19 @@ def test_hook_accel_needs_a_stop():
     with pytest.raises(Exception) as e:
         s.hook_accel(q["a"])
     assert "PSPH_ERR_STATE" in str(e.value)
+
+
+def test_repeated_hooked_runs_are_identical():
+    """Values handed back at a stop reach the device in full before the step resumes: identical
+    hooked runs give identical states, bit for bit."""
+    cfg, p, dt = cases.dam_break_2d(dx=0.05, scheme=0)
+    out = []
+    for _ in range(3):
+        g = Solver(cfg, p, dt)
+        g.set_hooks(True, True)
+        assert _hooked_run(g, 12, before_hook, after_hook) == 12
+        out.append(g.particles())
+        g.close()
+    for q in out[1:]:
+        for f in ("x", "v", "Density"):
+            np.testing.assert_array_equal(q[f], out[0][f], err_msg=f)
