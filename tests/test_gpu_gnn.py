@@ -2,9 +2,11 @@
 same network (oracle/gnn_ref.py): the radius graph, the forward pass (BF16 matrix-core
 GEMMs with FP32 accumulation), the backward pass, the Adam update and a short fit.
 
-Tolerances: activations are rounded to BF16 (8 significant bits) at every layer, so the
-forward output is held to 3% of its norm and each layer's gradient to a cosine similarity of
-0.99 and 10% of its norm, against FP64 on identical parameters and inputs."""
+Tolerances: against FP64 on identical parameters and inputs, where activations are rounded to
+BF16 (8 significant bits) at every layer, the forward output is held to 3% of its norm and each
+layer's gradient to 0.15 of its norm; against the BF16-rounding reference (gnn_ref.forward_bf16,
+gradient_bf16) to 1e-3 (forward) and 1e-2 (gradients) up to 3 layers and to the reference's own
+accumulation-order drift at 10; on problems where BF16 storage is exact, bit for bit."""
 import numpy as np
 import pytest
 
@@ -74,6 +76,28 @@ def test_forward_matches_reference(dim, layers):
     g.close()
 
 
+@pytest.mark.parametrize("dim,layers", [(2, 2), (3, 3), (2, 10), (3, 10)])
+def test_forward_matches_bf16_reference(dim, layers):
+    """Against a reference that rounds to BF16 where the kernels do (gnn_ref.forward_bf16), the
+    forward pass agrees to the FP32 accumulation order of the matrix cores: 1e-3 of the output
+    norm up to 3 message-passing layers, 1e-2 at 10 (the FP64 reference above is held to 3e-2).
+    Two CPU forwards that differ only in that order drift apart by a few 1e-3 at 10 layers
+    (test_gnn_ref.test_bf16_forward_sensitivity_to_accumulation_order), so no tighter bar holds
+    at that depth."""
+    n = 300
+    x, rp, cols, recv, X, U = _inputs(n, dim, 0.6, 3)
+    print(f"[gnn fwd bf16] dim={dim} layers={layers} n={n} edges={len(cols)}", flush=True)
+    g = _net(dim, layers, train=False)
+    g.load_inputs(X, U, rp, cols)
+    y = g.predict()
+    p = g.get_params()
+    y0 = R.forward_bf16(p, layers, X, U, recv, cols.astype(np.int64), 2*dim + 1)
+    err = float(np.linalg.norm(y - y0)/np.linalg.norm(y0))
+    print(f"[gnn fwd bf16] relative error {err:.3e}", flush=True)
+    assert err < (1e-3 if layers <= 3 else 1e-2), err
+    g.close()
+
+
 @pytest.mark.parametrize("dim,layers", [(2, 2), (3, 3)])
 def test_fused_inference_equals_training_forward(dim, layers):
     """Inference runs each edge update as one fused kernel (the first layer's activations stay
@@ -96,8 +120,16 @@ def test_fused_inference_equals_training_forward(dim, layers):
     gi.close()
 
 
-@pytest.mark.parametrize("dim,layers", [(2, 2), (3, 2)])
+@pytest.mark.parametrize("dim,layers", [(2, 2), (3, 2), (3, 10)])
 def test_gradients_match_reference(dim, layers):
+    """Gradients against the BF16-rounding reference (gnn_ref.gradient_bf16, which rounds where
+    the training kernels do).  Up to 3 layers: every parameter block within 1e-2 of its norm
+    (two FP32 accumulation orders of that reference differ by ~5e-4).  At 10 layers the BF16
+    gradient is itself sensitive to the accumulation order (rounding flips compound through
+    the layers: the reference's two orders differ by several per cent per block), so each block
+    is held to three times that measured drift of the reference on the same inputs (at least
+    1e-2), and the whole vector likewise.  The FP64 reference (gnn_ref.gradient) is held to the
+    BF16 precision of the kernels: 0.15 per block, cosine above 0.99."""
     n = 250
     x, rp, cols, recv, X, U = _inputs(n, dim, 0.6, 4)
     rng = np.random.default_rng(5)
@@ -110,22 +142,61 @@ def test_gradients_match_reference(dim, layers):
     g.set_targets(tgt, wgt)
     loss = g.train_step(0.0, update=False)
     gg = g.grads().astype(np.float64)
-    p = g.get_params().astype(np.float64)
-    L0, g0 = R.gradient(p, layers, _bf16(X), _bf16(U), recv, cols.astype(np.int64), tgt.astype(np.float64),
+    p = g.get_params()
+    send = cols.astype(np.int64)
+    Lb, gb = R.gradient_bf16(p, layers, X, U, recv, send, tgt, wgt)
+    _, gb2 = R.gradient_bf16(p, layers, X, U, recv, send, tgt, wgt, f32_products=True)
+    L0, g0 = R.gradient(p.astype(np.float64), layers, _bf16(X), _bf16(U), recv, send, tgt.astype(np.float64),
                         wgt.astype(np.float64))
+    assert abs(loss - Lb) <= 1e-3*Lb, (loss, Lb)
     assert abs(loss - L0) <= 3e-2*L0, (loss, L0)
+    deep = layers > 3
+    worst = 0.0
     o = 0
     for li, k in enumerate(R.layer_inputs(layers)):
         for size in (k*R.H, R.H):
-            a, b = gg[o:o + size], g0[o:o + size]
+            a, b, b2, c = gg[o:o + size], gb[o:o + size], gb2[o:o + size], g0[o:o + size]
             o += size
             nb = np.linalg.norm(b)
             if nb == 0.0:
                 assert np.linalg.norm(a) < 1e-6
                 continue
-            cos = float(a @ b)/(np.linalg.norm(a)*nb)
             rel = np.linalg.norm(a - b)/nb
-            assert cos > 0.99 and rel < 0.1, (li, size, cos, rel)
+            bar = max(3.0*np.linalg.norm(b2 - b)/nb, 1e-2) if deep else 1e-2
+            worst = max(worst, rel/bar)
+            assert rel < bar, (li, size, rel, bar)
+            cos = float(a @ c)/(np.linalg.norm(a)*np.linalg.norm(c))
+            assert cos > 0.99 and np.linalg.norm(a - c)/np.linalg.norm(c) < 0.15, (li, size, cos)
+    glob = np.linalg.norm(gg - gb)/np.linalg.norm(gb)
+    gbar = max(3.0*np.linalg.norm(gb2 - gb)/np.linalg.norm(gb), 1e-2) if deep else 1e-2
+    print(f"[gnn grad] vs bf16 reference: whole vector {glob:.3e} (bar {gbar:.3e}), worst block at "
+          f"{worst:.2f} of its bar", flush=True)
+    assert glob < gbar
+    g.close()
+
+
+@pytest.mark.parametrize("layers,seed,dens", [(2, 0, 0.01), (2, 4, 0.01), (10, 3, 0.006), (10, 1, 0.006)])
+def test_bf16_exact_problem_matches_reference_to_the_bit(layers, seed, dens):
+    """On problems where every stored value fits BF16 (gnn_ref.bf16_exact_problem: sparse +-1
+    weights, inputs in halves, a ring graph), every FP32 accumulation is exact in any order, so
+    the matrix-core forward and training gradients equal the BF16-rounding reference
+    (gnn_ref.forward_bf16, gradient_bf16) bit for bit, 10 message-passing layers included."""
+    dim = 2
+    dout = 2*dim + 1
+    p, X, U, rp, cols, recv, send, tgt, wgt = R.bf16_exact_problem(seed, layers, dout, density=dens)
+    print(f"[gnn exact] layers={layers} seed={seed} n={len(X)} edges={len(cols)}", flush=True)
+    g = _net(dim, layers)
+    g.set_params(p)
+    g.load_inputs(X, U, rp, cols)
+    y = g.predict()
+    np.testing.assert_array_equal(y, R.forward_bf16(p, layers, X, U, recv, send, dout))
+    g.set_targets(tgt, wgt)
+    loss = g.train_step(0.0, update=False)
+    gg = g.grads()
+    Lb, gb = R.gradient_bf16(p, layers, X, U, recv, send, tgt, wgt)
+    assert np.count_nonzero(gb) > 4000
+    assert abs(loss - Lb) <= 1e-6*Lb, (loss, Lb)
+    np.testing.assert_array_equal(gg, gb.astype(np.float32))
     g.close()
 
 
