@@ -24,7 +24,10 @@ extern "C" {
 /* ---- status codes (0 = ok, negative = error; message via psph_last_error) ---- */
 #define PSPH_OK               0
 #define PSPH_ERR_INVALID     -1  /* bad argument or enum value (reference: abort() in Functions.cpp switches) */
-#define PSPH_ERR_UNSUPPORTED -2  /* feature outside this path (in-out-flow BCs, FSI, multi-material pairs) */
+#define PSPH_ERR_UNSUPPORTED -2  /* feature outside this path: in/out-flow with non-fluid particles or
+                                    the Shepard filter, solid and soil particles in one run, FP32-mixed
+                                    runs of several materials, a 2-D periodic z, a second Solve on a
+                                    preset box (the message names which) */
 #define PSPH_ERR_HIP         -3  /* HIP runtime failure */
 #define PSPH_ERR_TIMESTEP    -4  /* adaptive dt collapsed below deltatint/1e5 (Domain.cpp:150-151) */
 #define PSPH_ERR_CFL         -5  /* user dt above the CFL/viscous limit (Domain.cpp:1721-1733) */
@@ -61,8 +64,10 @@ typedef struct psph_config {
     int32_t precision;       /* PSPH_FP64 or PSPH_FP32_MIXED */
     int32_t periodic[3];     /* Boundary::Periodic (z only in 3-D; needs 3+ cells across a period) */
     int32_t in_out_flow;     /* Boundary::InOutFlow: 1 inflow, 2 outflow, 3 both (x direction) */
-    int32_t swi_type;        /* Domain::SWIType (only the single-material meaning is used) */
-    int32_t fsi;             /* Domain::FSI (nonzero -> PSPH_ERR_UNSUPPORTED) */
+    int32_t swi_type;        /* Domain::SWIType: the soil-water interaction of multi-material runs
+                                (0-3, CalcForce13 and the saturation of Domain.cpp) */
+    int32_t fsi;             /* Domain::FSI: nonzero enables the fluid-solid interaction sums
+                                (CalcForce12 with the FSI pressure and no-slip velocity) */
     double  cellfac;         /* Domain::Cellfac (2 for kernels 0/1, 3 for kernel 2) */
     double  gravity[3];      /* Domain::Gravity */
     double  xsph;            /* Domain::XSPH */

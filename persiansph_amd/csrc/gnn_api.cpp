@@ -363,10 +363,12 @@ int psph_gnn_load_state(psph_gnn *g, const double *x, const double *v, const dou
     int rc = ensure_nodes(g, n);
     if (rc) return rc;
     g->n = n;
-    HC(hipMemcpyAsync(g->x, x, sizeof(double)*3*n, hipMemcpyHostToDevice, g->s));
-    HC(hipMemcpyAsync(g->v, v, sizeof(double)*3*n, hipMemcpyHostToDevice, g->s));
-    HC(hipMemcpyAsync(g->rho, rho, sizeof(double)*n, hipMemcpyHostToDevice, g->s));
-    HC(hipMemcpyAsync(g->isfree, isfree, sizeof(int32_t)*n, hipMemcpyHostToDevice, g->s));
+    // blocking copies from the caller's (pageable) arrays, after the stream's earlier work
+    HC(hipStreamSynchronize(g->s));
+    HC(hipMemcpy(g->x, x, sizeof(double)*3*n, hipMemcpyHostToDevice));
+    HC(hipMemcpy(g->v, v, sizeof(double)*3*n, hipMemcpyHostToDevice));
+    HC(hipMemcpy(g->rho, rho, sizeof(double)*n, hipMemcpyHostToDevice));
+    HC(hipMemcpy(g->isfree, isfree, sizeof(int32_t)*n, hipMemcpyHostToDevice));
     g->wsum = 0.0;
     for (int64_t i = 0; i < n; ++i) g->wsum += isfree[i] ? 1.0 : 0.0;
     rc = build_graph(g, x);

@@ -32,6 +32,7 @@ void launch_cellsort(const Dev &D, const Params &P, hipStream_t s);
 void launch_reorder(const Dev &D, const Params &P, hipStream_t s);
 void launch_wall(const Dev &D, const Params &P, hipStream_t s, int after_hook);
 void launch_hook_accel(const Dev &D, const Params &P, const double *a, const double *dd, hipStream_t s);
+void launch_fixed_sums(const Dev &D, const Params &P, hipStream_t s);
 void launch_force(const Dev &D, const Params &P, hipStream_t s);
 void launch_integrate(const Dev &D, const Params &P, hipStream_t s);
 void launch_tail(const Dev &D, const Params &P, hipStream_t s);
@@ -127,6 +128,7 @@ struct psph_handle {
     bool broken = false;
     psph_grid grid{};
     bool trans_ok = false;             // transient columns match the current order
+    bool fixed_ok = false;             // fixed particles' pair sums formed for this stop
     double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
     bool hook_b = false, hook_a = false;   // host hook stops (psph_set_hooks)
     int regrowths = 0;                 // device buffers re-created for more particles (in/out-flow)
@@ -602,6 +604,8 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
         }
     }
     const Cols &D = h->D.buf[buf];
+    // blocking copies on the null stream do not wait for the (non-blocking) solver stream
+    HIPCHECK(hipStreamSynchronize(h->stream));
     HIPCHECK(hipMemcpy(D.pos, pos.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.vel, vel.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.vh, vh.data(), sizeof(double4)*n, hipMemcpyHostToDevice));
@@ -621,12 +625,29 @@ static int write_state(psph_handle *h, const psph_particle *p, size_t n, int buf
 
 // Reads the current device state into h->host (reference order by the orig column).
 // After a completed step the transient columns share the current order.
+// The fixed particles' pair sums of a step stopped after its force pass (outputs and the
+// GeneralAfter hook), formed once per stop: they feed nothing in the dynamics, so only a state
+// the host reads carries them.  The control block on the host must be current.
+static int form_fixed_sums(psph_handle *h)
+{
+    if (h->fixed_ok || !h->trans_ok || h->ctl_host->mid_step != 1) return 0;
+    launch_fixed_sums(h->D, h->P, h->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    h->fixed_ok = true;
+    return 0;
+}
+
 static int read_state(psph_handle *h)
 {
     Params &P = h->P;
     size_t n = (size_t)P.n;
     HIPCHECK(hipStreamSynchronize(h->stream));
     HIPCHECK(hipMemcpy(h->ctl_host, h->ctl_dev, sizeof(Ctl), hipMemcpyDeviceToHost));
+    {
+        int rc0 = form_fixed_sums(h);
+        if (rc0) return rc0;
+    }
     const Dev D = with_state(h, state_buffer(h));
     std::vector<double4> pos(n), vel(n), vh(n), aux(n), wal(n), acc(n), xsp(n);
     std::vector<double2> mmu(n), shep(n);
@@ -703,8 +724,14 @@ static int read_state(psph_handle *h)
                 if (P.any_xsph) { q.VXSPH[0] = xsp[i].x; q.VXSPH[1] = xsp[i].y; q.VXSPH[2] = xsp[i].z; }
                 else { q.VXSPH[0] = q.VXSPH[1] = q.VXSPH[2] = 0.0; }
                 if (shep_on) { q.ZWab = xsp[i].w; q.SumDen = shep[i].x; }
+            } else if (h->fixed_ok) {
+                // stopped after the force pass: the pair sums of the fixed particle (k_fixed_sums)
+                q.a[0] = acc[i].x; q.a[1] = acc[i].y; q.a[2] = acc[i].z; q.dDensity = acc[i].w;
+                q.NSv[0] = wal[i].x; q.NSv[1] = wal[i].y; q.NSv[2] = wal[i].z;
+                q.SumKernel = shep[i].y;
             } else {
-                // the pair sums of fixed particles are not formed; their a stays the gravity reset
+                // between steps the pair sums of fixed particles are not held; a stays the
+                // gravity reset
                 for (int c = 0; c < 3; ++c) q.a[c] = h->cfg.gravity[c];
                 q.dDensity = 0.0;
                 q.NSv[0] = wal[i].x; q.NSv[1] = wal[i].y; q.NSv[2] = wal[i].z;
@@ -1076,6 +1103,7 @@ static int write_back(psph_handle *h)
     h->ctl_host->leave = 0;
     h->ctl_host->halt = kRun;
     h->trans_ok = false;
+    h->fixed_ok = false;
     return prime_first_step(h);
 }
 
@@ -1417,6 +1445,7 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     if (!h) return fail(PSPH_ERR_INVALID, "null handle");
     if (!h->uploaded) return fail(PSPH_ERR_STATE, "psph_run before psph_upload");
     if (h->broken) return fail(PSPH_ERR_STATE, "the solver stopped on an earlier error: %s", g_err.c_str());
+    h->fixed_ok = false;
     psph_status s;
     std::memset(&s, 0, sizeof s);
     // read the control block, set the run limits
@@ -1579,6 +1608,8 @@ int psph_output_begin(psph_handle *h)
     int rc = sync_ctl(h);
     if (rc) return rc;
     if (h->ctl_host->mid_step == 2) return fail(PSPH_ERR_STATE, "output at a stop before the pair sums");
+    rc = form_fixed_sums(h);
+    if (rc) return rc;
     const Params &P = h->P;
     const size_t n = (size_t)P.n, nf = (size_t)output_floats();
     if (n > h->out_cap) {
@@ -1597,7 +1628,7 @@ int psph_output_begin(psph_handle *h)
     if (!h->out_event) HIPCHECK(hipEventCreateWithFlags(&h->out_event, hipEventDisableTiming));
     const Dev D = with_state(h, state_buffer(h));
     const bool acc_dev = h->trans_ok, tensors_dev = P.any_solid != 0;
-    launch_pack_output(D, P, h->out_dev, acc_dev ? 1 : 0, tensors_dev ? 1 : 0, h->stream);
+    launch_pack_output(D, P, h->out_dev, acc_dev ? (h->fixed_ok ? 2 : 1) : 0, tensors_dev ? 1 : 0, h->stream);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(h->out_host, h->out_dev, sizeof(float)*nf*n, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipEventRecord(h->out_event, h->stream));
@@ -1709,6 +1740,7 @@ static int pair_query(psph_handle *h, int64_t *pairs, size_t cap, size_t *npairs
     if (rc) return rc;
     HIPCHECK(hipStreamSynchronize(h->stream));
     h->trans_ok = false;
+    h->fixed_ok = false;
     if (pairs && (size_t)tot > cap) return fail(PSPH_ERR_CAPACITY, "need room for %lld pairs", tot);
     if (pairs && !fits_staging)
         return fail(PSPH_ERR_CAPACITY, "%lld pairs exceed the %zu that the no-allocation pair query "

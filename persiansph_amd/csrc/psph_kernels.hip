@@ -910,8 +910,11 @@ struct AccT {
 using Acc = AccT<double>;
 
 // CalcForce11 (Interaction.cpp:25-184) contribution of neighbour j to free particle i as P1.
+// tw (when not null) receives the pair's temp[3], temp1, mi, di and dj: what the reference adds to
+// P2 as P2->a -= mi temp, P2->dDensity += mi (dj/di) temp1 (Interaction.cpp:166-167).
 __device__ __forceinline__ void pair11(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
-                              uint32_t fi, double4 pi, double4 vi, double4 ai, Acc &A, int shep_on)
+                              uint32_t fi, double4 pi, double4 vi, double4 ai, Acc &A, int shep_on,
+                              double *tw = nullptr)
 {
     uint32_t fj = D.flags_b[j];
     int ci = flag_cls(fi), cj = flag_cls(fj);
@@ -997,6 +1000,7 @@ __device__ __forceinline__ void pair11(const Dev &D, const Params &P, const Clas
     if (P.dim == 2) t2 = 0.0;
     double g0 = GK*x0, g1 = GK*x1, g2 = GK*x2;
     double temp1 = u0*g0 + u1*g1 + u2*g2;
+    if (tw) { tw[0] = t0; tw[1] = t1; tw[2] = t2; tw[3] = temp1; tw[4] = mi; tw[5] = di; tw[6] = dj; }
     A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
     A.dd += mj * (di/dj) * temp1;
     if (sr_on && (Ci_.T0 > 0.0 || Ci_.LES)) {
@@ -1017,7 +1021,7 @@ __device__ __forceinline__ void pair11(const Dev &D, const Params &P, const Clas
 template <class T>
 __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                                 uint32_t fi, double4 pi, double4 vi, double4 ai, const double *sgi,
-                                const double *tri, AccT<T> &A, int shep_on)
+                                const double *tri, AccT<T> &A, int shep_on, double *tw = nullptr)
 {
     uint32_t fj = D.flags_b[j];
     int ci = flag_cls(fi), cj = flag_cls(fj);
@@ -1113,6 +1117,10 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     T t2 = g0*Mf[2] + g1*Mf[5] + g2*Mf[8];
     if (P.dim == 2) t2 = (T)0.0;
     T temp1 = u0*g0 + u1*g1 + u2*g2;
+    if (tw) {
+        tw[0] = (double)t0; tw[1] = (double)t1; tw[2] = (double)t2; tw[3] = (double)temp1;
+        tw[4] = (double)mi; tw[5] = (double)di; tw[6] = (double)dj;
+    }
     A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
     if constexpr (sizeof(T) == 4) {
         T c = mj * (di/dj) * temp1;
@@ -2905,6 +2913,71 @@ __global__ void __launch_bounds__(kBlock) k_hook_accel(Dev D0, Params P, const d
     D.acc[i] = v;
 }
 
+// The pair sums of fixed particles (read at a stop after the force pass: outputs and the
+// GeneralAfter hook).  The reference adds every pair into both particles, so a fixed particle
+// P2 of a pair with free P1 gets a -= mi temp, dDensity += mi (dj/di) temp1 on top of the
+// StartAcceleration reset a = Gravity (Interaction.cpp:166-167, 339-340; Domain.cpp:1002).
+// They feed nothing in the dynamics; the force pass does not form them.  Pairs of the fixed
+// particle's own material, found by the reference's test (neighbouring cells, rij/h <=
+// Cellfac), each evaluated by the free particle's pair function; written to acc of the fixed
+// particle.  Pairs with free particles of another material (CalcForce12/13) are not added.
+template <bool SOLID>
+__global__ void __launch_bounds__(kBlock) k_fixed_sums(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    __shared__ ClassParams cp[kMaxClasses];
+    for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
+    __syncthreads();
+    const int w = blockIdx.x*kBlock + threadIdx.x;
+    if (w >= P.n) return;
+    const uint32_t fw = D.flags_b[w];
+    if (flag_free(fw)) return;
+    const int mat = flag_mat(fw), cw = flag_cls(fw);
+    const double4 pw = D.pos_b[w];
+    int kx, ky, kz;
+    decode_key(P, D.key[w], kx, ky, kz);
+    double a0 = P.gravity[0], a1 = P.gravity[1], a2 = P.gravity[2], dd = 0.0;
+    const int nrows = (P.dim == 3) ? 9 : 3;
+    for (int rs = 0; rs < 2*nrows; ++rs) {
+        int beg = 0, end = 0;
+        if (P.any_periodic) {
+            int b0, e0, b1, e1;
+            row_range_periodic(D, P, kx, ky, kz, rs >> 1, b0, e0, b1, e1);
+            beg = (rs & 1) ? b1 : b0;
+            end = (rs & 1) ? e1 : e0;
+        } else if (!(rs & 1)) {
+            row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+        }
+        for (int j = beg; j < end; ++j) {
+            const uint32_t fj = D.flags_b[j];
+            if (!flag_free(fj) || flag_mat(fj) != mat) continue;
+            const double4 pj = D.pos_b[j];
+            const double h = pair_h(cp, P, flag_cls(fj), cw);
+            double x0 = pj.x - pw.x, x1 = pj.y - pw.y, x2 = pj.z - pw.z;
+            if (P.any_periodic) periodic_xij(D, P, h, D.key[j], w, x0, x1, x2);
+            if (sqrt(x0*x0 + x1*x1 + x2*x2)/h > P.cellfac) continue;
+            const double4 vj = D.vel_b[j], aj = D.aux[j];
+            Acc A;
+            A.a0 = A.a1 = A.a2 = A.dd = A.ddc = 0.0;
+            A.vx0 = A.vx1 = A.vx2 = 0.0; A.zw = 0.0; A.sd = 0.0; A.sw = 0.0;
+            for (int k = 0; k < 6; ++k) A.sr[k] = 0.0;
+            A.rr[0] = A.rr[1] = A.rr[2] = 0.0;
+            double tw[7];
+            if (SOLID && mat != 1) {
+                double sgj[6], trj[6];
+                ld6(D.sig_b, j, sgj);
+                ld6(D.tir, j, trj);
+                pair2233<double>(D, P, cp, j, w, fj, pj, vj, aj, sgj, trj, A, 0, tw);
+            } else {
+                pair11(D, P, cp, j, w, fj, pj, vj, aj, A, 0, tw);
+            }
+            a0 -= tw[4]*tw[0]; a1 -= tw[4]*tw[1]; a2 -= tw[4]*tw[2];
+            dd += tw[4]*(tw[6]/tw[5])*tw[3];
+        }
+    }
+    D.acc[w] = make_double4(a0, a1, a2, dd);
+}
+
 // ------------------------------------------------------------------------------------------
 // test hook: the neighbour pair set with the reference's orientation (MainNeighbourSearch
 // Domain.cpp:808-945 + the rij/h <= Cellfac test of CalcForce11/2233).  Pass 0 counts pairs
@@ -2982,7 +3055,8 @@ __global__ void __launch_bounds__(kBlock) k_pack_output(Dev D, Params P, float *
     vo[0] = (float)v.x; vo[1] = (float)v.y; vo[2] = (float)v.z;
     if (acc_ok) {
         float *ao = out + 6*n + 3*o;
-        if (flag_free(f)) {
+        if (flag_free(f) || acc_ok == 2) {
+            // acc_ok == 2: the fixed particles' pair sums are formed (k_fixed_sums)
             const double4 a = D.acc[i];
             ao[0] = (float)a.x; ao[1] = (float)a.y; ao[2] = (float)a.z;
         } else {
@@ -3142,6 +3216,12 @@ void launch_integrate(const Dev &D, const Params &P, hipStream_t s)
 
 void launch_tail(const Dev &D, const Params &P, hipStream_t s)
 { hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, s, D, P); }
+
+void launch_fixed_sums(const Dev &D, const Params &P, hipStream_t s)
+{
+    if (P.any_solid) hipLaunchKernelGGL(k_fixed_sums<true>, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+    else hipLaunchKernelGGL(k_fixed_sums<false>, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P);
+}
 
 void launch_hook_accel(const Dev &D, const Params &P, const double *a, const double *dd, hipStream_t s)
 { hipLaunchKernelGGL(k_hook_accel, dim3(nblk(P.n, kBlock)), dim3(kBlock), 0, s, D, P, a, dd); }

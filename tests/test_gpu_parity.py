@@ -83,6 +83,14 @@ def test_first_force_pass(case):
     _check_metric("wall Pressure", G["Pressure"][fx], O["Pressure"][fx], sc[fx, 4])
     _check_metric("wall NSv", G["NSv"][fx], O["NSv"][fx], sc[fx, 5:8])
     np.testing.assert_array_equal(G["SumKernel"][fx] > 0, O["SumKernel"][fx] > 0)
+    # the walls' own pair sums (Interaction.cpp:166-167): Gravity plus their pairs' share
+    _check_metric("wall a", G["a"][fx], O["a"][fx], sc[fx, 0:3])
+    _check_metric("wall dDensity", G["dDensity"][fx], O["dDensity"][fx], sc[fx, 3])
+    assert np.abs(O["a"][fx] - np.array(cfg.gravity)).max() > 1.0
+    # the packed output carries the same walls' accelerations
+    g.output_begin()
+    out = g.output_end()
+    np.testing.assert_array_equal(out["acceleration"], G["a"].astype(np.float32))
 
 
 @pytest.mark.parametrize("case", list(CASES))

@@ -41,9 +41,12 @@ def test_soil_first_force_pass(dim, scheme):
     _check("StrainRate", G["StrainRate"][fr][:, sr_idx], O_["StrainRate"][fr][:, sr_idx], sc[fr, 9:15], 1e-12)
     rr_idx = [1, 2, 5]
     _check("RotationRate", G["RotationRate"][fr][:, rr_idx], O_["RotationRate"][fr][:, rr_idx], sc[fr, 15:18], 1e-12)
-    # walls: smoothed stress (Sigma = sum K sigma_j / sum K)
+    # walls: smoothed stress (Sigma = sum K sigma_j / sum K) and their own pair sums (the
+    # reference adds every pair into both particles)
     fx = ~fr
     np.testing.assert_allclose(G["Sigma"][fx], O_["Sigma"][fx], rtol=1e-12, atol=1e-12*np.abs(O_["Sigma"]).max())
+    _check("wall a", G["a"][fx], O_["a"][fx], sc[fx, 0:3], 1e-12)
+    _check("wall dDensity", G["dDensity"][fx], O_["dDensity"][fx], sc[fx, 3], 1e-12)
 
 
 # 20 steps: this column unloads to the stress-free Drucker-Prager apex around step 40, where
