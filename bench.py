@@ -57,6 +57,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def force_pass_name(precision):
+    """The kernels of the timed force pass of this workload."""
+    if precision == "fp64" and not os.environ.get("PSPH_NO_TILES"):
+        return ("force pass: k_nlist + k_tile_table + k_tile_union (list builds) + "
+                "k_force11_tile (FP64, LDS-staged tiles) + overflow pass")
+    return ("force pass: k_nlist (list builds) + " +
+            ("k_force11_nl (FP64)" if precision == "fp64" else "k_force11_f32_nl (FP32-mixed)") + " + overflow pass")
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -147,9 +156,8 @@ def time_tier(s, args, n, nfree, label, precision):
         res["roofline"] = {
             "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach/HBM_PEAK_GBS, 5), "traffic": int(tb) if tb else None,
-            "kernel": "force pass: k_nlist (list builds) + " +
-                      ("k_force11_nl (FP64)" if precision == "fp64" else "k_force11_f32_nl (FP32-mixed)") +
-                      " + overflow pass", "kernel_ms": round(fms, 5), "algorithmic_bytes_per_launch": int(force_b)}
+            "kernel": force_pass_name(precision), "kernel_ms": round(fms, 5),
+            "algorithmic_bytes_per_launch": int(force_b)}
         res["kernel_means_ms"] = {k: round(v[0], 5) for k, v in stats.items() if v[1]}
         log(f"[{label}] kernel means (ms): " + ", ".join(f"{k}={v[0]:.4f}" for k, v in stats.items() if v[1]))
     if args.amort_steps > 0:

@@ -1755,12 +1755,17 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const double *p
 {
     const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
+    // the pair test on the reference's r^2 (no contraction: the same set as a list build)
     double r2 = x0*x0 + x1*x1 + x2*x2;
     if (SKIN) {
         bool in = in_support(P, r2);
         if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[jf()]);
         if (!in) return;
     }
+    // the contribution: products and sums may fuse (each fused pair rounds once instead of
+    // twice, far inside the 1e-12 bar on the force pass)
+    {
+#pragma clang fp contract(fast)
     double invr = rsq_nr(r2);
     double rij = r2 > 0.0 ? r2*invr : 0.0;
     double q = rij*P.invh;
@@ -1802,6 +1807,7 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const double *p
     double temp1 = (u0*x0 + u1*x1 + u2*x2)*GK;
     A.a0 += mj*t0; A.a1 += mj*t1; A.a2 += mj*t2;
     A.dd += mj*(di*invdj)*temp1;
+    }
 }
 
 // SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
@@ -2261,11 +2267,11 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
             }
             const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
             // all four records read at once; the wall record's slot is clamped for free ones
+            // a wall's viscous velocity; read for every slot (clamped), used only for walls
             const double4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
             const double4 w = sw[slot >= uf ? slot - uf : 0];
-            const double4 vw = slot >= uf ? w : a1;
             pair11_rec<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
-                                  a0, a1, a2, vw, mu_ff, mu_w, A, nff, nfw);
+                                  a0, a1, a2, w, mu_ff, mu_w, A, nff, nfw);
         }
     } else {
         const size_t gs = (size_t)P.nfree;
