@@ -129,6 +129,7 @@ struct psph_handle {
     psph_grid grid{};
     bool trans_ok = false;             // transient columns match the current order
     bool fixed_ok = false;             // fixed particles' pair sums formed for this stop
+    bool no_lean = false;              // PSPH_NO_LEAN: queue every step in full (test switch)
     double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
     bool hook_b = false, hook_a = false;   // host hook stops (psph_set_hooks)
     int regrowths = 0;                 // device buffers re-created for more particles (in/out-flow)
@@ -787,17 +788,24 @@ static int read_state(psph_handle *h)
 // ------------------------------------------------------------------------------------------
 // Queues one step.  from = 0: a whole step; 1: the rest of a step stopped before the wall pass
 // (GeneralBefore hook stop); 2: the rest of a step stopped after the force pass (Move).
-static void enqueue_step(psph_handle *h, int from)
+// Queues one step from stage `from` (0: whole step; 1: from the wall pass; 2: Move and tail).
+// lean: list reuse, and the step is known not to rebuild, so the sort and list-build kernels
+// (no-ops on such steps) are not queued.  tail_lean: the step's tail halts (kHaltSort) when the
+// step after it must rebuild, because that step will be queued lean.
+static void enqueue_step(psph_handle *h, int from, bool lean, bool tail_lean)
 {
-    const Params &P = h->P;
+    Params P = h->P;
     const Dev &D = h->D;
+    P.lean = lean ? 1 : 0;
     if (from == 0) {
-        timed(h, KScan, [&] {
-            if (P.verlet) launch_keys(D, P, h->stream);      // only on steps that rebuild
-            launch_scan(D, P, h->stream);
-        });
-        timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
-        timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
+        if (!lean) {
+            timed(h, KScan, [&] {
+                if (P.verlet) launch_keys(D, P, h->stream);      // only on steps that rebuild
+                launch_scan(D, P, h->stream);
+            });
+            timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
+            timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
+        }
         timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
     }
     if (from <= 1) {
@@ -805,6 +813,7 @@ static void enqueue_step(psph_handle *h, int from)
         timed(h, KForce, [&] { launch_force(D, P, h->stream); });
     }
     timed(h, KIntegrate, [&] { launch_integrate(D, P, h->stream); });
+    P.lean = tail_lean ? 1 : 0;
     timed(h, KTail, [&] { launch_tail(D, P, h->stream); });
 }
 
@@ -1386,6 +1395,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     // the FP64 fast path with list reuse gathers per tile with the neighbours in LDS
     // (PSPH_NO_TILES: the per-target list gather instead)
     P.tiles = (P.fast11 && !P.mixed32 && P.verlet && !std::getenv("PSPH_NO_TILES")) ? 1 : 0;
+    P.lean = 0;
+    h->no_lean = std::getenv("PSPH_NO_LEAN") != nullptr;
     P.tile_items = kTileItems;
     if (const char *e = std::getenv("PSPH_TILE_ITEMS")) P.tile_items = std::max(0, std::min(kTileItems, std::atoi(e)));
     h->D.tile = nullptr; h->D.items = nullptr; h->D.tl = nullptr;
@@ -1466,12 +1477,16 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     int pending_from = (C.mid_step != 0 && max_steps > 0) ? (C.mid_step == 2 ? 1 : 2) : 0;
     bool pending_finish = pending_from != 0;
     const long long batch = 16;
+    // list reuse: steps after the first of a batch are queued lean (their tails halt before a
+    // step that must rebuild); the first is lean when the control block says it keeps the lists
+    const bool lean_ok = h->P.verlet && !h->no_lean;
     for (;;) {
         // queue a batch; every kernel is a no-op once the device has halted
         long long nq = std::min<long long>(batch, max_steps - queued);
         if (pending_finish && nq < 1) nq = 1;
         for (long long k = 0; k < nq; ++k) {
-            enqueue_step(h, pending_finish ? pending_from : 0);
+            const bool lean = lean_ok && (k > 0 || !C.rebuild);
+            enqueue_step(h, pending_finish ? pending_from : 0, lean, lean_ok);
             pending_finish = false;
             ++queued;
         }
@@ -1496,6 +1511,14 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
         }
         if (C.halt == kRun) {
             if (queued >= max_steps) break;   // cannot happen: the tail halts at max_steps
+            continue;
+        }
+        if (C.halt == kHaltSort) {
+            // the next step rebuilds: queue it in full (C.rebuild is set)
+            C.halt = kRun;
+            rc = push_ctl(h);
+            if (rc) return rc;
+            queued = C.steps_done;
             continue;
         }
         if (C.halt == kHaltLeave) {

@@ -54,6 +54,8 @@ enum : int {
     kHaltError = 5,    // see Ctl::error
     kHaltHookBefore = 6,   // host hook before the wall smoothing and the pair sums (mid-step)
     kHaltHookAfter = 7,    // host hook after the pair sums, before Move (mid-step)
+    kHaltSort = 8,     // list reuse: the next step rebuilds, but was queued without its sort
+                       // and list-build kernels (Params::lean); the host queues it in full
 };
 
 // Solve-loop scalars, resident in device memory so steps run without host round trips.
@@ -160,6 +162,10 @@ struct Params {
     // slots per tile (kTileTargets times the list capacity rounded up to 4 kTileLanes)
     int tiles, tile_max, tl_stride;
     int tile_items;                    // items a tile may stage (kTileItems; less: test switch)
+    // list reuse, set per launch by the host: the step is queued without the kernels that only
+    // work on rebuilding steps (sort, list build, tile union), and the tail halts (kHaltSort)
+    // when the step after it must rebuild
+    int lean;
 };
 
 // One set of the persistent columns.

@@ -2874,6 +2874,8 @@ __global__ void k_tail(Dev D, Params P)
     if (c->leave) { c->halt = kHaltLeave; return; }
     if (c->steps_done >= c->max_steps) { c->halt = kHaltSteps; return; }
     if (!(c->time < c->tf)) { c->halt = kHaltTend; return; }
+    // the next step was queued lean but must sort and rebuild: the host queues it in full
+    if (P.lean && c->rebuild) c->halt = kHaltSort;
 }
 
 // Head of a run call: stop at once if the loop condition fails (Domain.cpp:1762).
@@ -3172,11 +3174,14 @@ template <int DIM, int REACH>
 static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
 {
     const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
-    hipLaunchKernelGGL((k_nlist<DIM, REACH>), gl, dim3(kBlock), 0, s, D, P);
+    // the list builds are no-ops on steps that keep the lists; a lean step does not queue them
+    if (!P.lean) hipLaunchKernelGGL((k_nlist<DIM, REACH>), gl, dim3(kBlock), 0, s, D, P);
     if (REACH == 2 && P.tiles) {
         const dim3 gt(P.tile_max);
-        hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, D, P);
-        hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
+        if (!P.lean) {
+            hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, D, P);
+            hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
+        }
         hipLaunchKernelGGL((k_force11_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
         hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         return;
