@@ -1314,6 +1314,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         if (c.IsFree && c.Shepard && P.scheme == 0) P.any_shepard = 1;
     }
     P.any_xsph = (h->cfg.xsph != 0.0 && P.scheme == 0) ? 1 : 0;
+    if (P.uniform_h) {
+        // the band of the r^2 pair test (support_test): outside it r^2 decides exactly as
+        // sqrt(r^2)/h <= Cellfac does, inside it that test runs
+        const double sup = P.cellfac*P.h_uniform;
+        P.sup2_lo = sup*sup*(1.0 - 1e-13);
+        P.sup2_hi = sup*sup*(1.0 + 1e-13);
+    }
     // fast fluid path: cubic spline, Morris, squared-density form, no artificial stress, no
     // strain-rate viscosity models, no XSPH/Shepard terms, uniform h and free-particle mu
     {
@@ -1335,9 +1342,6 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.kgh = P.kg/hh;
             P.invh = 1.0/hh;
             P.h2c = 0.01*hh*hh;
-            double sup = P.cellfac*hh;
-            P.sup2_lo = sup*sup*(1.0 - 1e-13);
-            P.sup2_hi = sup*sup*(1.0 + 1e-13);
             for (int a = 0; a < P.nclasses; ++a)
                 for (int b = 0; b < P.nclasses; ++b) {
                     P.pair_alpha[a*8+b] = (h->classes[a].Alpha + h->classes[b].Alpha)/2.0;

@@ -121,6 +121,26 @@ __device__ inline int cell_rank(const Dev &D, int key, bool isfree)
 // With neighbour-list reuse the sort and the list builds run only on steps that rebuild.
 __device__ inline bool sort_skipped(const Dev &D, const Params &P) { return P.verlet && !D.ctl->rebuild; }
 
+// The reference's pair test rij/h <= Cellfac (CalcForce11, Interaction.cpp:32) on a squared
+// distance: an r^2 band decides all but a 1e-13-wide shell around the support radius, where
+// the exact test runs on sqrt(r^2) as the reference forms rij.
+__device__ inline bool in_support(const Params &P, double r2)
+{
+    if (r2 <= P.sup2_lo) return true;
+    if (r2 > P.sup2_hi) return false;
+    return (sqrt(r2)/P.h_uniform) <= P.cellfac;
+}
+
+// The reference's pair test rij/h <= Cellfac with rij = sqrt(r2) (CalcForce11/2233): with one h
+// for every class it is decided on r2 outside a 1e-13 band around the support radius
+// (in_support), else as written.
+__device__ inline bool support_test(const Params &P, double r2, double h)
+{
+    if (P.uniform_h) return in_support(P, r2);
+    return (sqrt(r2)/h) <= P.cellfac;
+}
+
+
 // Cell keys and arrival ranks of the current positions: at upload (and after host-side changes)
 // for the first step, and with list reuse at the head of every step that rebuilds.
 __global__ void __launch_bounds__(kBlock) k_keys(Dev D0, Params P)
@@ -627,16 +647,6 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
 // ------------------------------------------------------------------------------------------
 // pair membership and cell codes of the fast fluid path
 // ------------------------------------------------------------------------------------------
-
-// The reference's pair test rij/h <= Cellfac (CalcForce11, Interaction.cpp:32) on a squared
-// distance: an r^2 band decides all but a 1e-13-wide shell around the support radius, where
-// the exact test runs on sqrt(r^2) as the reference forms rij.
-__device__ inline bool in_support(const Params &P, double r2)
-{
-    if (r2 <= P.sup2_lo) return true;
-    if (r2 > P.sup2_hi) return false;
-    return (sqrt(r2)/P.h_uniform) <= P.cellfac;
-}
 
 // Whether the cells of two keys are the same or neighbours: the reference pairs particles of
 // neighbouring cells only (MainNeighbourSearch, Domain.cpp:808-945).  Cells can be slightly
@@ -1224,8 +1234,7 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
                 double h = P.uniform_h ? P.h_uniform : (hcls[ci] + hcls[flag_cls(fj)])/2;
                 double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
                 if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
-                double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-                if ((rij/h) <= P.cellfac) out.push(j);
+                if (support_test(P, x0*x0 + x1*x1 + x2*x2, h)) out.push(j);
             }
         }
     }
@@ -1330,8 +1339,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
                     double h = pair_h(cp, P, ci, flag_cls(fj));
                     double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
                     if (P.any_periodic) periodic_xij(D, P, h, D.key[i], j, x0, x1, x2);
-                    double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-                    if ((rij/h) <= P.cellfac && seen++ >= skip) {
+                    if (support_test(P, x0*x0 + x1*x1 + x2*x2, h) && seen++ >= skip) {
                         lst[cnt*kForceBlock + tid] = j;
                         ++cnt;
                         if (flag_free(fj)) ++nff; else ++nfw;
@@ -1688,8 +1696,9 @@ __global__ void __launch_bounds__(kForceBlock) k_cross(Dev D0, Params P)
                 const double h = (prod == 3) ? fmin(hi, hj) : (hi + hj)/2;
                 double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
                 if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
-                const double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
-                if (!((rij/h) <= P.cellfac)) continue;
+                const double r2 = x0*x0 + x1*x1 + x2*x2;
+                if (!support_test(P, r2, h)) continue;
+                const double rij = sqrt(r2);
                 if (flag_free(fj)) ++nff; else ++nfw;
                 if (prod == 2) pair12(D, P, cp, i, j, fi, fj, pi, pj, x0, x1, x2, rij, h, A);
                 else pair13(D, P, cp, i, j, fi, fj, pi, pj, x0, x1, x2, rij, h, A);
@@ -2963,7 +2972,7 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sums(Dev D0, Params P)
             const double h = pair_h(cp, P, flag_cls(fj), cw);
             double x0 = pj.x - pw.x, x1 = pj.y - pw.y, x2 = pj.z - pw.z;
             if (P.any_periodic) periodic_xij(D, P, h, D.key[j], w, x0, x1, x2);
-            if (sqrt(x0*x0 + x1*x1 + x2*x2)/h > P.cellfac) continue;
+            if (!support_test(P, x0*x0 + x1*x1 + x2*x2, h)) continue;
             const double4 vj = D.vel_b[j], aj = D.aux[j];
             Acc A;
             A.a0 = A.a1 = A.a2 = A.dd = A.ddc = 0.0;
