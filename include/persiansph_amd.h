@@ -254,7 +254,9 @@ int psph_stream(psph_handle *h, void **stream);
 
 /* Bytes of algorithmic (compulsory) HBM traffic per step for the current particle set and
  * precision tier (FP32-mixed: FP32 neighbour records), and of one force pass, as used for
- * roofline figures: every array a kernel needs is read once and written once. */
+ * roofline figures: every array a kernel needs is read once and written once.  Soil and solid
+ * runs add their stress-rate tensors (~650 B per free particle; the force pass reads Sigma and
+ * TIR of every particle and writes StrainRate and RotationRate of the free ones). */
 int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_bytes);
 
 int psph_destroy(psph_handle *h);

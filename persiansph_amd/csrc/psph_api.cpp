@@ -1857,6 +1857,14 @@ int psph_traffic_model(psph_handle *h, double *bytes_per_step, double *force_byt
     // keys 32 B, sort 16 B, cell table 8 B per cell, reorder of the persistent state 2 x 112 B,
     // integration of the free particles 192 B
     double total = 32.0*N + 16.0*N + 8.0*C + 224.0*N + views + wall + force + 192.0*Nf;
+    if (h->P.any_solid) {
+        // soil and solid: the tensors Sigma, StrainRate, RotationRate, Strain, TIR read and
+        // written once, ~650 B per free particle; of these the force pass reads Sigma and TIR
+        // of every particle (96 B) and writes StrainRate and RotationRate of the free ones (72 B)
+        const double tens = 96.0*N + 72.0*Nf;
+        force += tens;
+        total += std::max(650.0*Nf, tens);
+    }
     if (bytes_per_step) *bytes_per_step = total;
     if (force_bytes) *force_bytes = force;
     return 0;

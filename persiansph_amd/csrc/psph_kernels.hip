@@ -1028,11 +1028,16 @@ __device__ __forceinline__ void pair11(const Dev &D, const Params &P, const Clas
 // CalcForce2233 (Interaction.cpp:186-377) contribution of neighbour j to free particle i as P1,
 // with the pair arithmetic and sums in T (double: the parity tier; float: the FP32-mixed tier,
 // positions differenced in FP64 and rounded, Kahan-compensated continuity sum).
+// tiw (when not null): 1/W(dp) of each class pair (k_force's table, see ti_table), the kernel
+// value at the artificial-stress spacing that the reference evaluates per pair.
 template <class T>
 __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const ClassParams *cp, int i, int j,
                                 uint32_t fi, double4 pi, double4 vi, double4 ai, const double *sgi,
-                                const double *tri, AccT<T> &A, int shep_on, double *tw = nullptr)
+                                const double *tri, AccT<T> &A, int shep_on, double *tw = nullptr,
+                                const double *tiw = nullptr)
 {
+    // divisions by refined reciprocals and fused products (a few ulp per term, inside the bars)
+#pragma clang fp contract(fast)
     uint32_t fj = D.flags_b[j];
     int ci = flag_cls(fi), cj = flag_cls(fj);
     const ClassParams &Ci_ = cp[ci];
@@ -1058,26 +1063,29 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
         if (!freej) { dj = (T)wj.w; mj = (T)Cj_.FPMassC * mi; } else { dj = (T)pj.w; mj = (T)aj.x; }
     }
     T u0 = (T)(vi.x - vj.x), u1 = (T)(vi.y - vj.y), u2 = (T)(vi.z - vj.z);
-    T q = rij/h;
+    T q = rij*rcpx(h);
     T GK = grad_kernel<T>(P.dim, P.kernel_type, q, h);
     T K = kernel_w<T>(P.dim, P.kernel_type, q, h);
+    const T rdj = rcpx(dj), rdm = rcpx((T)0.5*(di+dj));
     T PIs = 0.0;
     bool pi_on = false;
     if (Alpha != (T)0.0 || Beta != (T)0.0) {
         T dvx = u0*x0 + u1*x1 + u2*x2;
-        T MUij = h*dvx/(rij*rij+(T)0.01*h*h);
+        T MUij = h*dvx*rcpx(rij*rij+(T)0.01*h*h);
         T Cij;
         if (soil) Cij = (T)(0.5*(Ci_.Cs+Cj_.Cs));
         else Cij = (T)(0.5*(ai.z+aj.z));
-        if (dvx < (T)0) { PIs = (Alpha*Cij*MUij+Beta*MUij*MUij)/((T)0.5*(di+dj)); pi_on = true; }
+        if (dvx < (T)0) { PIs = (Alpha*Cij*MUij+Beta*MUij*MUij)*rdm; pi_on = true; }
     }
     T TI6[6] = {0, 0, 0, 0, 0, 0};
     if (Ci_.TI > 0.0 || Cj_.TI > 0.0) {
         // pow(f, n) of the reference as exp(n log f): f = W(r)/W(dp) >= 0 (f = 0 gives 0 either
         // way), |n log f| stays below ~10, so the result is within ~1e-15 relative of pow's at
         // about half of its instructions
-        T sT = exp((T)((Ci_.TIn+Cj_.TIn)/2.0)*
-                   log(K/kernel_w<T>(P.dim, P.kernel_type, (T)((Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*(double)h)), h)));
+        const T rwdp = tiw ? (T)tiw[ci*8 + cj]
+                           : rcpx(kernel_w<T>(P.dim, P.kernel_type,
+                                              (T)((Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*(double)h)), h));
+        T sT = exp((T)((Ci_.TIn+Cj_.TIn)/2.0)*log(K*rwdp));
         double trj[6];
         ld6(D.tir, j, trj);
         for (int k = 0; k < 6; ++k) TI6[k] = (T)(tri[k]+trj[k])*sT;
@@ -1095,7 +1103,7 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     T s11 = ((T)2.0*b1*x1)*f05, s12 = (b1*x2+b2*x1)*f05, s22 = ((T)2.0*b2*x2)*f05;
     T r01 = (b0*x1-b1*x0)*f05, r02 = (b0*x2-b2*x0)*f05, r12 = (b1*x2-b2*x1)*f05;
     if (P.xsph != 0.0 && freej) {
-        T sx = (T)P.xsph*mj/((T)0.5*(di+dj))*K;
+        T sx = (T)P.xsph*mj*rdm*K;
         A.vx0 += sx*-u0; A.vx1 += sx*-u1; A.vx2 += sx*-u2;
     }
     // M = Sigma_i/di^2 + Sigma_j/dj^2 + PI I + TI, then temp = (GK xij)^T M (matvec.h:806-811)
@@ -1109,13 +1117,13 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     sym_to_full(sgj, Mj);
     sym_to_full(TI6, T9);
     if (P.gradient_type == 0) {
-        T fa = (T)1.0/(di*di), fb = (T)1.0/(dj*dj);
+        T fa = rcpx(di*di), fb = rdj*rdj;
         for (int k = 0; k < 9; ++k) {
             T pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? (T)1.0 : (T)0.0)*PIs : (T)0.0;
             Mf[k] = Mi[k]*fa + Mj[k]*fb + pim + T9[k];
         }
     } else {
-        T fa = (T)1.0/(di*dj);
+        T fa = rcpx(di*dj);
         for (int k = 0; k < 9; ++k) {
             T pim = pi_on ? ((k == 0 || k == 4 || k == 8) ? (T)1.0 : (T)0.0)*PIs : (T)0.0;
             Mf[k] = (Mi[k] + Mj[k])*fa + pim + T9[k];
@@ -1133,17 +1141,17 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     }
     A.a0 += mj * t0; A.a1 += mj * t1; A.a2 += mj * t2;
     if constexpr (sizeof(T) == 4) {
-        T c = mj * (di/dj) * temp1;
+        T c = mj * (di*rdj) * temp1;
         T y = c - A.ddc;
         T tt = A.dd + y;
         A.ddc = (tt - A.dd) - y;
         A.dd = tt;
     } else {
-        A.dd += mj * (di/dj) * temp1;
+        A.dd += mj * (di*rdj) * temp1;
     }
-    T fq = mj/dj;
-    A.zw += mj/dj* K;
-    if (P.swi1) A.sw = A.sw + mj/dj*b0*x1*-GK;         // Interaction.cpp:347
+    T fq = mj*rdj;
+    A.zw += fq* K;
+    if (P.swi1) A.sw = A.sw + fq*b0*x1*-GK;            // Interaction.cpp:347
     A.sr[0] = A.sr[0] + s00*fq; A.sr[1] = A.sr[1] + s01*fq; A.sr[2] = A.sr[2] + s02*fq;
     A.sr[3] = A.sr[3] + s11*fq; A.sr[4] = A.sr[4] + s12*fq; A.sr[5] = A.sr[5] + s22*fq;
     A.rr[0] = A.rr[0] + r01*fq; A.rr[1] = A.rr[1] + r02*fq; A.rr[2] = A.rr[2] + r12*fq;
@@ -1255,6 +1263,24 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     __shared__ int lst[kCap*kForceBlock];
     for (int k = threadIdx.x; k < P.nclasses; k += kForceBlock) cp[k] = D.cls[k];
     __syncthreads();
+    // 1/W(dp) of each class pair for the artificial stress (up to 8 classes; else per pair)
+    __shared__ double tiw[64];
+    const double *tiwp = nullptr;
+    if constexpr (SOLID) {
+        if (P.any_ti && P.nclasses <= 8) {
+            for (int k = threadIdx.x; k < 64; k += kForceBlock) {
+                const int a = k >> 3, b = k & 7;
+                double v = 0.0;
+                if (a < P.nclasses && b < P.nclasses) {
+                    const double h = pair_h(cp, P, a, b);
+                    v = rcpx(kernel_w<double>(P.dim, P.kernel_type, (cp[a].TIInitDist + cp[b].TIInitDist)/(2.0*h), h));
+                }
+                tiw[k] = v;
+            }
+            tiwp = tiw;
+            __syncthreads();
+        }
+    }
     const int tid = threadIdx.x;
     int t = xcd_block()*kForceBlock + tid;
     unsigned long long dtb = ~0ull;      // no dt candidate from an inactive lane
@@ -1291,7 +1317,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
             if constexpr (sizeof(T) == 8) {
                 if (mat == 1) { pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on); return; }
             }
-            pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on);
+            pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on, nullptr, tiwp);
         } else {
             pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on);
         }
@@ -1719,24 +1745,6 @@ __global__ void __launch_bounds__(kForceBlock) k_cross(Dev D0, Params P)
 // test is decided on rij^2 except in a 1e-13 band around the support radius, where the exact
 // rij/h <= Cellfac test of the reference runs.
 // ------------------------------------------------------------------------------------------
-__device__ inline double rcp_nr(double x)
-{
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
-}
-
-__device__ inline double rsq_nr(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-    double hx = 0.5*x;
-    double e = fma(-hx*y, y, 0.5);
-    y = fma(y, e, y);
-    e = fma(-hx*y, y, 0.5);
-    return fma(y, e, y);
-}
 
 // Whether list entry j of free target i is a pair of the reference's set at the current
 // positions, given their squared distance: inside the support (in_support) and in neighbouring

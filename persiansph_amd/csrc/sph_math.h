@@ -13,6 +13,33 @@ template <class T> struct K_;
 template <> struct K_<double> { static constexpr double PI = 3.14159265358979323846; };
 template <> struct K_<float> { static constexpr float PI = 3.14159265358979323846f; };
 
+// 1/x and 1/sqrt(x) in FP64 from the hardware estimates refined by two Newton steps (each
+// within ~2 ulp of the correctly rounded value; about half the instructions of an IEEE divide).
+__device__ inline double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ inline double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double hx = 0.5*x;
+    double e = fma(-hx*y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-hx*y, y, 0.5);
+    return fma(y, e, y);
+}
+
+// 1/x for the pair formulas: refined reciprocal in FP64, the hardware reciprocal (1 ulp) in
+// FP32.  The reference divides; the difference is a few ulp per quotient, far inside the
+// force-pass bars (1e-12 FP64, 1e-5 FP32-mixed).
+__device__ inline double rcpx(double x) { return rcp_nr(x); }
+__device__ inline float rcpx(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Integer powers: the reference calls pow(x, n.0) with n = 2..5.  Here they are multiplication
 // chains (x2 = x*x, then x2*x, x2*x2, x2*x2*x): at most 3 roundings, a relative error of a few
 // 1e-16 against the correctly rounded power, far inside the FP64 parity bar, and about an order
@@ -42,16 +69,16 @@ __device__ __forceinline__ T kernel_w(int dim, int kt, T q, T h)
     T C;
     switch (kt) {
     case 0:
-        C = (dim == 2) ? T(10.0)/(T(7.0)*h*h*PI) : T(1.0)/(h*h*h*PI);
+        C = (dim == 2) ? T(10.0)*rcpx(T(7.0)*h*h*PI) : rcpx(h*h*h*PI);
         if (q < T(1.0)) return C*(T(1.0)-(T(3.0)/T(2.0))*q*q+(T(3.0)/T(4.0))*q*q*q);
         else if (q < T(2.0)) return C*((T(1.0)/T(4.0))*(T(2.0)-q)*(T(2.0)-q)*(T(2.0)-q));
         return T(0.0);
     case 1:
-        C = (dim == 2) ? T(7.0)/(T(4.0)*h*h*PI) : T(7.0)/(T(8.0)*h*h*h*PI);
+        C = (dim == 2) ? T(7.0)*rcpx(T(4.0)*h*h*PI) : T(7.0)*rcpx(T(8.0)*h*h*h*PI);
         if (q < T(2.0)) return C*ipow((T(1.0)-q/T(2.0)),T(4.0))*(T(2.0)*q+T(1.0));
         return T(0.0);
     default:
-        C = (dim == 2) ? T(7.0)/(T(478.0)*h*h*PI) : T(1.0)/(T(120.0)*h*h*h*PI);
+        C = (dim == 2) ? T(7.0)*rcpx(T(478.0)*h*h*PI) : rcpx(T(120.0)*h*h*h*PI);
         if (q < T(1.0)) return C*(ipow((T(3.0)-q),T(5.0))-T(6.0)*ipow((T(2.0)-q),T(5.0))+T(15.0)*ipow((T(1.0)-q),T(5.0)));
         else if (q < T(2.0)) return C*(ipow((T(3.0)-q),T(5.0))-T(6.0)*ipow((T(2.0)-q),T(5.0)));
         else if (q < T(3.0)) return C*(ipow((T(3.0)-q),T(5.0)));
@@ -67,22 +94,22 @@ __device__ __forceinline__ T grad_kernel(int dim, int kt, T q, T h)
     T C;
     switch (kt) {
     case 0:
-        C = (dim == 2) ? T(10.0)/(T(7.0)*h*h*h*PI) : T(1.0)/(h*h*h*h*PI);
-        if (q == T(0.0)) return C/h*(T(-3.0)+(T(9.0)/T(2.0))*q);
-        else if (q < T(1.0)) return C/(q*h)*(T(-3.0)*q+(T(9.0)/T(4.0))*q*q);
-        else if (q < T(2.0)) return C/(q*h)*((T(-3.0)/T(4.0))*(T(2.0)-q)*(T(2.0)-q));
+        C = (dim == 2) ? T(10.0)*rcpx(T(7.0)*h*h*h*PI) : rcpx(h*h*h*h*PI);
+        if (q == T(0.0)) return C*rcpx(h)*(T(-3.0)+(T(9.0)/T(2.0))*q);
+        else if (q < T(1.0)) return C*rcpx(q*h)*(T(-3.0)*q+(T(9.0)/T(4.0))*q*q);
+        else if (q < T(2.0)) return C*rcpx(q*h)*((T(-3.0)/T(4.0))*(T(2.0)-q)*(T(2.0)-q));
         return T(0.0);
     case 1:
-        C = (dim == 2) ? T(7.0)/(T(4.0)*h*h*h*PI) : T(7.0)/(T(8.0)*h*h*h*h*PI);
-        if (q == T(0.0)) return C*T(-5.0)/h*(ipow((T(1.0)-q/T(2.0)),T(3.0))-T(3.0)*q/T(2.0)*ipow((T(1.0)-q/T(2.0)),T(2.0)));
-        else if (q < T(2.0)) return C/(q*h)*T(-5.0)*q*ipow((T(1.0)-q/T(2.0)),T(3.0));
+        C = (dim == 2) ? T(7.0)*rcpx(T(4.0)*h*h*h*PI) : T(7.0)*rcpx(T(8.0)*h*h*h*h*PI);
+        if (q == T(0.0)) return C*T(-5.0)*rcpx(h)*(ipow((T(1.0)-q/T(2.0)),T(3.0))-T(3.0)*q/T(2.0)*ipow((T(1.0)-q/T(2.0)),T(2.0)));
+        else if (q < T(2.0)) return C*rcpx(q*h)*T(-5.0)*q*ipow((T(1.0)-q/T(2.0)),T(3.0));
         return T(0.0);
     default:
-        C = (dim == 2) ? T(7.0)/(T(478.0)*h*h*h*PI) : T(1.0)/(T(120.0)*h*h*h*h*PI);
-        if (q == T(0.0)) return C/h*(T(20.0)*ipow((T(3.0)-q),T(3.0))-T(120.0)*ipow((T(2.0)-q),T(3.0))+T(300.0)*ipow((T(1.0)-q),T(3.0)));
-        else if (q < T(1.0)) return C/(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0))+T(30.0)*ipow((T(2.0)-q),T(4.0))-T(75.0)*ipow((T(1.0)-q),T(4.0)));
-        else if (q < T(2.0)) return C/(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0))+T(30.0)*ipow((T(2.0)-q),T(4.0)));
-        else if (q < T(3.0)) return C/(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0)));
+        C = (dim == 2) ? T(7.0)*rcpx(T(478.0)*h*h*h*PI) : rcpx(T(120.0)*h*h*h*h*PI);
+        if (q == T(0.0)) return C*rcpx(h)*(T(20.0)*ipow((T(3.0)-q),T(3.0))-T(120.0)*ipow((T(2.0)-q),T(3.0))+T(300.0)*ipow((T(1.0)-q),T(3.0)));
+        else if (q < T(1.0)) return C*rcpx(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0))+T(30.0)*ipow((T(2.0)-q),T(4.0))-T(75.0)*ipow((T(1.0)-q),T(4.0)));
+        else if (q < T(2.0)) return C*rcpx(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0))+T(30.0)*ipow((T(2.0)-q),T(4.0)));
+        else if (q < T(3.0)) return C*rcpx(q*h)*(T(-5.0)*ipow((T(3.0)-q),T(4.0)));
         return T(0.0);
     }
 }

@@ -2,6 +2,13 @@
 
     python -m persiansph_amd.granular --dx 0.004 --t-end 0.5 --samples 10 --out granular.json
 
+    python -m persiansph_amd.granular --dx 0.004 --measure 0.05 500
+
+--measure T STEPS advances each tier to t = T, then times STEPS steps with HIP events on the
+solver's stream around every stage (sort, wall, force, integrate) and prints one JSON line per
+tier: ms/step, particle-updates/s, the stage means, the algorithmic bytes per step and per
+force pass (psph_traffic_model, soil tensors included) and the force pass's achieved GB/s.
+
 Column 0.2 x 0.4 (high) x 0.2 m of Drucker-Prager soil (phi 30 deg, c 0, psi 0, E 5 MPa,
 nu 0.3, rho 2650, Fail 3) on a rigid floor, cubic spline h = 1.5 dx, artificial stress on.
 The FP32-mixed tier sums the pair terms (acceleration, continuity, strain and rotation
@@ -23,15 +30,53 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def measure(cfg, p, dt, t_start, steps, label):
+    """Stage times and the force pass's roofline figures of `steps` steps from t = t_start."""
+    s = Solver(cfg, p, dt)
+    n, nfree = len(p), int((p["IsFree"] == 1).sum())
+    if t_start > 0.0:
+        s.run(10**9, tf=t_start)
+    s.run(5)
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    st = s.run(steps)
+    el = time.perf_counter() - t0
+    stats = s.kernel_stats()
+    s.set_timing(False)
+    total_b, force_b = s.traffic_model()
+    fms = stats.get("force", (0.0, 0))[0]
+    s.close()
+    res = {"tier": label, "t": st.time, "steps": st.steps, "ms_per_step": 1e3*el/st.steps,
+           "particle_updates_per_s": n*st.steps/el, "free_particle_updates_per_s": nfree*st.steps/el,
+           "pairs_per_step": st.pairs, "stage_means_ms": {k: round(v[0], 5) for k, v in stats.items() if v[1]},
+           "algorithmic_bytes_per_step": total_b, "force_bytes_per_launch": force_b}
+    if fms > 0:
+        ach = force_b/(fms*1e-3)/1e9
+        res["force_roofline"] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(ach/HBM_PEAK_GBS, 5), "kernel_ms": round(fms, 5)}
+    log(json.dumps(res))
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--dx", type=float, default=0.004)
     ap.add_argument("--t-end", type=float, default=0.5)
     ap.add_argument("--samples", type=int, default=10)
     ap.add_argument("--out", default="")
+    ap.add_argument("--measure", nargs=2, metavar=("T", "STEPS"))
     args = ap.parse_args(argv)
     dim = 3
     cfg, p, dt = cases.granular_column(dim=dim, dx=args.dx, scheme=1, precision=FP64)
+    if args.measure:
+        cfg32 = type(cfg).from_buffer_copy(cfg)
+        cfg32.precision = FP32_MIXED
+        for label, c in (("fp64", cfg), ("fp32", cfg32)):
+            print(json.dumps(measure(c, p, dt, float(args.measure[0]), int(args.measure[1]), label)), flush=True)
+        return
     cfg32 = type(cfg).from_buffer_copy(cfg)
     cfg32.precision = FP32_MIXED
     n0 = len(p)
