@@ -1254,7 +1254,10 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
     if ((threadIdx.x & 63) == 0 && anyx) atomicOr(&D.ctl->xmat, 1);
 }
 
-template <bool SOLID, class T>
+// FLUID (SOLID kernels): runs with free fluid particles besides soil or solid ones; without
+// them the fluid pair function is not compiled in, which keeps the FP64 kernel within 256 VGPRs
+// (two waves per SIMD).
+template <bool SOLID, class T, bool FLUID = true>
 __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -1314,7 +1317,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     // soil or solid together take the SOLID kernel
     auto pair_same = [&](int jj) __attribute__((always_inline)) {
         if constexpr (SOLID) {
-            if constexpr (sizeof(T) == 8) {
+            if constexpr (sizeof(T) == 8 && FLUID) {
                 if (mat == 1) { pair11(D, P, cp, i, jj, fi, pi, vi, ai, A, shep_on); return; }
             }
             pair2233<T>(D, P, cp, i, jj, fi, pi, vi, ai, sgi, tri, A, shep_on, nullptr, tiwp);
@@ -3228,8 +3231,10 @@ static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
     }
     if (P.any_solid && P.soil32)
         hipLaunchKernelGGL((k_force<true, float>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+    else if (P.any_solid && P.multi)
+        hipLaunchKernelGGL((k_force<true, double, true>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
     else if (P.any_solid)
-        hipLaunchKernelGGL((k_force<true, double>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
+        hipLaunchKernelGGL((k_force<true, double, false>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
     else
         hipLaunchKernelGGL((k_force<false, double>), dim3(nblk(P.nfree, kForceBlock)), dim3(kForceBlock), 0, s, D, P);
 }
