@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, AL al, BL bl,
 // sums of the row chunks meet in dW by FP32 atomic adds.
 template <class XL>
 __global__ void __launch_bounds__(256) k_gemm_wgrad(int M, int Kin, int chunk, XL xl, const uint16_t *dY,
-                                                    float *dW)
+                                                    float *dW, float *db)
 {
     __shared__ __attribute__((aligned(16))) uint16_t As[BM*LDK];   // [k][m]
     __shared__ __attribute__((aligned(16))) uint16_t Bs[BN*LDK];   // [n][m]
@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(256) k_gemm_wgrad(int M, int Kin, int chunk, X
     const int k0 = blockIdx.x*BM;
     const int mb = blockIdx.y*chunk, me = min(M, mb + chunk);
     f32x16 acc[2][2];
+    float csum = 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -135,6 +136,17 @@ __global__ void __launch_bounds__(256) k_gemm_wgrad(int M, int Kin, int chunk, X
             }
         }
         __syncthreads();
+        if (blockIdx.x == 0) {
+            // the bias gradient: column sums of the staged dY rows (rows past the end are zeros)
+            const uint4 *col = reinterpret_cast<const uint4 *>(&Bs[(tid & 127)*LDK + (tid >> 7)*16]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint4 v = col[h];
+                const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) csum += bf2f(e[q]);
+            }
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 16) {
             bf16x8 a[2], b[2];
@@ -163,6 +175,7 @@ __global__ void __launch_bounds__(256) k_gemm_wgrad(int M, int Kin, int chunk, X
                 int c = wn*64 + j*32 + (lane & 31);
                 if (r < Kin) atomicAdd(&dW[(size_t)r*128 + c], acc[i][j][q]);
             }
+    if (blockIdx.x == 0) atomicAdd(&db[tid & 127], csum);
 }
 
 // ---- operand loaders ----------------------------------------------------------------------
@@ -369,19 +382,6 @@ __global__ void __launch_bounds__(256) k_edge_mlp(int M, LdEdgeCat al, const uin
 }
 
 // ---- elementwise / graph kernels ----------------------------------------------------------
-
-// column sums of a bf16 [M][128] matrix into db (FP32 atomics per block of rows)
-__global__ void __launch_bounds__(256) k_colsum(int M, const uint16_t *dY, float *db, int rows)
-{
-    int c = threadIdx.x & 127, half = threadIdx.x >> 7;
-    int mb = blockIdx.x*rows, me = min(M, mb + rows);
-    float s = 0.0f;
-    for (int m = mb + half; m < me; m += 2) s += bf2f(dY[(size_t)m*128 + c]);
-    __shared__ float sh[256];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    if (half == 0) atomicAdd(&db[c], sh[c] + sh[c + 128]);
-}
 
 // a[i] = mean of E over the CSR segment of i (bf16 out, FP32 sums); one wave per node
 __global__ void __launch_bounds__(256) k_segsum(int n, const int32_t *rp, const uint16_t *E, uint16_t *a)
@@ -782,8 +782,7 @@ static void wgrad(hipStream_t s, int M, int kin, XL xl, const uint16_t *dY, floa
 {
     const int chunk = 2048;
     dim3 g(nb(kin, BM), nb(M, chunk));
-    hipLaunchKernelGGL((k_gemm_wgrad<XL>), g, dim3(256), 0, s, M, kin, chunk, xl, dY, dW);
-    hipLaunchKernelGGL(k_colsum, dim3(nb(M, 1024)), dim3(256), 0, s, M, dY, db, 1024);
+    hipLaunchKernelGGL((k_gemm_wgrad<XL>), g, dim3(256), 0, s, M, kin, chunk, xl, dY, dW, db);
 }
 void gemm_wgrad_rows(hipStream_t s, int M, int kin, const uint16_t *X, int ldx, const uint16_t *dY, float *dW, float *db)
 { wgrad(s, M, kin, LdRows{X, ldx}, dY, dW, db); }
