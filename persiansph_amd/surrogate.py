@@ -264,15 +264,31 @@ def _variants(n, seed):
     return np.random.default_rng(seed).uniform(0.5, 2.0, size=n)
 
 
+def trajectories(aspects, dx, area, steps, k, workers=8, log=None):
+    """Plain-solver trajectories of the dam-break variants with the given aspect ratios, in
+    variant order.  A ~20 k-particle run leaves most of the GPU idle, so `workers` runs proceed
+    at once, each a solver on its own stream driven from its own thread (the library's calls
+    release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+
+    def one(item):
+        i, asp = item
+        cfg, p, dt = dam_break_variant(float(asp), dx=dx, area=area)
+        fr = trajectory(cfg, p, dt, steps*dt, k*dt)
+        if log:
+            log(f"[data] variant {i} aspect {asp:.3f}: {len(p)} particles, {len(fr)} frames "
+                f"({time.perf_counter() - t0:.1f}s)")
+        return fr
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(one, enumerate(aspects)))
+
+
 def cmd_train(a):
     import json
     t0 = time.perf_counter()
-    trajs = []
-    for i, asp in enumerate(_variants(a.variants, a.seed)):
-        cfg, p, dt = dam_break_variant(float(asp), dx=a.dx, area=a.area)
-        trajs.append(trajectory(cfg, p, dt, a.steps*dt, a.k*dt))
-        _log(f"[data] variant {i} aspect {asp:.3f}: {len(p)} particles, {len(trajs[-1])} frames "
-             f"({time.perf_counter() - t0:.1f}s)")
+    trajs = trajectories(_variants(a.variants, a.seed), a.dx, a.area, a.steps, a.k, a.workers, _log)
+    t_data = time.perf_counter() - t0
     radius = 2.0*1.3*a.dx
     dt = dam_break_variant(1.0, dx=a.dx, area=a.area)[2]
     sc = scales_from(trajs, a.k, dt, radius)
@@ -283,7 +299,8 @@ def cmd_train(a):
     np.save(a.out + ".params.npy", net.get_params())
     meta = dict(scales=sc, layers=a.layers, k=a.k, dx=a.dx, area=a.area, variants=a.variants, steps=a.steps,
                 iters=a.iters, lr=a.lr, seed=a.seed, loss_first=float(np.mean(losses[:20])),
-                loss_last=float(np.mean(losses[-20:])), train_s=t_train, ms_per_iter=1e3*t_train/max(a.iters, 1))
+                loss_last=float(np.mean(losses[-20:])), train_s=t_train, ms_per_iter=1e3*t_train/max(a.iters, 1),
+                data_s=t_data, frames=int(sum(len(t) for t in trajs)), workers=a.workers)
     with open(a.out + ".json", "w") as f:
         json.dump(meta, f, indent=1)
     print(json.dumps(meta))
@@ -398,6 +415,7 @@ def main(argv=None):
     t.add_argument("--iters", type=int, default=20000)
     t.add_argument("--lr", type=float, default=1e-4)
     t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--workers", type=int, default=8, help="trajectories generated at once")
     t.add_argument("--out", required=True)
     e = sub.add_parser("eval")
     e.add_argument("--model", required=True)
