@@ -401,6 +401,38 @@ def cmd_bench3d(a):
     print(json.dumps(res))
 
 
+def cmd_engine3d(a):
+    """The revert engine end to end on the 3-D dam break: `windows` windows of M dt0, each
+    advanced by M/k calls of the 10-layer network (3-D inputs; the network is not trained in
+    3-D, so its windows are expected to revert) and judged by the plain FP64 solver over the
+    whole window.  Reports the engine's wall time, accept/revert counts and each window's
+    front error, against the plain solver alone over the same simulated time."""
+    import json
+    cfg, p, dt = cases.dam_break_3d(dx=a.dx)
+    n = len(p)
+    T = a.k*dt
+    sc = dict(radius=2.0*1.3*a.dx, rho0=1000.0, vel_scale=math.sqrt(2.0*G*2.0), rho_scale=0.01,
+              out_dx=0.5*G*T*T, out_dv=G*T, out_drho=1.0, drift_t=T,
+              deg_scale=float(analysis.bulk_neighbours(3)))
+    net = make_network(3, a.layers, sc, train=False, seed=a.seed)
+    eng = HybridEngine(cfg, p, dt, net_predictor(net), k=a.k, M=a.M, dim=3)
+    t0 = time.perf_counter()
+    out = eng.run(a.windows)
+    el = time.perf_counter() - t0
+    span = out.t
+    s = Solver(cfg, p, dt)
+    t0 = time.perf_counter()
+    st = s.run(10**9, tf=span)
+    el_plain = time.perf_counter() - t0
+    s.close()
+    net.close()
+    res = dict(particles=n, layers=a.layers, k=a.k, M=a.M, windows=a.windows, simulated_s=span,
+               engine=dict(eng.stats, wall_s=el, history=eng.history),
+               plain=dict(steps=st.steps, wall_s=el_plain),
+               engine_speedup_over_plain=el_plain/el)
+    print(json.dumps(res, default=float))
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="python -m persiansph_amd.surrogate")
@@ -425,6 +457,13 @@ def main(argv=None):
     e.add_argument("--M", type=int, default=50)
     e.add_argument("--seed", type=int, default=1)
     e.add_argument("--out", default="")
+    g3 = sub.add_parser("engine3d")
+    g3.add_argument("--dx", type=float, default=0.01)
+    g3.add_argument("--layers", type=int, default=10)
+    g3.add_argument("--k", type=int, default=10)
+    g3.add_argument("--M", type=int, default=50)
+    g3.add_argument("--windows", type=int, default=4)
+    g3.add_argument("--seed", type=int, default=5)
     b = sub.add_parser("bench3d")
     b.add_argument("--dx", type=float, default=0.01)
     b.add_argument("--layers", type=int, default=10)
@@ -433,7 +472,7 @@ def main(argv=None):
     b.add_argument("--edges", action="store_true", help="also count edges (reads the graph back)")
     b.add_argument("--seed", type=int, default=1)
     a = ap.parse_args(argv)
-    {"train": cmd_train, "eval": cmd_eval, "bench3d": cmd_bench3d}[a.cmd](a)
+    {"train": cmd_train, "eval": cmd_eval, "bench3d": cmd_bench3d, "engine3d": cmd_engine3d}[a.cmd](a)
 
 
 if __name__ == "__main__":
