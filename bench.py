@@ -59,9 +59,10 @@ def log(*a):
 
 def force_pass_name(precision):
     """The kernels of the timed force pass of this workload."""
-    if precision == "fp64" and not os.environ.get("PSPH_NO_TILES"):
-        return ("force pass: k_nlist + k_tile_table + k_tile_union (list builds) + "
-                "k_force11_tile (FP64, LDS-staged tiles) + overflow pass")
+    if not os.environ.get("PSPH_NO_TILES"):
+        return ("force pass: k_nlist + k_tile_table + k_tile_union (list builds) + " +
+                ("k_force11_tile (FP64" if precision == "fp64" else "k_force32_tile (FP32-mixed") +
+                ", LDS-staged tiles) + overflow pass")
     return ("force pass: k_nlist (list builds) + " +
             ("k_force11_nl (FP64)" if precision == "fp64" else "k_force11_f32_nl (FP32-mixed)") + " + overflow pass")
 

@@ -204,7 +204,7 @@ static int nl_alloc(psph_handle *h, int cap)
     P.nl_cap = cap;
     h->D.nl = h->nl;
     if (P.tiles) {
-        // tiles of the FP64 fast path: at most nfree / kTileTargets + one per cell row
+        // tiles of the fast path: at most nfree / kTileTargets + one per cell row
         P.tile_max = (int)(nf/kTileTargets) + P.nc[1]*P.nc[2] + 1;
         // slot lists: per target, groups of 4*kTileLanes entries (one 8-byte word per lane)
         P.tl_stride = kTileTargets*((cap + 4*kTileLanes - 1)/(4*kTileLanes))*4*kTileLanes;
@@ -1396,9 +1396,11 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         rc = inflow_init(h);
         if (rc) return rc;
     }
-    // the FP64 fast path with list reuse gathers per tile with the neighbours in LDS
-    // (PSPH_NO_TILES: the per-target list gather instead)
-    P.tiles = (P.fast11 && !P.mixed32 && P.verlet && !std::getenv("PSPH_NO_TILES")) ? 1 : 0;
+    // the fast path with list reuse gathers per tile with the neighbours in LDS (PSPH_NO_TILES:
+    // the per-target list gather instead); the FP32 tiles pack each neighbour's cell in 10 bits
+    // an axis
+    const bool cells10 = P.nc[0] <= 1024 && P.nc[1] <= 1024 && P.nc[2] <= 1024;
+    P.tiles = (P.fast11 && P.verlet && (!P.mixed32 || cells10) && !std::getenv("PSPH_NO_TILES")) ? 1 : 0;
     P.lean = 0;
     h->no_lean = std::getenv("PSPH_NO_LEAN") != nullptr;
     P.tile_items = kTileItems;

@@ -2356,20 +2356,27 @@ __device__ inline bool skin_member32(const Dev &D, const Params &P, const Band32
     return r2 <= B.adj || keys_adjacent(P, D.key[i], D.key[j]);
 }
 
-// SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
-// current positions (skin_member32); i is the target.
-template <int DIM, bool SKIN>
-__device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
-                                  float4 gi0, float4 gi1, float4 gi2, int ci, int i, int j,
-                                  float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
-                                  float kw, float kg, float kgh, float invh, float h, float h2c,
-                                  float mu_ff, float mu_w, const Band32 &band)
+// SKIN: the neighbour comes from a skin list and counts only if it is a pair of the
+// reference's set at the current positions (skin_member32); i is the target.  The neighbour's
+// records are a0..a2 and vw (its viscous velocity, read only for walls); jf() gives its index,
+// called only inside the FP32 test's band.  o = (k_j - k_i) CellSize, the offset of the two
+// particles' cells at the last sort.
+template <int DIM, bool SKIN, class JF>
+__device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const float *pa, const float *pb,
+                                      float4 gi0, float4 gi1, float4 gi2, int ci, int i, JF jf,
+                                      float4 a0, float4 a1, float4 a2, float4 vw,
+                                      float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
+                                      float kw, float kg, float kgh, float invh, float h, float h2c,
+                                      float mu_ff, float mu_w, const Band32 &band)
 {
-    float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
     const float cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     float x0 = (gi0.x - a0.x) - o0, x1 = (gi0.y - a0.y) - o1, x2 = (gi0.z - a0.z) - o2;
     float r2 = x0*x0 + x1*x1 + x2*x2;
-    if (SKIN && !skin_member32(D, P, band, i, j, r2)) return;
+    if (SKIN) {
+        if (r2 > band.hi) return;
+        if (r2 >= band.lo) { if (!skin_member64(D, P, i, jf())) return; }
+        else if (!(r2 <= band.adj || keys_adjacent(P, D.key[i], D.key[jf()]))) return;
+    }
     float invr = __builtin_amdgcn_rsqf(r2);
     float rij = r2 > 0.0f ? r2*invr : 0.0f;
     float q = rij*invh;
@@ -2388,7 +2395,7 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
     nfw += freej ? 0u : 1u;
     int cj = freej ? (int)meta : (int)(-1.0f - meta);
     float w0 = a1.x, w1 = a1.y, w2 = a1.z;               // viscous velocity of the neighbour
-    if (!freej) { float4 vw = D.fw[j]; w0 = vw.x; w1 = vw.y; w2 = vw.z; }
+    if (!freej) { w0 = vw.x; w1 = vw.y; w2 = vw.z; }
     float alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
     float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     float dvx = u0*x0 + u1*x1 + u2*x2;
@@ -2413,6 +2420,19 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
     float tt = A.dd + y;
     A.ddc = (tt - A.dd) - y;
     A.dd = tt;
+}
+
+template <int DIM, bool SKIN>
+__device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
+                                  float4 gi0, float4 gi1, float4 gi2, int ci, int i, int j,
+                                  float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
+                                  float kw, float kg, float kgh, float invh, float h, float h2c,
+                                  float mu_ff, float mu_w, const Band32 &band)
+{
+    const float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
+    const float4 vw = (a2.y < 0.0f) ? D.fw[j] : a1;
+    pair11_f32_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
+                              o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
 }
 
 // FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
@@ -2478,8 +2498,10 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
     force_partials(D, dtb, nff, nfw);
 }
 
-// Ends the overflowed targets of the FP32 list pass (see k_force11_nl_over); the queued FP32
-// sums are exact in FP64 and restored as FP32.
+// Ends the overflowed targets of the FP32 list and tile passes (see k_force11_nl_over).  After
+// the list pass the queued FP32 sums (exact in FP64) are restored as FP32 and continued in list
+// order; after the tile pass (P.tiles) the entries past the capacity are summed in FP32 from
+// zero and added in FP64 to the queued FP64 sums of the first ones.
 template <int DIM, int REACH>
 __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Params P)
 {
@@ -2502,9 +2524,14 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
         int i = D.free_list[t];
         float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
         int ci = (int)gi2.y;
-        double4 s0 = D.ovs[2*q], s1 = D.ovs[2*q+1];
+        double4 s0 = D.ovs[2*q];
         Acc32 A;
-        A.a0 = (float)s0.x; A.a1 = (float)s0.y; A.a2 = (float)s0.z; A.dd = (float)s0.w; A.ddc = (float)s1.x;
+        A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
+        if (!P.tiles) {
+            A.a0 = (float)s0.x; A.a1 = (float)s0.y; A.a2 = (float)s0.z; A.dd = (float)s0.w;
+            A.ddc = (float)D.ovs[2*q+1].x;
+            s0 = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
         unsigned nff = 0, nfw = 0;
         int kx, ky, kz;
         decode_key(P, D.key[i], kx, ky, kz);
@@ -2522,14 +2549,134 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
             ++k;
         });
         const double4 ai = a_start(D, P, i);
-        double a0 = ai.x + (double)A.a0, a1 = ai.y + (double)A.a1, a2 = ai.z + (double)A.a2;
-        D.acc[i] = make_double4(a0, a1, a2, ai.w + ((double)A.dd + (double)(-A.ddc)));
+        double a0 = ai.x + (s0.x + (double)A.a0), a1 = ai.y + (s0.y + (double)A.a1), a2 = ai.z + (s0.z + (double)A.a2);
+        D.acc[i] = make_double4(a0, a1, a2, ai.w + (s0.w + ((double)A.dd + (double)(-A.ddc))));
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         atomicMin(&D.ctl->dtmin_bits, dbits(cand));
         if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
         if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
     }
+}
+
+// The FP32-mixed gather per tile (the tile table and union of k_force11_tile): each
+// neighbour's three FP32 records and its cell at the last sort (kx | ky << 10 | kz << 20) are
+// staged in LDS, walls' viscous velocities beside them; the pair is pair11_f32's, with the cell
+// offset from the two cells.  Lane l of a target adds entries l, l + kTileLanes, ... in list order
+// in FP32 (Kahan-compensated continuity per lane); the lanes' sums are widened and meet in FP64
+// in the fixed order of k_force11_tile.  Overflowed targets go to k_force11_f32_nl_over.
+template <int DIM>
+__global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0, Params P)
+{
+    const Dev D = bind(D0);
+    if (D.ctl->halt) return;
+    const int nb = D.ctl->ntiles;
+    if ((int)blockIdx.x >= nb) return;
+    __shared__ float4 s0[kTileItems], s1[kTileItems], s2[kTileItems], sw[kTileWalls];
+    __shared__ int sk[kTileItems];
+    __shared__ float pa[64], pb[64];
+    const int tile = xcd_remap((int)blockIdx.x, nb);
+    const int4 ti = D.tile[tile];
+    const int t0 = ti.x, nt = ti.y, uf = ti.z, uw = ti.w;
+    const bool staged = uw >= 0;
+    const int tid = threadIdx.x;
+    if (tid < 64) { pa[tid] = (float)P.pair_alpha[tid]; pb[tid] = (float)P.pair_beta[tid]; }
+    const int32_t *items = D.items + (size_t)tile*kTileItems;
+    const int tt = tid / kTileLanes, l = tid % kTileLanes;
+    const bool act = tt < nt;
+    const int t = t0 + (act ? tt : 0);
+    const int i = D.free_list[t];
+    const float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
+    const int cap = P.nl_cap;
+    const int cnt = act ? D.ncnt[t] : 0;
+    const int m = cnt < cap ? cnt : cap;
+    const uint64_t *tq = reinterpret_cast<const uint64_t *>(D.tl + (size_t)tile*P.tl_stride) +
+                         tt*kTileLanes + l;
+    constexpr size_t kWordStride = (size_t)kTileTargets*kTileLanes;
+    uint64_t qw[kSlotWords];
+#pragma unroll
+    for (int G = 0; G < kSlotWords; ++G)
+        qw[G] = (staged && l + 4*kTileLanes*G < m) ? tq[G*kWordStride] : 0ull;
+    auto pack = [&](int key) {
+        int kx, ky, kz;
+        decode_key(P, key, kx, ky, kz);
+        return kx | (ky << 10) | (kz << 20);
+    };
+    if (staged) {
+        const int U = uf + uw, u0 = tid, u1 = tid + (int)blockDim.x;
+        const int j0 = u0 < U ? items[u0] : -1, j1 = u1 < U ? items[u1] : -1;
+        float4 r00, r01, r02, r0w, r10, r11, r12, r1w;
+        int k0 = 0, k1 = 0;
+        if (j0 >= 0) { r00 = D.f0[j0]; r01 = D.f1[j0]; r02 = D.f2[j0]; if (u0 >= uf) r0w = D.fw[j0]; k0 = D.key_b[j0]; }
+        if (j1 >= 0) { r10 = D.f0[j1]; r11 = D.f1[j1]; r12 = D.f2[j1]; if (u1 >= uf) r1w = D.fw[j1]; k1 = D.key_b[j1]; }
+        if (j0 >= 0) { s0[u0] = r00; s1[u0] = r01; s2[u0] = r02; sk[u0] = pack(k0); if (u0 >= uf) sw[u0 - uf] = r0w; }
+        if (j1 >= 0) { s0[u1] = r10; s1[u1] = r11; s2[u1] = r12; sk[u1] = pack(k1); if (u1 >= uf) sw[u1 - uf] = r1w; }
+    }
+    __syncthreads();
+    const int ci = (int)gi2.y;
+    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
+    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const Band32 band(P);
+    Acc32 A;
+    A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
+    unsigned nff = 0, nfw = 0;
+    if (staged) {
+        const int ki = pack(D.key_b[i]);
+        const int kix = ki & 1023, kiy = (ki >> 10) & 1023, kiz = ki >> 20;
+        uint64_t q = 0ull;
+        for (int e = l, g = 0; e < m; e += kTileLanes, ++g) {
+            if ((g & 3) == 0) {
+                const int G = g >> 2;
+                q = qw[0];
+#pragma unroll
+                for (int k = 1; k < kSlotWords; ++k) q = (G == k) ? qw[k] : q;
+                if (G >= kSlotWords) q = tq[(size_t)G*kWordStride];
+            }
+            const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
+            const float4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
+            const float4 w = sw[slot >= uf ? slot - uf : 0];
+            const int kj = sk[slot];
+            const float o0 = (float)((kj & 1023) - kix)*csx, o1 = (float)(((kj >> 10) & 1023) - kiy)*csy;
+            const float o2 = (DIM == 3) ? (float)((kj >> 20) - kiz)*csz : 0.0f;
+            pair11_f32_rec<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
+                                      a0, a1, a2, w, o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c,
+                                      mu_ff, mu_w, band);
+        }
+    } else {
+        const size_t gs = (size_t)P.nfree;
+        for (int e = l; e < m; e += kTileLanes) {
+            const int v = D.nl[(((size_t)(e >> 2))*gs + (size_t)t)*4 + (e & 3)];
+            int dx, dy, dz;
+            code_offset<2>(nl_code<2>(v), dx, dy, dz);
+            const float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
+            pair11_f32<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, nl_index<2>(v), p0, p1, p2,
+                                  A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
+        }
+    }
+    double b0 = (double)A.a0, b1 = (double)A.a1, b2 = (double)A.a2, bd = (double)A.dd + (double)(-A.ddc);
+    for (int o = 1; o < kTileLanes; o <<= 1) {
+        b0 += __shfl_xor(b0, o, 64);
+        b1 += __shfl_xor(b1, o, 64);
+        b2 += __shfl_xor(b2, o, 64);
+        bd += __shfl_xor(bd, o, 64);
+    }
+    unsigned long long dtb = ~0ull;
+    if (act && l == 0) {
+        if (cnt > cap) {
+            int q = atomicAdd(&D.ctl->nl_q, 1);
+            D.ovq[q] = t;
+            D.ovs[2*q] = make_double4(b0, b1, b2, bd);
+        } else {
+            const double4 ai = a_start(D, P, i);
+            const double a0 = ai.x + b0, a1 = ai.y + b1, a2 = ai.z + b2;
+            D.acc[i] = make_double4(a0, a1, a2, ai.w + bd);
+            double na = sqrt(a0*a0 + a1*a1 + a2*a2);
+            double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
+            dtb = dbits(cand);
+        }
+    }
+    force_partials(D, dtb, nff, nfw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -3202,8 +3349,13 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
             hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, D, P);
             hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
         }
-        hipLaunchKernelGGL((k_force11_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
-        hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        if (P.mixed32) {
+            hipLaunchKernelGGL((k_force32_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        } else {
+            hipLaunchKernelGGL((k_force11_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        }
         return;
     }
     if (P.mixed32) {

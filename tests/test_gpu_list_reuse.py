@@ -151,28 +151,34 @@ def test_deletion_with_reused_lists(monkeypatch):
     g.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("case", list(CASES))
-def test_tile_gather_matches_list_gather(monkeypatch, case):
-    """The FP64 gather per tile (neighbour records staged in LDS, eight lanes per target) against
-    the per-target list gather (PSPH_NO_TILES) and against tiles that mostly read their
-    neighbours from HBM (PSPH_TILE_ITEMS=64: few tiles fit): the same pairs, and states that
-    agree to the rounding of the regrouped sums."""
+def test_tile_gather_matches_list_gather(monkeypatch, case, precision):
+    """The gather per tile (neighbour records staged in LDS, eight lanes per target; FP32-mixed:
+    the cell offsets from the staged cells) against the per-target list gather (PSPH_NO_TILES)
+    and against tiles that mostly read their neighbours from HBM (PSPH_TILE_ITEMS=64: few tiles
+    fit): the same pairs, and states that agree to the rounding of the regrouped sums (FP32
+    sums for the FP32-mixed tier)."""
     cfg, p, dt = CASES[case]()
+    if precision == "fp32":
+        cfg = type(cfg).from_buffer_copy(cfg)
+        cfg.precision = FP32_MIXED
     steps = 120
     res = {}
     for name, env in (("tiles", {}), ("lists", {"PSPH_NO_TILES": "1"}), ("fallback", {"PSPH_TILE_ITEMS": "64"})):
         for k in ("PSPH_NO_TILES", "PSPH_TILE_ITEMS"):
             monkeypatch.delenv(k, raising=False)
-        print(f"[tiles] {case} {name}", flush=True)
+        print(f"[tiles] {case} {precision} {name}", flush=True)
         g = _solver(monkeypatch, env, cfg, p, dt)
         st, builds = _run(g, steps, chunk=40)
         res[name] = (st, g.particles())
         g.close()
     st0, R = res["lists"]
+    tol = {} if precision == "fp64" else dict(x=1e-8, v=1e-5, rho=1e-6)
     for name in ("tiles", "fallback"):
         st, G = res[name]
         assert st.steps == st0.steps and st.pairs == st0.pairs, name
-        _close(G, R)
+        _close(G, R, **tol)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
