@@ -262,7 +262,8 @@ struct Dev {
     // row in groups of at most kTileTargets, tile = (first target, targets, free items, wall
     // items; -1 wall items: the tile's neighbours do not fit, it reads them from HBM).  A tile's
     // items are the union of its targets' skin-list neighbours (free ones first, then walls, each
-    // in index order), and its targets' lists hold item slots (16 bits) at tl[tile*tl_stride +
+    // in the order of the cell rows around the tile's row), and its targets' lists hold item
+    // slots (16 bits) at tl[tile*tl_stride +
     // tl_index(target, e)]: lane e % kTileLanes of a target reads its slots four at a time
     int4 *tile;
     int32_t *items;
@@ -275,9 +276,9 @@ constexpr int kTileTargets = 112;
 constexpr int kTileLanes = 8;
 constexpr int kTileItems = 1408;
 constexpr int kTileWalls = 640;
-constexpr int kTileWords = 8192;
+constexpr int kTileWords = 2048;
 constexpr int kSlotWords = 6;          // slot words a lane holds in registers (4 slots each)
-constexpr int kUnionBlocks = 256;      // grid of the tile union (one block per CU)
+constexpr int kUnionBlocks = 1024;     // grid of the tile union (four blocks per CU)
 static_assert(kTileItems <= 2*kTileTargets*kTileLanes, "staging takes at most two items a thread");
 
 }  // namespace psph

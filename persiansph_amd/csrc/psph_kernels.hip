@@ -2126,19 +2126,26 @@ __device__ inline int tl_index(int tt, int e)
     return (((g >> 2)*kTileTargets + tt)*kTileLanes + l)*4 + (g & 3);
 }
 
-// Union of a tile's list entries (list builds only): two bitmaps over the tile's index window
-// (free neighbours and walls), their prefix popcounts give each neighbour its item slot (free
-// ones first), and the targets' lists are rewritten as slots.  The same neighbour met in several
-// lists writes the same item.
+// Union of a tile's list entries (list builds only).  A tile's targets lie in one cell row
+// (cells kx0..kx1), so their skin-list neighbours lie in the W^(DIM-1) rows around it, cells
+// kx0 - REACH .. kx1 + REACH: W^(DIM-1) contiguous runs of the sorted order.  An entry's cell
+// code names its run, and the runs laid end to end index two bitmaps (free neighbours and
+// walls); their prefix popcounts give each neighbour its item slot (free ones first, each kind
+// in run order), and the targets' lists are rewritten as slots.  The same neighbour met in
+// several lists writes the same item.  A tile whose runs exceed the bitmaps, whose items exceed
+// the LDS staging, or with an entry outside its runs is marked to read from HBM.
 template <int REACH>
 __global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
 {
     const Dev D = bind(D0);
     if (D.ctl->halt || !D.ctl->rebuild) return;
+    constexpr int W = Codes<REACH>::W, NQ = W*W;
     __shared__ uint32_t bmf[kTileWords], bmw[kTileWords];
     __shared__ uint16_t pf[kTileWords], pw[kTileWords];
+    __shared__ int rbeg[NQ], rlen[NQ], roff[NQ + 1];
+    __shared__ int scnt[kTileTargets], smax[2];
     __shared__ int sh[17];
-    __shared__ int jlo, jhi;
+    __shared__ int bad;
     const int ntiles = D.ctl->ntiles, cap = P.nl_cap;
     const size_t gs = (size_t)P.nfree;
     // a bounded grid walks the tiles, so steps without a list build cost one wave of blocks
@@ -2146,43 +2153,81 @@ __global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
         const int4 ti = D.tile[tile];
         const int t0 = ti.x, nt = ti.y;
         __syncthreads();                       // the previous tile's reads of the shared arrays
-        if (threadIdx.x == 0) { jlo = 0x7fffffff; jhi = -1; }
-        __syncthreads();
-        auto entry = [&](int k, int &tt, int &e) -> bool {
-            tt = k / cap;
-            e = k - tt*cap;
-            const int c = D.ncnt[t0 + tt];
-            return e < (c < cap ? c : cap);
-        };
-        auto list_j = [&](int tt, int e) {
-            const int v = D.nl[(((size_t)(e >> 2))*gs + (size_t)(t0 + tt))*4 + (e & 3)];
-            return nl_index<REACH>(v);
-        };
-        int lo = 0x7fffffff, hi = -1;
-        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
-            int tt, e;
-            if (!entry(k, tt, e)) continue;
-            const int j = list_j(tt, e);
-            lo = j < lo ? j : lo;
-            hi = j > hi ? j : hi;
+        if (threadIdx.x < 64) {
+            // wave 0, lane q < NQ: run q = code / W, the row (dy, dz) = (q % W, q / W) - REACH
+            // (dz = 0 in 2-D); the runs' offsets by a wave scan
+            const int q = threadIdx.x;
+            int beg = 0, len = 0;
+            if (q < NQ) {
+                int kx0, kx1, ky, kz;
+                decode_key(P, D.key[D.free_list[t0]], kx0, ky, kz);
+                decode_key(P, D.key[D.free_list[t0 + nt - 1]], kx1, ky, kz);
+                const int yy = ky + q % W - REACH, zz = kz + q / W - REACH;
+                if (yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
+                    const int base = P.nc[0]*(yy + P.nc[1]*zz);
+                    const int xa = max(kx0 - REACH, 0), xb = min(kx1 + REACH, P.nc[0] - 1);
+                    beg = D.cstart[base + xa];
+                    len = D.cstart[base + xb + 1] - beg;
+                }
+            }
+            int x = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (q >= o) x += y;
+            }
+            if (q < NQ) { rbeg[q] = beg; rlen[q] = len; roff[q] = x - len; }
+            if (q == NQ - 1) roff[NQ] = x;
+            if (q == 0) bad = 0;
+        } else if (threadIdx.x < 64 + 128) {
+            // waves 1 and 2: the targets' entry counts and their maximum
+            static_assert(kTileTargets <= 128, "two waves hold a tile's counts");
+            const int tt = (int)threadIdx.x - 64;
+            int m = 0;
+            if (tt < nt) { const int c = D.ncnt[t0 + tt]; m = c < cap ? c : cap; }
+            if (tt < kTileTargets) scnt[tt] = m;
+            for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+            if ((threadIdx.x & 63) == 0) smax[((int)threadIdx.x >> 6) - 1] = m;
         }
-        if (hi >= 0) { atomicMin(&jlo, lo); atomicMax(&jhi, hi); }
         __syncthreads();
-        const int base = jlo;
-        const int nwords = (jhi >= jlo) ? ((jhi - jlo) >> 5) + 1 : 0;
+        const int ngrp = (max(smax[0], smax[1]) + 3) >> 2;
+        const int nwords = (roff[NQ] + 31) >> 5;
         if (nwords > kTileWords) {
             if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, 0, -1);
             continue;
         }
         for (int w = threadIdx.x; w < nwords; w += blockDim.x) { bmf[w] = 0u; bmw[w] = 0u; }
         __syncthreads();
-        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
-            int tt, e;
-            if (!entry(k, tt, e)) continue;
-            const int j = list_j(tt, e), b = j - base;
-            atomicOr(flag_free(D.flags_b[j]) ? &bmf[b >> 5] : &bmw[b >> 5], 1u << (b & 31));
+        auto bit_of = [&](int v) {
+            const int j = nl_index<REACH>(v), q = nl_code<REACH>(v)/W;
+            const int d = j - rbeg[q];
+            return ((unsigned)d < (unsigned)rlen[q]) ? roff[q] + d : -1;
+        };
+        // the lists by groups of four entries: group g of the tile's targets is one contiguous
+        // run of nt 16-B words, so each thread loads whole words and the neighbours' flags of
+        // its four entries together
+        const int nwd = ngrp*nt;
+        const int4 *l4 = reinterpret_cast<const int4 *>(D.nl);
+        for (int k = threadIdx.x; k < nwd; k += blockDim.x) {
+            const int g = k / nt, tt = k - g*nt, m = scnt[tt];
+            if (4*g >= m) continue;
+            const int4 w4 = l4[(size_t)g*gs + (size_t)(t0 + tt)];
+            const int vs[4] = {w4.x, w4.y, w4.z, w4.w};
+            uint32_t fl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) fl[u] = (4*g + u < m) ? D.flags_b[nl_index<REACH>(vs[u])] : 0u;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (4*g + u >= m) break;
+                const int b = bit_of(vs[u]);
+                if (b < 0) { bad = 1; continue; }
+                atomicOr(flag_free(fl[u]) ? &bmf[b >> 5] : &bmw[b >> 5], 1u << (b & 31));
+            }
         }
         __syncthreads();
+        if (bad) {
+            if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, 0, -1);
+            continue;
+        }
         // prefix popcounts: each thread a contiguous run of words
         const int per = (nwords + (int)blockDim.x - 1)/(int)blockDim.x;
         const int w0 = (int)threadIdx.x*per, w1 = min(w0 + per, nwords);
@@ -2202,15 +2247,22 @@ __global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
         __syncthreads();
         int32_t *items = D.items + (size_t)tile*kTileItems;
         uint16_t *tl = D.tl + (size_t)tile*P.tl_stride;
-        for (int k = threadIdx.x; k < nt*cap; k += blockDim.x) {
-            int tt, e;
-            if (!entry(k, tt, e)) continue;
-            const int j = list_j(tt, e), b = j - base, w = b >> 5;
-            const uint32_t below = (1u << (b & 31)) - 1u;
-            const int slot = flag_free(D.flags_b[j]) ? pf[w] + __popc(bmf[w] & below)
-                                                     : uf + pw[w] + __popc(bmw[w] & below);
-            items[slot] = j;
-            tl[tl_index(tt, e)] = (uint16_t)slot;
+        for (int k = threadIdx.x; k < nwd; k += blockDim.x) {
+            const int g = k / nt, tt = k - g*nt, m = scnt[tt];
+            if (4*g >= m) continue;
+            const int4 w4 = l4[(size_t)g*gs + (size_t)(t0 + tt)];
+            const int vs[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = 4*g + u;
+                if (e >= m) break;
+                const int b = bit_of(vs[u]), w = b >> 5;
+                const uint32_t bit = 1u << (b & 31), below = bit - 1u;
+                const int slot = (bmf[w] & bit) ? pf[w] + __popc(bmf[w] & below)
+                                                : uf + pw[w] + __popc(bmw[w] & below);
+                items[slot] = nl_index<REACH>(vs[u]);
+                tl[tl_index(tt, e)] = (uint16_t)slot;
+            }
         }
         if (threadIdx.x == 0) D.tile[tile] = make_int4(t0, nt, uf, uw);
     }
