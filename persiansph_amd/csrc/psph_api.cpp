@@ -1333,6 +1333,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                   mu_uniform && P.nclasses <= 8 && have_free && !P.any_periodic &&
                   !std::getenv("PSPH_GENERAL_KERNELS");
         P.fast11 = ok ? 1 : 0;
+        P.ab_two = 0;
         if (ok) {
             double hh = P.h_uniform;
             P.mu_ff = 2.0*mu0*mu0/(mu0+mu0);
@@ -1347,6 +1348,24 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                     P.pair_alpha[a*8+b] = (h->classes[a].Alpha + h->classes[b].Alpha)/2.0;
                     P.pair_beta[a*8+b] = (h->classes[a].Beta + h->classes[b].Beta)/2.0;
                 }
+            // one free class, and the same pair constants with every fixed class: the gathers
+            // take (alpha, beta) from the neighbour's kind instead of the class-pair table
+            int cf = -1, nfc = 0;
+            for (int a = 0; a < P.nclasses; ++a)
+                if (h->classes[a].IsFree) { cf = a; ++nfc; }
+            P.ab_two = nfc == 1 ? 1 : 0;
+            P.ab_free[0] = P.ab_free[1] = P.ab_fixed[0] = P.ab_fixed[1] = 0.0;
+            if (P.ab_two) {
+                P.ab_free[0] = P.pair_alpha[cf*8+cf];
+                P.ab_free[1] = P.pair_beta[cf*8+cf];
+                bool first = true;
+                for (int b = 0; b < P.nclasses; ++b) {
+                    if (h->classes[b].IsFree) continue;
+                    const double al = P.pair_alpha[cf*8+b], be = P.pair_beta[cf*8+b];
+                    if (first) { P.ab_fixed[0] = al; P.ab_fixed[1] = be; first = false; }
+                    else if (al != P.ab_fixed[0] || be != P.ab_fixed[1]) P.ab_two = 0;
+                }
+            }
         }
     }
     P.soil32 = 0;

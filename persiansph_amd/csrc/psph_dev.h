@@ -125,6 +125,10 @@ struct Params {
     double kw, kg, kgh, invh, h2c;     // kernel constants: C of W, C of (1/r)dW/dr, C/h, 1/h, 0.01 h^2
     double sup2_lo, sup2_hi;           // (Cellfac h)^2 band around the exact rij/h <= Cellfac test
     double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
+    // every free particle of one class whose pair constants with any fixed particle are the
+    // same: (alpha, beta) of a free-free pair (ab_free) and of a free-fixed pair (ab_fixed)
+    int ab_two;
+    double ab_free[2], ab_fixed[2];
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;
     // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4), and
@@ -272,10 +276,10 @@ struct Dev {
 
 // tiles of the FP64 fast path: targets per tile, lanes per target, the items and wall items
 // whose gather records fit a workgroup's LDS, and the bitmap words of the union build
-constexpr int kTileTargets = 112;
+constexpr int kTileTargets = 64;
 constexpr int kTileLanes = 8;
-constexpr int kTileItems = 1408;
-constexpr int kTileWalls = 640;
+constexpr int kTileItems = 720;
+constexpr int kTileWalls = 320;
 constexpr int kTileWords = 2048;
 constexpr int kSlotWords = 6;          // slot words a lane holds in registers (4 slots each)
 constexpr int kUnionBlocks = 1024;     // grid of the tile union (four blocks per CU)

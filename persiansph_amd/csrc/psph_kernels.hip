@@ -1761,14 +1761,45 @@ __device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, 
 
 struct Acc11 { double a0, a1, a2, dd; };
 
+// Artificial-viscosity constants (alpha, beta) of a pair of a target of class ci with a
+// neighbour (meta: its class, or -1 - class for a fixed particle): ABTable reads them from the
+// class-pair table in LDS, ABTwo from two register pairs chosen by the neighbour's kind (every
+// free particle of one class, the same constants with every fixed class: Params::ab_two).
+template <class T>
+struct ABTable {
+    const T *pa, *pb;
+    __device__ void get(int ci, T meta, bool freej, T &alpha, T &beta) const {
+        const int cj = freej ? (int)meta : (int)((T)-1 - meta);
+        alpha = pa[ci*8 + cj];
+        beta = pb[ci*8 + cj];
+    }
+};
+template <class T>
+struct ABTwo {
+    T af, bf, aw, bw;
+    __device__ void get(int, T, bool freej, T &alpha, T &beta) const {
+        alpha = freej ? af : aw;
+        beta = freej ? bf : bw;
+    }
+};
+template <bool TWO, class T> struct ABPick;
+template <class T> struct ABPick<true, T> {
+    static __device__ ABTwo<T> make(const Params &P, const T *, const T *) {
+        return {(T)P.ab_free[0], (T)P.ab_free[1], (T)P.ab_fixed[0], (T)P.ab_fixed[1]};
+    }
+};
+template <class T> struct ABPick<false, T> {
+    static __device__ ABTable<T> make(const Params &, const T *pa, const T *pb) { return {pa, pb}; }
+};
+
 // SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
 // current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
 // target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
 // The pair of free target i (views gi0..gi2, class ci) with the neighbour whose records are
 // a0..a2 (and vw, its viscous velocity, read only for walls).  jf() gives the neighbour's index
 // in the current order; it is called only for the rare key-adjacency part of the skin test.
-template <int DIM, bool SKIN, class JF>
-__device__ inline void pair11_rec(const Dev &D, const Params &P, const double *pa, const double *pb,
+template <int DIM, bool SKIN, class AB, class JF>
+__device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, JF jf,
                                   double4 a0, double4 a1, double4 a2, double4 vw,
                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
@@ -1804,10 +1835,10 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const double *p
     const bool freej = meta >= 0.0;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    int cj = freej ? (int)meta : (int)(-1.0 - meta);
     // viscous velocity of the neighbour: a free neighbour's is its v
     double w0 = freej ? a1.x : vw.x, w1 = freej ? a1.y : vw.y, w2 = freej ? a1.z : vw.z;
-    double alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
+    double alpha, beta;
+    ab.get(ci, meta, freej, alpha, beta);
     double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     double dvx = u0*x0 + u1*x1 + u2*x2;
     double di = gi0.w, dj = a0.w;
@@ -1841,7 +1872,7 @@ __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
     double4 vw = a1;
     if (a2.y < 0.0) vw = D.gw[j];
-    pair11_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
+    pair11_rec<DIM, SKIN>(D, P, ABTable<double>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
                           mu_ff, mu_w, A, nff, nfw);
 }
 
@@ -2272,14 +2303,17 @@ __global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
 // list order, the first lane starting from the target's initial acceleration; the lanes' sums
 // meet as ((l0 + l1) + (l2 + l3)) + ((l4 + l5) + (l6 + l7)).  A target whose list overflowed
 // queues the result for k_force11_nl_over, which adds the entries past the capacity.
-template <int DIM>
+template <int DIM, bool AB2>
 __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0, Params P)
 {
     const Dev D = bind(D0);
     if (D.ctl->halt) return;
     const int nb = D.ctl->ntiles;
     if ((int)blockIdx.x >= nb) return;
-    __shared__ double4 s0[kTileItems], s1[kTileItems], s2[kTileItems], sw[kTileWalls];
+    // the staged records in 16-B halves (x y | z d | vx vy | vz P/d^2 | c meta | m 1/d; walls'
+    // viscous velocity), so that lanes reading different slots spread over all the LDS banks
+    __shared__ double2 sxy[kTileItems], szd[kTileItems], svv[kTileItems], svp[kTileItems],
+                       scm[kTileItems], smr[kTileItems], swa[kTileWalls], swb[kTileWalls];
     __shared__ double pa[64], pb[64];
     const int tile = xcd_remap((int)blockIdx.x, nb);
     const int4 ti = D.tile[tile];
@@ -2314,8 +2348,19 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
         double4 r00, r01, r02, r0w, r10, r11, r12, r1w;
         if (j0 >= 0) { r00 = D.g0[j0]; r01 = D.g1[j0]; r02 = D.g2[j0]; if (u0 >= uf) r0w = D.gw[j0]; }
         if (j1 >= 0) { r10 = D.g0[j1]; r11 = D.g1[j1]; r12 = D.g2[j1]; if (u1 >= uf) r1w = D.gw[j1]; }
-        if (j0 >= 0) { s0[u0] = r00; s1[u0] = r01; s2[u0] = r02; if (u0 >= uf) sw[u0 - uf] = r0w; }
-        if (j1 >= 0) { s0[u1] = r10; s1[u1] = r11; s2[u1] = r12; if (u1 >= uf) sw[u1 - uf] = r1w; }
+        auto put = [&](int u, double4 r0, double4 r1, double4 r2) {
+            sxy[u] = make_double2(r0.x, r0.y); szd[u] = make_double2(r0.z, r0.w);
+            svv[u] = make_double2(r1.x, r1.y); svp[u] = make_double2(r1.z, r1.w);
+            scm[u] = make_double2(r2.x, r2.y); smr[u] = make_double2(r2.z, r2.w);
+        };
+        if (j0 >= 0) {
+            put(u0, r00, r01, r02);
+            if (u0 >= uf) { swa[u0 - uf] = make_double2(r0w.x, r0w.y); swb[u0 - uf] = make_double2(r0w.z, r0w.w); }
+        }
+        if (j1 >= 0) {
+            put(u1, r10, r11, r12);
+            if (u1 >= uf) { swa[u1 - uf] = make_double2(r1w.x, r1w.y); swb[u1 - uf] = make_double2(r1w.z, r1w.w); }
+        }
     }
     __syncthreads();
     const int ci = (int)gi2.y;
@@ -2327,6 +2372,9 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
     }
     unsigned nff = 0, nfw = 0;
     if (staged) {
+        // slots from the lane's slot words, records from LDS (a wall's viscous velocity only
+        // for walls; a free neighbour's is its v)
+        const auto ab = ABPick<AB2, double>::make(P, pa, pb);
         uint64_t q = 0ull;
         for (int e = l, g = 0; e < m; e += kTileLanes, ++g) {
             if ((g & 3) == 0) {
@@ -2338,11 +2386,16 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
                 if (G >= kSlotWords) q = tq[(size_t)G*kWordStride];
             }
             const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
-            // all four records read at once; the wall record's slot is clamped for free ones
-            // a wall's viscous velocity; read for every slot (clamped), used only for walls
-            const double4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
-            const double4 w = sw[slot >= uf ? slot - uf : 0];
-            pair11_rec<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
+            const double2 r0 = sxy[slot], r1 = szd[slot], r2 = svv[slot], r3 = svp[slot],
+                          r4 = scm[slot], r5 = smr[slot];
+            const double4 a0 = make_double4(r0.x, r0.y, r1.x, r1.y), a1 = make_double4(r2.x, r2.y, r3.x, r3.y),
+                          a2 = make_double4(r4.x, r4.y, r5.x, r5.y);
+            double4 w = a1;
+            if (r4.y < 0.0) {
+                const double2 wa = swa[slot - uf], wb = swb[slot - uf];
+                w = make_double4(wa.x, wa.y, wb.x, wb.y);
+            }
+            pair11_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
                                   a0, a1, a2, w, mu_ff, mu_w, A, nff, nfw);
         }
     } else {
@@ -2413,8 +2466,8 @@ __device__ inline bool skin_member32(const Dev &D, const Params &P, const Band32
 // records are a0..a2 and vw (its viscous velocity, read only for walls); jf() gives its index,
 // called only inside the FP32 test's band.  o = (k_j - k_i) CellSize, the offset of the two
 // particles' cells at the last sort.
-template <int DIM, bool SKIN, class JF>
-__device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const float *pa, const float *pb,
+template <int DIM, bool SKIN, class AB, class JF>
+__device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &ab,
                                       float4 gi0, float4 gi1, float4 gi2, int ci, int i, JF jf,
                                       float4 a0, float4 a1, float4 a2, float4 vw,
                                       float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
@@ -2445,10 +2498,10 @@ __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const float
     const bool freej = meta >= 0.0f;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    int cj = freej ? (int)meta : (int)(-1.0f - meta);
     float w0 = a1.x, w1 = a1.y, w2 = a1.z;               // viscous velocity of the neighbour
     if (!freej) { w0 = vw.x; w1 = vw.y; w2 = vw.z; }
-    float alpha = pa[ci*8 + cj], beta = pb[ci*8 + cj];
+    float alpha, beta;
+    ab.get(ci, meta, freej, alpha, beta);
     float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     float dvx = u0*x0 + u1*x1 + u2*x2;
     float di = gi0.w, dj = a0.w;
@@ -2483,7 +2536,7 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
 {
     const float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
     const float4 vw = (a2.y < 0.0f) ? D.fw[j] : a1;
-    pair11_f32_rec<DIM, SKIN>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
+    pair11_f32_rec<DIM, SKIN>(D, P, ABTable<float>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
                               o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
 }
 
@@ -2617,7 +2670,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
 // offset from the two cells.  Lane l of a target adds entries l, l + kTileLanes, ... in list order
 // in FP32 (Kahan-compensated continuity per lane); the lanes' sums are widened and meet in FP64
 // in the fixed order of k_force11_tile.  Overflowed targets go to k_force11_f32_nl_over.
-template <int DIM>
+template <int DIM, bool AB2>
 __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -2674,6 +2727,8 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
     unsigned nff = 0, nfw = 0;
     if (staged) {
+        // a wall's viscous velocity read only for walls; a free neighbour's is its v
+        const auto ab = ABPick<AB2, float>::make(P, pa, pb);
         const int ki = pack(D.key_b[i]);
         const int kix = ki & 1023, kiy = (ki >> 10) & 1023, kiz = ki >> 20;
         uint64_t q = 0ull;
@@ -2687,11 +2742,12 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             }
             const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
             const float4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
-            const float4 w = sw[slot >= uf ? slot - uf : 0];
+            float4 w = a1;
+            if (a2.y < 0.0f) w = sw[slot - uf];
             const int kj = sk[slot];
             const float o0 = (float)((kj & 1023) - kix)*csx, o1 = (float)(((kj >> 10) & 1023) - kiy)*csy;
             const float o2 = (DIM == 3) ? (float)((kj >> 20) - kiz)*csz : 0.0f;
-            pair11_f32_rec<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
+            pair11_f32_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
                                       a0, a1, a2, w, o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c,
                                       mu_ff, mu_w, band);
         }
@@ -3402,10 +3458,12 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
             hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
         }
         if (P.mixed32) {
-            hipLaunchKernelGGL((k_force32_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            if (P.ab_two) hipLaunchKernelGGL((k_force32_tile<DIM, true>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else hipLaunchKernelGGL((k_force32_tile<DIM, false>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         } else {
-            hipLaunchKernelGGL((k_force11_tile<DIM>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            if (P.ab_two) hipLaunchKernelGGL((k_force11_tile<DIM, true>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else hipLaunchKernelGGL((k_force11_tile<DIM, false>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         }
         return;
