@@ -1333,7 +1333,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                   mu_uniform && P.nclasses <= 8 && have_free && !P.any_periodic &&
                   !std::getenv("PSPH_GENERAL_KERNELS");
         P.fast11 = ok ? 1 : 0;
-        P.ab_two = 0;
+        P.ab_two = P.ab_one = 0;
         if (ok) {
             double hh = P.h_uniform;
             P.mu_ff = 2.0*mu0*mu0/(mu0+mu0);
@@ -1365,6 +1365,8 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                     if (first) { P.ab_fixed[0] = al; P.ab_fixed[1] = be; first = false; }
                     else if (al != P.ab_fixed[0] || be != P.ab_fixed[1]) P.ab_two = 0;
                 }
+                if (first) { P.ab_fixed[0] = P.ab_free[0]; P.ab_fixed[1] = P.ab_free[1]; }
+                P.ab_one = P.ab_two && P.ab_fixed[0] == P.ab_free[0] && P.ab_fixed[1] == P.ab_free[1];
             }
         }
     }

@@ -126,8 +126,9 @@ struct Params {
     double sup2_lo, sup2_hi;           // (Cellfac h)^2 band around the exact rij/h <= Cellfac test
     double pair_alpha[64], pair_beta[64];   // (Alpha_i + Alpha_j)/2 per class pair (<= 8 classes)
     // every free particle of one class whose pair constants with any fixed particle are the
-    // same: (alpha, beta) of a free-free pair (ab_free) and of a free-fixed pair (ab_fixed)
-    int ab_two;
+    // same: (alpha, beta) of a free-free pair (ab_free) and of a free-fixed pair (ab_fixed);
+    // ab_one: the two pairs are equal
+    int ab_two, ab_one;
     double ab_free[2], ab_fixed[2];
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;

@@ -1762,9 +1762,12 @@ __device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, 
 struct Acc11 { double a0, a1, a2, dd; };
 
 // Artificial-viscosity constants (alpha, beta) of a pair of a target of class ci with a
-// neighbour (meta: its class, or -1 - class for a fixed particle): ABTable reads them from the
-// class-pair table in LDS, ABTwo from two register pairs chosen by the neighbour's kind (every
-// free particle of one class, the same constants with every fixed class: Params::ab_two).
+// neighbour (meta: its class, or -1 - class for a fixed particle), and whether the pair has an
+// artificial viscosity term at all (on: alpha or beta nonzero).  ABTable reads them from the
+// class-pair table in LDS; ABTwo chooses between two register pairs by the neighbour's kind
+// (every free particle of one class, the same constants with every fixed class:
+// Params::ab_two); ABOne holds the one pair of constants of such a run whose free-free and
+// free-fixed constants are equal (Params::ab_one).
 template <class T>
 struct ABTable {
     const T *pa, *pb;
@@ -1773,22 +1776,39 @@ struct ABTable {
         alpha = pa[ci*8 + cj];
         beta = pb[ci*8 + cj];
     }
+    __device__ bool on(T alpha, T beta, bool) const { return alpha != (T)0 || beta != (T)0; }
 };
 template <class T>
 struct ABTwo {
     T af, bf, aw, bw;
+    bool onf, onw;
     __device__ void get(int, T, bool freej, T &alpha, T &beta) const {
         alpha = freej ? af : aw;
         beta = freej ? bf : bw;
     }
+    __device__ bool on(T, T, bool freej) const { return freej ? onf : onw; }
 };
-template <bool TWO, class T> struct ABPick;
-template <class T> struct ABPick<true, T> {
-    static __device__ ABTwo<T> make(const Params &P, const T *, const T *) {
-        return {(T)P.ab_free[0], (T)P.ab_free[1], (T)P.ab_fixed[0], (T)P.ab_fixed[1]};
+template <class T>
+struct ABOne {
+    T a, b;
+    bool any;
+    __device__ void get(int, T, bool, T &alpha, T &beta) const { alpha = a; beta = b; }
+    __device__ bool on(T, T, bool) const { return any; }
+};
+// ABM: 0 the class-pair table, 1 two pairs by the neighbour's kind, 2 one pair
+template <int ABM, class T> struct ABPick;
+template <class T> struct ABPick<2, T> {
+    static __device__ ABOne<T> make(const Params &P, const T *, const T *) {
+        return {(T)P.ab_free[0], (T)P.ab_free[1], P.ab_free[0] != 0.0 || P.ab_free[1] != 0.0};
     }
 };
-template <class T> struct ABPick<false, T> {
+template <class T> struct ABPick<1, T> {
+    static __device__ ABTwo<T> make(const Params &P, const T *, const T *) {
+        return {(T)P.ab_free[0], (T)P.ab_free[1], (T)P.ab_fixed[0], (T)P.ab_fixed[1],
+                P.ab_free[0] != 0.0 || P.ab_free[1] != 0.0, P.ab_fixed[0] != 0.0 || P.ab_fixed[1] != 0.0};
+    }
+};
+template <class T> struct ABPick<0, T> {
     static __device__ ABTable<T> make(const Params &, const T *pa, const T *pb) { return {pa, pb}; }
 };
 
@@ -1802,7 +1822,7 @@ template <int DIM, bool SKIN, class AB, class JF>
 __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, JF jf,
                                   double4 a0, double4 a1, double4 a2, double4 vw,
-                                  double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
+                                  double vsf, double vsw, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
     const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
@@ -1835,22 +1855,22 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
     const bool freej = meta >= 0.0;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    // viscous velocity of the neighbour: a free neighbour's is its v
-    double w0 = freej ? a1.x : vw.x, w1 = freej ? a1.y : vw.y, w2 = freej ? a1.z : vw.z;
+    // viscous velocity of the neighbour (the caller passes a free neighbour's v)
+    const double w0 = vw.x, w1 = vw.y, w2 = vw.z;
     double alpha, beta;
     ab.get(ci, meta, freej, alpha, beta);
     double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     double dvx = u0*x0 + u1*x1 + u2*x2;
     double di = gi0.w, dj = a0.w;
     double PIij = 0.0;
-    if ((alpha != 0.0 || beta != 0.0) && dvx < 0.0) {
+    if (ab.on(alpha, beta, freej) && dvx < 0.0) {
         double MU = P.h_uniform*dvx*rcp_nr(r2 + P.h2c);
         PIij = (-alpha*0.5*(gi2.x + cj_snd)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
     }
     double mi = gi2.z;
     double mj = freej ? mrec : mrec*mi;
-    double mu = freej ? mu_ff : mu_w;
-    double vs = 2.0*mu*gi2.w*invdj*GK;
+    // vsf, vsw: 2 mu / d_i of the target with a free and with a fixed neighbour
+    double vs = (freej ? vsf : vsw)*invdj*GK;
     double s = -(gi1.w + a1.w + PIij)*GK;
     double b0 = gi1.x - w0, b1 = gi1.y - w1, b2 = gi1.z - w2;
     double t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
@@ -1867,13 +1887,13 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
 template <int DIM, bool SKIN>
 __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
                                    double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
-                                   double mu_ff, double mu_w, Acc11 &A, unsigned &nff, unsigned &nfw)
+                                   double vsf, double vsw, Acc11 &A, unsigned &nff, unsigned &nfw)
 {
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
     double4 vw = a1;
     if (a2.y < 0.0) vw = D.gw[j];
     pair11_rec<DIM, SKIN>(D, P, ABTable<double>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
-                          mu_ff, mu_w, A, nff, nfw);
+                          vsf, vsw, A, nff, nfw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1992,7 +2012,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
     int i = act ? D.free_list[t] : 0;
     double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i];
     int ci = (int)gi2.y;
-    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
+    const double vsf = 2.0*P.mu_ff*gi2.w, vsw = 2.0*P.mu_ff_wall*gi2.w;   // 2 mu / d_i
     Acc11 A;
     {
         const double4 a_init = act ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -2011,7 +2031,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
         pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, codes ? nl_index<REACH>(v) : v,
-                                         mu_ff, mu_w, A, nff, nfw);
+                                         vsf, vsw, A, nff, nfw);
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_nl_over continues this target from these sums
@@ -2044,7 +2064,6 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params 
     __shared__ double pa[64], pb[64];
     if (threadIdx.x < 64) { pa[threadIdx.x] = P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = P.pair_beta[threadIdx.x]; }
     __syncthreads();
-    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
     const int cap = P.nl_cap;
     const double reach2 = P.reach2;
     const double cull2 = ((REACH == 2) ? reach2 : P.sup2_hi)*(1.0 + 1e-9);
@@ -2053,6 +2072,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params 
         int i = D.free_list[t];
         double4 gi0 = D.g0[i], gi1 = D.g1[i], gi2 = D.g2[i];
         int ci = (int)gi2.y;
+        const double vsf = 2.0*P.mu_ff*gi2.w, vsw = 2.0*P.mu_ff_wall*gi2.w;   // 2 mu / d_i
         double4 s0 = D.ovs[2*q];
         Acc11 A;
         A.a0 = s0.x; A.a1 = s0.y; A.a2 = s0.z; A.dd = s0.w;
@@ -2062,7 +2082,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params 
         int k = 0;
         walk_rows<DIM, REACH>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, cull2, [&](int j, int, double r2) {
             if (!((REACH == 2) ? (r2 <= reach2) : in_support(P, r2))) return;      // a list entry
-            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, mu_ff, mu_w, A, nff, nfw);
+            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, vsf, vsw, A, nff, nfw);
             ++k;
         });
         D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
@@ -2303,7 +2323,7 @@ __global__ void __launch_bounds__(512) k_tile_union(Dev D0, Params P)
 // list order, the first lane starting from the target's initial acceleration; the lanes' sums
 // meet as ((l0 + l1) + (l2 + l3)) + ((l4 + l5) + (l6 + l7)).  A target whose list overflowed
 // queues the result for k_force11_nl_over, which adds the entries past the capacity.
-template <int DIM, bool AB2>
+template <int DIM, int ABM>
 __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -2364,7 +2384,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
     }
     __syncthreads();
     const int ci = (int)gi2.y;
-    const double mu_ff = P.mu_ff, mu_w = P.mu_ff_wall;
+    const double vsf = 2.0*P.mu_ff*gi2.w, vsw = 2.0*P.mu_ff_wall*gi2.w;   // 2 mu / d_i
     Acc11 A;
     {
         const double4 a_init = (act && l == 0) ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -2374,7 +2394,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
     if (staged) {
         // slots from the lane's slot words, records from LDS (a wall's viscous velocity only
         // for walls; a free neighbour's is its v)
-        const auto ab = ABPick<AB2, double>::make(P, pa, pb);
+        const auto ab = ABPick<ABM, double>::make(P, pa, pb);
         uint64_t q = 0ull;
         for (int e = l, g = 0; e < m; e += kTileLanes, ++g) {
             if ((g & 3) == 0) {
@@ -2396,13 +2416,13 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
                 w = make_double4(wa.x, wa.y, wb.x, wb.y);
             }
             pair11_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
-                                  a0, a1, a2, w, mu_ff, mu_w, A, nff, nfw);
+                                  a0, a1, a2, w, vsf, vsw, A, nff, nfw);
         }
     } else {
         const size_t gs = (size_t)P.nfree;
         for (int e = l; e < m; e += kTileLanes) {
             const int j = nl_index<2>(D.nl[(((size_t)(e >> 2))*gs + (size_t)t)*4 + (e & 3)]);
-            pair11_fast<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, mu_ff, mu_w, A, nff, nfw);
+            pair11_fast<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, vsf, vsw, A, nff, nfw);
         }
     }
     // the lanes' sums meet in a fixed order
@@ -2472,7 +2492,7 @@ __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &a
                                       float4 a0, float4 a1, float4 a2, float4 vw,
                                       float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
                                       float kw, float kg, float kgh, float invh, float h, float h2c,
-                                      float mu_ff, float mu_w, const Band32 &band)
+                                      float vsf, float vsw, const Band32 &band)
 {
     const float cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     float x0 = (gi0.x - a0.x) - o0, x1 = (gi0.y - a0.y) - o1, x2 = (gi0.z - a0.z) - o2;
@@ -2498,22 +2518,20 @@ __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &a
     const bool freej = meta >= 0.0f;
     nff += freej ? 1u : 0u;
     nfw += freej ? 0u : 1u;
-    float w0 = a1.x, w1 = a1.y, w2 = a1.z;               // viscous velocity of the neighbour
-    if (!freej) { w0 = vw.x; w1 = vw.y; w2 = vw.z; }
+    const float w0 = vw.x, w1 = vw.y, w2 = vw.z;         // viscous velocity (a free neighbour's v)
     float alpha, beta;
     ab.get(ci, meta, freej, alpha, beta);
     float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     float dvx = u0*x0 + u1*x1 + u2*x2;
     float di = gi0.w, dj = a0.w;
     float PIij = 0.0f;
-    if ((alpha != 0.0f || beta != 0.0f) && dvx < 0.0f) {
+    if (ab.on(alpha, beta, freej) && dvx < 0.0f) {
         float MU = h*dvx*__builtin_amdgcn_rcpf(r2 + h2c);
         PIij = (-alpha*0.5f*(gi2.x + cj_snd)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
     }
     float mi = gi2.z;
     float mj = freej ? mrec : mrec*mi;
-    float mu = freej ? mu_ff : mu_w;
-    float vs = 2.0f*mu*gi2.w*invdj*GK;
+    float vs = (freej ? vsf : vsw)*invdj*GK;                 // 2 mu / d_i by the neighbour's kind
     float s = -(gi1.w + a1.w + PIij)*GK;
     float b0 = gi1.x - w0, b1 = gi1.y - w1, b2 = gi1.z - w2;
     float t0 = s*x0 + vs*b0, t1 = s*x1 + vs*b1, t2 = s*x2 + vs*b2;
@@ -2532,12 +2550,12 @@ __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa
                                   float4 gi0, float4 gi1, float4 gi2, int ci, int i, int j,
                                   float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
                                   float kw, float kg, float kgh, float invh, float h, float h2c,
-                                  float mu_ff, float mu_w, const Band32 &band)
+                                  float vsf, float vsw, const Band32 &band)
 {
     const float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
     const float4 vw = (a2.y < 0.0f) ? D.fw[j] : a1;
     pair11_f32_rec<DIM, SKIN>(D, P, ABTable<float>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
-                              o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
+                              o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
 }
 
 // FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
@@ -2560,7 +2578,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
     int ci = (int)gi2.y;
     const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
+    const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
@@ -2575,7 +2594,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
         code_offset<REACH>(c, dx, dy, dz);
         float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
         pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
-                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
+                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
     };
     int4 w = make_int4(0, 0, 0, 0);
     for (int e = 0; e < m; ++e) {
@@ -2619,7 +2638,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
     __syncthreads();
     const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
     const int cap = P.nl_cap;
     const Band32 band(P);
     const double reach2 = P.reach2;
@@ -2629,6 +2648,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
         int i = D.free_list[t];
         float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
         int ci = (int)gi2.y;
+        const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
         double4 s0 = D.ovs[2*q];
         Acc32 A;
         A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
@@ -2649,7 +2669,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
                 code_offset<REACH>(c, dx, dy, dz);
                 float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
                 pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
-                                            A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
+                                            A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
             }
             ++k;
         });
@@ -2670,7 +2690,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
 // offset from the two cells.  Lane l of a target adds entries l, l + kTileLanes, ... in list order
 // in FP32 (Kahan-compensated continuity per lane); the lanes' sums are widened and meet in FP64
 // in the fixed order of k_force11_tile.  Overflowed targets go to k_force11_f32_nl_over.
-template <int DIM, bool AB2>
+template <int DIM, int ABM>
 __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -2721,14 +2741,15 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
     const int ci = (int)gi2.y;
     const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
     const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c, mu_ff = (float)P.mu_ff, mu_w = (float)P.mu_ff_wall;
+    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
+    const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
     unsigned nff = 0, nfw = 0;
     if (staged) {
         // a wall's viscous velocity read only for walls; a free neighbour's is its v
-        const auto ab = ABPick<AB2, float>::make(P, pa, pb);
+        const auto ab = ABPick<ABM, float>::make(P, pa, pb);
         const int ki = pack(D.key_b[i]);
         const int kix = ki & 1023, kiy = (ki >> 10) & 1023, kiz = ki >> 20;
         uint64_t q = 0ull;
@@ -2749,7 +2770,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             const float o2 = (DIM == 3) ? (float)((kj >> 20) - kiz)*csz : 0.0f;
             pair11_f32_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
                                       a0, a1, a2, w, o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c,
-                                      mu_ff, mu_w, band);
+                                      vsf, vsw, band);
         }
     } else {
         const size_t gs = (size_t)P.nfree;
@@ -2759,7 +2780,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             code_offset<2>(nl_code<2>(v), dx, dy, dz);
             const float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
             pair11_f32<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, nl_index<2>(v), p0, p1, p2,
-                                  A, nff, nfw, kw, kg, kgh, invh, h, h2c, mu_ff, mu_w, band);
+                                  A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
         }
     }
     double b0 = (double)A.a0, b1 = (double)A.a1, b2 = (double)A.a2, bd = (double)A.dd + (double)(-A.ddc);
@@ -3458,12 +3479,14 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
             hipLaunchKernelGGL((k_tile_union<REACH>), dim3(std::min(P.tile_max, kUnionBlocks)), dim3(512), 0, s, D, P);
         }
         if (P.mixed32) {
-            if (P.ab_two) hipLaunchKernelGGL((k_force32_tile<DIM, true>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
-            else hipLaunchKernelGGL((k_force32_tile<DIM, false>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            if (P.ab_one) hipLaunchKernelGGL((k_force32_tile<DIM, 2>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else if (P.ab_two) hipLaunchKernelGGL((k_force32_tile<DIM, 1>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else hipLaunchKernelGGL((k_force32_tile<DIM, 0>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         } else {
-            if (P.ab_two) hipLaunchKernelGGL((k_force11_tile<DIM, true>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
-            else hipLaunchKernelGGL((k_force11_tile<DIM, false>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            if (P.ab_one) hipLaunchKernelGGL((k_force11_tile<DIM, 2>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else if (P.ab_two) hipLaunchKernelGGL((k_force11_tile<DIM, 1>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
+            else hipLaunchKernelGGL((k_force11_tile<DIM, 0>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         }
         return;
