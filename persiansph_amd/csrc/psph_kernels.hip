@@ -3470,7 +3470,9 @@ template <int DIM, int REACH>
 static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
 {
     const dim3 gl(nblk(P.nfree, kBlock)), gf(nblk(P.nfree, kForceBlock)), go(kOverBlocks);
-    // the list builds are no-ops on steps that keep the lists; a lean step does not queue them
+    // the list builds are no-ops on steps that keep the lists; a lean step does not queue them,
+    // nor the overflow passes: a list overflows only at a build, and a build that overflowed
+    // makes every following step rebuild (Ctl::list_over) until the host has grown the lists
     if (!P.lean) hipLaunchKernelGGL((k_nlist<DIM, REACH>), gl, dim3(kBlock), 0, s, D, P);
     if (REACH == 2 && P.tiles) {
         const dim3 gt(P.tile_max);
@@ -3482,21 +3484,21 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
             if (P.ab_one) hipLaunchKernelGGL((k_force32_tile<DIM, 2>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             else if (P.ab_two) hipLaunchKernelGGL((k_force32_tile<DIM, 1>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             else hipLaunchKernelGGL((k_force32_tile<DIM, 0>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
-            hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+            if (!P.lean) hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         } else {
             if (P.ab_one) hipLaunchKernelGGL((k_force11_tile<DIM, 2>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             else if (P.ab_two) hipLaunchKernelGGL((k_force11_tile<DIM, 1>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
             else hipLaunchKernelGGL((k_force11_tile<DIM, 0>), gt, dim3(kTileTargets*kTileLanes), 0, s, D, P);
-            hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+            if (!P.lean) hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
         }
         return;
     }
     if (P.mixed32) {
         hipLaunchKernelGGL((k_force11_f32_nl<DIM, REACH>), gf, dim3(kForceBlock), 0, s, D, P);
-        hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        if (!P.lean) hipLaunchKernelGGL((k_force11_f32_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
     } else {
         hipLaunchKernelGGL((k_force11_nl<DIM, REACH>), gf, dim3(kForceBlock), 0, s, D, P);
-        hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
+        if (!P.lean) hipLaunchKernelGGL((k_force11_nl_over<DIM, REACH>), go, dim3(kForceBlock), 0, s, D, P);
     }
 }
 
