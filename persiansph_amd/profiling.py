@@ -7,7 +7,8 @@
     python -m persiansph_amd.profiling traffic --fetch F.db --write W.db --copy-fetch CF.db
             --copy-write CW.db --copy-bytes BYTES --kernels k_nlist,k_force11_nl [--out T.json]
         HBM bytes per dispatch of the named kernels: FETCH_SIZE and WRITE_SIZE (KiB per
-        dispatch) each scaled by bytes / counter of a plain device copy of known size
+        dispatch) each scaled by bytes / counter of a plain device copy of known size; per
+        pass, a kernel dispatched on some passes only counts by its share of the passes
 
 FETCH_SIZE/WRITE_SIZE count the L2's traffic to memory; a copy of B bytes moves B in and B
 out, so B / (counter of the copy in bytes) calibrates each counter.
@@ -37,10 +38,11 @@ def counters(db_paths):
 
 
 def _counter_mean(path, counter, match):
+    """{kernel: (mean per dispatch, dispatches)} of one counter."""
     got = {}
     for k, cs in counters([path]).items():
         if match(k) and counter in cs:
-            got[k] = cs[counter][0]
+            got[k] = cs[counter]
     return got
 
 
@@ -49,12 +51,14 @@ def _is_copy(k):
 
 
 def traffic(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes, kernels):
-    """HBM bytes per dispatch of each kernel whose name contains one of `kernels`."""
+    """HBM bytes per dispatch of each kernel whose name contains one of `kernels`, and per pass:
+    kernels dispatched on some passes only (list builds) count in proportion to their
+    dispatches against the most frequent kernel's."""
     cf = _counter_mean(copy_fetch_db, "FETCH_SIZE", _is_copy)
     cw = _counter_mean(copy_write_db, "WRITE_SIZE", _is_copy)
     # the copy kernel is the one with the largest counter (the probe's own launch)
-    f_copy = max(cf.values())*1024.0
-    w_copy = max(cw.values())*1024.0
+    f_copy = max(v[0] for v in cf.values())*1024.0
+    w_copy = max(v[0] for v in cw.values())*1024.0
     f_scale = copy_bytes/f_copy
     w_scale = copy_bytes/w_copy
     want = lambda k: any(s in k for s in kernels)
@@ -62,14 +66,17 @@ def traffic(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_bytes, kernel
     wr = _counter_mean(write_db, "WRITE_SIZE", want)
     per = {}
     for k in sorted(set(fe) | set(wr)):
-        rb = fe.get(k, 0.0)*1024.0*f_scale
-        wb = wr.get(k, 0.0)*1024.0*w_scale
-        per[k] = {"read_bytes": rb, "write_bytes": wb, "bytes": rb + wb}
+        rb = fe.get(k, (0.0, 0))[0]*1024.0*f_scale
+        wb = wr.get(k, (0.0, 0))[0]*1024.0*w_scale
+        n = max(fe.get(k, (0.0, 0))[1], wr.get(k, (0.0, 0))[1])
+        per[k] = {"read_bytes": rb, "write_bytes": wb, "bytes": rb + wb, "dispatches": n}
+    nmax = max((v["dispatches"] for v in per.values()), default=1) or 1
     return {
         "fetch_scale": f_scale, "write_scale": w_scale,
         "copy_bytes": copy_bytes,
         "kernels": per,
-        "bytes_per_pass": sum(v["bytes"] for v in per.values()),
+        "passes": nmax,
+        "bytes_per_pass": sum(v["bytes"]*v["dispatches"]/nmax for v in per.values()),
     }
 
 
