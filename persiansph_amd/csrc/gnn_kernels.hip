@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "gnn_dev.h"
 
 namespace psgnn {
@@ -21,14 +23,9 @@ namespace psgnn {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-// float -> bf16, round to nearest even (NaN stays NaN)
-__device__ inline uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// float -> bf16, round to nearest even (NaN stays NaN): one v_cvt_pk_bf16_f32 on gfx950
+// (FP32 denormals are kept in these kernels, so the conversion rounds them too)
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // ------------------------------------------------------------------------------------------
@@ -243,141 +240,165 @@ struct EpF32 {            // D = acc (FP32), leading dimension ld
 };
 
 // ---- fused edge update (inference) -------------------------------------------------------
-// e' = e + W2^T relu(W1^T [e | h_recv | h_send] + b1) + b2 for a tile of 128 edges: the first
-// layer's 128 x 128 activation tile stays in LDS as the second layer's A operand, so it never
-// goes to HBM.  Stage k+1 of the first layer's operands is loaded into registers while stage k
-// is multiplied (double-buffered LDS, one barrier per stage).  The result is staged through
-// LDS as FP32 and written with 16-byte row-contiguous stores, the residual read the same way.
-// The operations and their order are those of the unfused pair k_gemm<LdEdgeCat, ., EpBias> +
-// k_gemm<LdRows, ., EpResid>, so the result is the same to the bit.
-constexpr int ZLD = 128 + 8;           // LDS row stride (bf16) of a 128-wide tile
-constexpr int kEdgeStage = 2*BM*LDK + 2*BN*LDK;   // phase 1: A, B double-buffered
-constexpr int kEdgeZW = 2*128*ZLD;                 // phase 2: Z tile and W2^T
-constexpr int kEdgeOut = 2*128*128;                // epilogue: 128 x 128 FP32
-constexpr int kEdgeLds = kEdgeStage > kEdgeZW ? (kEdgeStage > kEdgeOut ? kEdgeStage : kEdgeOut)
-                                              : (kEdgeZW > kEdgeOut ? kEdgeZW : kEdgeOut);
+// e' = e + W2^T relu(W1^T [e | h_recv | h_send] + b1) + b2.  One workgroup per CU keeps both
+// layers' weights in LDS for its whole life (W1^T 96 KB, W2^T 32 KB), and each of its four
+// waves takes 32-edge tiles in turn, with no workgroup barrier after the weights are in:
+//   - the tile's A operand (the 32 x 384 concatenated input) goes from memory straight into the
+//     MFMA operand registers, one 16-B piece of one row per lane and k-step, and the next
+//     tile's pieces are in flight while this tile is multiplied;
+//   - layer 1 (K = 384) accumulates four 32 x 32 output tiles, its ReLU'd BF16 activations go
+//     to the wave's own 8 KB of LDS, which is layer 2's A operand (K = 128);
+//   - the residual input (the first 128 columns of the A operand, still in registers) is put
+//     in the same LDS, added to the result there, and the tile leaves as row-contiguous 16-B
+//     stores.
+// Every weight fragment is read from LDS once per 32 edges and used by one MFMA; the 16-B
+// chunks of each LDS row are XOR-swizzled by the row, so the 32 rows a fragment read touches
+// spread over all banks.  Each output element sees the MFMAs of k_gemm<LdEdgeCat, ., EpBias>
+// and k_gemm<LdRows, ., EpResid> in the same k order with the same operands, and the same
+// bias, ReLU, rounding and residual additions, so the result is the same to the bit.
+constexpr int kPwW1 = 128*384, kPwW2 = 128*128, kPwZ = 32*128;     // bf16 elements
+constexpr int kPwLds = kPwW1 + kPwW2 + 4*kPwZ;                      // 160 KB
+
+// element offset of 16-B chunk c of row r (rows of `len` elements)
+__device__ __forceinline__ int pw_at(int r, int c, int len) { return r*len + ((c ^ (r & 15)) << 3); }
 
 __global__ void __launch_bounds__(256) k_edge_mlp(int M, LdEdgeCat al, const uint16_t *W1t, const float *b1,
                                                   const uint16_t *W2t, const float *b2, uint16_t *Eout)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t lds[kEdgeLds];
-    // the three phases reuse one LDS region: staging, then Z and W2^T, then the FP32 result
-    uint16_t *As = lds, *Bs = lds + 2*BM*LDK;          // phase 1 staging, [buf][row][k]
-    uint16_t *Zs = lds, *Ws = lds + 128*ZLD;            // phase 2: Z [row][k], W2^T [n][k]
-    float *Os = reinterpret_cast<float *>(lds);         // epilogue: FP32 [row][128]
+    __shared__ __attribute__((aligned(16))) uint16_t lds[kPwLds];
+    uint16_t *W1s = lds, *W2s = lds + kPwW1;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wm = w >> 1, wn = w & 1;
-    const int m0 = blockIdx.x*BM;
-    const LdRows bl{W1t, 384};
-    f32x16 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.0f;
-    // phase 1: K = 384 in 12 stages of 32
-    uint4 ra[2], rb[2];
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            int r = (tid >> 2) + 64*p, kc = (tid & 3)*8;
-            ra[p] = (m0 + r < M) ? al.load8(m0 + r, k0 + kc) : make_uint4(0, 0, 0, 0);
-            rb[p] = bl.load8(r, k0 + kc);
-        }
-    };
-    auto stash = [&](int buf) {
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            int r = (tid >> 2) + 64*p, kc = (tid & 3)*8;
-            *(uint4 *)&As[(buf*BM + r)*LDK + kc] = ra[p];
-            *(uint4 *)&Bs[(buf*BN + r)*LDK + kc] = rb[p];
-        }
-    };
-    fetch(0);
-    stash(0);
-    __syncthreads();
-    for (int st = 0; st < 12; ++st) {
-        const int buf = st & 1;
-        if (st + 1 < 12) fetch((st + 1)*BK);
-#pragma unroll
-        for (int ks = 0; ks < BK; ks += 16) {
-            bf16x8 a[2], b[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                int rA = wm*64 + t*32 + (lane & 31);
-                a[t] = *(const bf16x8 *)&As[(buf*BM + rA)*LDK + ks + (lane >> 5)*8];
-                int rB = wn*64 + t*32 + (lane & 31);
-                b[t] = *(const bf16x8 *)&Bs[(buf*BN + rB)*LDK + ks + (lane >> 5)*8];
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        if (st + 1 < 12) stash(buf ^ 1);
-        __syncthreads();
+    uint16_t *Zs = lds + kPwW1 + kPwW2 + w*kPwZ;
+    for (int e = tid; e < 128*48; e += 256) {
+        const int n = e / 48, c = e - n*48;
+        *(uint4 *)&W1s[pw_at(n, c, 384)] = *(const uint4 *)(W1t + (size_t)n*384 + c*8);
     }
-    // Z = bf16(relu(acc + b1)) into LDS; W2^T into the staging space
+    for (int e = tid; e < 128*16; e += 256) {
+        const int n = e >> 4, c = e & 15;
+        *(uint4 *)&W2s[pw_at(n, c, 128)] = *(const uint4 *)(W2t + (size_t)n*128 + c*8);
+    }
+    __syncthreads();
+    const int l31 = lane & 31, h = lane >> 5;
+    float bias1[4], bias2[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j) { bias1[j] = b1[32*j + l31]; bias2[j] = b2[32*j + l31]; }
+    const int ntiles = (M + 31) >> 5, stride = gridDim.x*4;
+    // this lane's 16-B pieces of row l31 of a tile: k-step s covers k = 16 s + 8 h .. + 7.
+    // Rows past the edges read the last edge (the loads stay unconditional, so that the waits
+    // for them are exact); each output row depends on its own input row only, and such rows
+    // are not stored.
+    // The receiver and sender of the row are loaded one fetch ahead (ri, si: those of `tile`
+    // on entry, of the tile after it on return), so no fetch waits on its own index loads.
+    auto row_of = [&](int tile) { const int r = tile*32 + l31; return r < M ? r : M - 1; };
+    auto fetch = [&](int tile, uint4 *A, int &ri, int &si) {
+        const int r = row_of(tile);
+        const uint16_t *pe = al.E + (size_t)r*128 + 8*h;
+        const uint16_t *pr = al.H + (size_t)ri*128 + 8*h;
+        const uint16_t *ps = al.H + (size_t)si*128 + 8*h;
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int s = 0; s < 8; ++s) {
+            A[s] = *(const uint4 *)(pe + 16*s);
+            A[8 + s] = *(const uint4 *)(pr + 16*s);
+            A[16 + s] = *(const uint4 *)(ps + 16*s);
+        }
+        const int rn = row_of(tile + stride);
+        ri = al.recv[rn];
+        si = al.send[rn];
+    };
+    auto tile_pass = [&](int tile, const uint4 *A) {
+        f32x16 acc[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[j][q] = 0.0f;
+        // layer 1: K = 384; the weight fragments of step s + 1 are read while step s multiplies
+        bf16x8 bw[2][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bw[0][j] = *(const bf16x8 *)&W1s[pw_at(32*j + l31, h, 384)];
+#pragma unroll
+        for (int s = 0; s < 24; ++s) {
+            if (s + 1 < 24) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    bw[(s + 1) & 1][j] = *(const bf16x8 *)&W1s[pw_at(32*j + l31, 2*(s + 1) + h, 384)];
+            }
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&A[s]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[s & 1][j], acc[j], 0, 0, 0);
+        }
+        // Z = bf16(relu(acc + b1)), row r = 8 (q / 4) + 4 h + q % 4, column c = 32 j + l31
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                int r = wm*64 + i*32 + 8*(q >> 2) + 4*(lane >> 5) + (q & 3);
-                int c = wn*64 + j*32 + (lane & 31);
-                float v = acc[i][j][q] + b1[c];
+                const int r = 8*(q >> 2) + 4*h + (q & 3), c = 32*j + l31;
+                float v = acc[j][q] + bias1[j];
                 v = v > 0.0f ? v : 0.0f;
-                Zs[r*ZLD + c] = f2bf(v);
-                acc[i][j][q] = 0.0f;
+                Zs[pw_at(r, c >> 3, 128) + (c & 7)] = f2bf(v);
+                acc[j][q] = 0.0f;
             }
-    for (int e = tid; e < 128*16; e += 256) {
-        int n = e >> 4, kc = (e & 15)*8;
-        *(uint4 *)&Ws[n*ZLD + kc] = *(const uint4 *)(W2t + (size_t)n*128 + kc);
-    }
-    __syncthreads();
-    // phase 2: K = 128 from LDS
+        __builtin_amdgcn_wave_barrier();
+        // layer 2: K = 128 from the wave's LDS, fragments one step ahead
+        bf16x8 az[2];
+        az[0] = *(const bf16x8 *)&Zs[pw_at(l31, h, 128)];
 #pragma unroll
-    for (int k0 = 0; k0 < 128; k0 += 16) {
-        bf16x8 a[2], b[2];
+        for (int j = 0; j < 4; ++j) bw[0][j] = *(const bf16x8 *)&W2s[pw_at(32*j + l31, h, 128)];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            int rA = wm*64 + t*32 + (lane & 31);
-            a[t] = *(const bf16x8 *)&Zs[rA*ZLD + k0 + (lane >> 5)*8];
-            int rB = wn*64 + t*32 + (lane & 31);
-            b[t] = *(const bf16x8 *)&Ws[rB*ZLD + k0 + (lane >> 5)*8];
+        for (int s = 0; s < 8; ++s) {
+            if (s + 1 < 8) {
+                az[(s + 1) & 1] = *(const bf16x8 *)&Zs[pw_at(l31, 2*(s + 1) + h, 128)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    bw[(s + 1) & 1][j] = *(const bf16x8 *)&W2s[pw_at(32*j + l31, 2*(s + 1) + h, 128)];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(az[s & 1], bw[s & 1][j], acc[j], 0, 0, 0);
         }
+        __builtin_amdgcn_wave_barrier();
+        // the residual input (columns 0..127 of the A operand) into the same LDS
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int s = 0; s < 8; ++s) *(uint4 *)&Zs[pw_at(l31, 2*s + h, 128)] = A[s];
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
-    // epilogue: acc (FP32) through LDS, then row-contiguous residual read and store
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                int r = wm*64 + i*32 + 8*(q >> 2) + 4*(lane >> 5) + (q & 3);
-                int c = wn*64 + j*32 + (lane & 31);
-                Os[r*128 + c] = acc[i][j][q];
+                const int r = 8*(q >> 2) + 4*h + (q & 3), c = 32*j + l31;
+                const int o = pw_at(r, c >> 3, 128) + (c & 7);
+                Zs[o] = f2bf(bf2f(Zs[o]) + acc[j][q] + bias2[j]);
             }
-    __syncthreads();
-    for (int e = tid; e < 128*16; e += 256) {
-        int r = e >> 4, c0 = (e & 15)*8;
-        if (m0 + r >= M) continue;
-        size_t o = (size_t)(m0 + r)*128 + c0;
-        uint4 ev = *(const uint4 *)(al.E + o);
-        const uint16_t *eh = reinterpret_cast<const uint16_t *>(&ev);
-        uint16_t out[8];
+        __builtin_amdgcn_wave_barrier();
+    };
+    // the tile's result rows from the wave's LDS to memory
+    auto store = [&](int tile) {
+        const int r = tile*32 + l31;
+        if (r < M) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) out[k] = f2bf(bf2f(eh[k]) + Os[r*128 + c0 + k] + b2[c0 + k]);
-        *(uint4 *)(Eout + o) = *reinterpret_cast<const uint4 *>(out);
+            for (int s = 0; s < 8; ++s)
+                *(uint4 *)(Eout + (size_t)r*128 + 16*s + 8*h) = *(const uint4 *)&Zs[pw_at(l31, 2*s + h, 128)];
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    uint4 A0[24], A1[24];
+    int tile = blockIdx.x*4 + w;
+    int ri = al.recv[row_of(tile)], si = al.send[row_of(tile)];
+    fetch(tile, A0, ri, si);
+    // two tiles per round, so that each buffer keeps fixed registers; the scheduling barriers
+    // keep the next tile's loads ahead of this tile's multiplications
+    while (tile < ntiles) {
+        fetch(tile + stride, A1, ri, si);
+        __builtin_amdgcn_sched_barrier(0);
+        tile_pass(tile, A0);
+        store(tile);
+        tile += stride;
+        if (tile >= ntiles) break;
+        fetch(tile + stride, A0, ri, si);
+        __builtin_amdgcn_sched_barrier(0);
+        tile_pass(tile, A1);
+        store(tile);
+        tile += stride;
     }
 }
 
@@ -747,7 +768,17 @@ void edge_mlp_fused(hipStream_t s, int M, const uint16_t *E, const uint16_t *H, 
                     const int32_t *send, const uint16_t *W1t, const float *b1, const uint16_t *W2t,
                     const float *b2, uint16_t *Eout)
 {
-    hipLaunchKernelGGL(k_edge_mlp, dim3(nb(M, BM)), dim3(256), 0, s, M, LdEdgeCat{E, H, recv, send}, W1t, b1,
+    // one workgroup per CU (each holds 160 KB of LDS), none idle on small graphs
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const int tiles = (M + 31)/32;
+    const int grid = std::max(1, std::min(ncu, (tiles + 3)/4));
+    hipLaunchKernelGGL(k_edge_mlp, dim3(grid), dim3(256), 0, s, M, LdEdgeCat{E, H, recv, send}, W1t, b1,
                        W2t, b2, Eout);
 }
 void gemm_fwd_resid(hipStream_t s, int M, const uint16_t *Z, const uint16_t *Wt, const float *b,
