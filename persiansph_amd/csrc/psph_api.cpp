@@ -1391,7 +1391,12 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     P.verlet_force = 0;
     P.wl_cap = 0;
     const double support = P.cellfac*P.h_uniform;
-    if (P.fast11 || P.mixed32) {
+    // the general kernels reuse their pair lists the same way in runs of one material with one
+    // h, without periodic or in/out-flow boundaries (PSPH_NO_GREUSE: per-step lists instead)
+    const bool gen_reuse = !P.fast11 && !P.mixed32 && P.nfree > 0 && !P.multi && !P.inout &&
+                           !P.any_periodic && P.uniform_h && !std::getenv("PSPH_NO_GLIST") &&
+                           !std::getenv("PSPH_NO_GREUSE");
+    if (P.fast11 || P.mixed32 || gen_reuse) {
         double skin = 0.03;
         if (const char *e = std::getenv("PSPH_SKIN")) skin = std::atof(e);
         double csmin = std::min(P.csize[0], P.csize[1]);
@@ -1435,12 +1440,21 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
         return fail(PSPH_ERR_UNSUPPORTED, "the FP32-mixed tier holds at most 2^27 particles");
     P.glist = 0;
     if (!P.fast11 && !P.mixed32 && P.nfree > 0 && (!std::getenv("PSPH_NO_GLIST") || P.multi)) {
-        // pair lists of the general kernels: radius Cellfac hmax (the largest pair support)
-        int cap0 = nl_initial_cap(h, P.cellfac*P.hmax);
+        // pair lists of the general kernels: radius Cellfac hmax (the largest pair support), or
+        // with list reuse the support plus the skin; the walls' skin lists hold four sub-list
+        // capacities each
+        const double radius = P.verlet ? std::sqrt(P.reach2) : P.cellfac*P.hmax;
+        int cap0 = nl_initial_cap(h, radius);
         if (const char *e = std::getenv("PSPH_NL_CAP")) cap0 = std::max(4, (std::atoi(e) + 3) & ~3);
         rc = nl_alloc(h, cap0);
         if (rc) return rc;
         P.glist = 1;
+        if (P.verlet) {
+            int wcap = wl_initial_cap(h, radius);
+            if (const char *e = std::getenv("PSPH_WL_CAP")) wcap = std::max(4, (std::atoi(e) + 3) & ~3);
+            rc = wl_alloc(h, wcap);
+            if (rc) return rc;
+        }
     }
     if (P.fast11 || P.mixed32) {
         double radius = P.verlet ? std::sqrt(P.reach2) : support;

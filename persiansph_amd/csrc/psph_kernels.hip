@@ -514,11 +514,52 @@ __device__ inline double pair_h(const ClassParams *cp, const Params &P, int ci, 
     return P.uniform_h ? P.h_uniform : (cp[ci].h + cp[cj].h)/2;
 }
 
+// Whether the cells of two keys are the same or neighbours: the reference pairs particles of
+// neighbouring cells only (MainNeighbourSearch, Domain.cpp:808-945).  Cells can be slightly
+// smaller than the support (the CellNo rounding of CellInitiate), so a pair closer than the
+// support may still be two cells apart and outside the reference's pair set.
+__device__ __noinline__ bool keys_adjacent(const Params &P, int a, int b)
+{
+    int ax, ay, az, bx, by, bz;
+    decode_key(P, a, ax, ay, az);
+    decode_key(P, b, bx, by, bz);
+    return abs(ax - bx) <= 1 && abs(ay - by) <= 1 && abs(az - bz) <= 1;
+}
+
+// A lane's list in slot groups of four (one 16-byte store per group): entry e of the lane at
+// out[(e/4)*gs] component e%4.  Entries past the capacity are counted, not stored.
+struct ListWriter {
+    int4 *out;
+    size_t gs;
+    int cap, cnt;
+    int w0, w1, w2, w3;
+    __device__ ListWriter(int4 *o, size_t g, int c) : out(o), gs(g), cap(c), cnt(0), w0(0), w1(0), w2(0), w3(0) {}
+    __device__ void push(int v)
+    {
+        int sl = cnt & 3;
+        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
+        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+        ++cnt;
+    }
+    __device__ void flush()
+    {
+        if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
+    }
+};
+
 // ------------------------------------------------------------------------------------------
 // wall pass: PrimaryComputeAcceleration (Domain.cpp:1020-1059, 1163-1206) for fixed targets
 // ------------------------------------------------------------------------------------------
 // SOLID = false compiles the fluid-only smoothing (no Material 2/3 particles in the run).
-template <bool SOLID>
+//
+// REACH = 2 (list reuse in single-material runs): a step that rebuilds walks the 5^DIM cells
+// around the wall's cell, records every free candidate of the wall's material within the
+// support radius plus the skin in the wall's skin list, and adds the pairs of the neighbouring
+// cells; the steps until the next build walk the list instead and keep the entries that are
+// pairs of the reference's set at the current positions.  Between builds only the walls with
+// skin neighbours at the build (Ctl::nwact of them) have pairs; the others keep what the build
+// step wrote.
+template <bool SOLID, int REACH>
 __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -526,8 +567,16 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     __shared__ ClassParams cp[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) cp[k] = D.cls[k];
     __syncthreads();
-    int t = xcd_block()*kBlock + threadIdx.x;
-    if (t >= P.nfixed) return;
+    const bool reuse = REACH == 2 && !D.ctl->rebuild;
+    int t;
+    if (reuse) {
+        const int g = blockIdx.x*kBlock + threadIdx.x;
+        if (g >= D.ctl->nwact) return;
+        t = D.wact[g];
+    } else {
+        t = xcd_block()*kBlock + threadIdx.x;
+        if (t >= P.nfixed) return;
+    }
     int i = D.fixed_list[t];
     uint32_t fi = D.flags_b[i];
     int ci = flag_cls(fi);
@@ -545,6 +594,98 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
     const double sup = P.cellfac*P.hmax;
     const double sup2m = sup*sup*(1.0 + 1e-9);
     const double ox = P.blpf[0] + kx*P.csize[0], oy = P.blpf[1] + ky*P.csize[1], oz = P.blpf[2] + kz*P.csize[2];
+    // a free neighbour j of the wall's material at xij = x_wall - x_j inside the support: the
+    // pair's terms with the reference's orientation (P1 = smaller key / larger index, xij is
+    // x_P1 - x_P2; the negation is exact).  The sign is formed without short-circuit branches:
+    // hipcc (ROCm 7.2, -O3) dropped the same-cell negation when it was written as
+    // `if (!(a || (b && c))) x = -x;`.
+    auto add = [&](int j, double4 pj, double x0, double x1, double x2, double rij, double h) {
+        ++npair;
+        int kj = D.key[j];
+        int oj = D.orig_b[j];
+        int wall_first = pair_first(P, key_i, kj, orig_i, oj);
+        double sgn = wall_first ? 1.0 : -1.0;
+        x0 *= sgn; x1 *= sgn; x2 *= sgn;
+        double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h);
+        SK += K;
+        if (mat < 3) {
+            double4 vj = D.vel_b[j];
+            Pw += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
+        }
+        if (SOLID && mat > 1) {
+            double sj[6];
+            ld6(D.sig_b, j, sj);
+            for (int q = 0; q < 6; ++q) sg[q] = sg[q] + sj[q]*K;
+        }
+        if (C.NoSlip) {
+            double4 vj = D.vel_b[j];
+            n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K;
+        }
+    };
+    if constexpr (REACH == 2) {
+        ListWriter out(reinterpret_cast<int4 *>(D.wl) + t, (size_t)P.nfixed, 4*P.wl_cap);
+        if (reuse) {
+            // between builds: the wall's skin list, reduced to the reference's pairs
+            const int c = D.wcnt[t], m = c < 4*P.wl_cap ? c : 4*P.wl_cap;
+            const int *lst = D.wl;
+            for (int e = 0; e < m; ++e) {
+                const int j = lst[((size_t)(e >> 2)*P.nfixed + t)*4 + (e & 3)];
+                const double4 pj = D.pos_b[j];
+                const double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                const double r2 = x0*x0 + x1*x1 + x2*x2;
+                const double h = pair_h(cp, P, ci, flag_cls(D.flags_b[j]));
+                const double rij = sqrt(r2);
+                if (!((rij/h) <= P.cellfac)) continue;
+                if (r2 > P.adj2 && !keys_adjacent(P, key_i, D.key[j])) continue;
+                add(j, pj, x0, x1, x2, rij, h);
+            }
+        } else {
+            // a build: the 5^DIM cells, culled against the reach, in cell order
+            const double cx = P.csize[0], cy = P.csize[1], cz = P.csize[2];
+            const double cull2 = P.reach2*(1.0 + 1e-9);
+            const int ncell = (P.dim == 3) ? 125 : 25;
+            for (int cc = 0; cc < ncell; ++cc) {
+                const int dxk = (cc % 5) - 2, dyk = ((cc / 5) % 5) - 2, dzk = (P.dim == 3) ? (cc / 25) - 2 : 0;
+                const int xx = kx + dxk, yy = ky + dyk, zz = kz + dzk;
+                if (xx < 0 || xx >= P.nc[0] || yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+                const int cell = xx + P.nc[0]*(yy + P.nc[1]*zz);
+                if (D.cfree[cell] == 0) continue;
+                double ex = (dxk < 0) ? pi.x - (ox + (dxk + 1)*cx) : (dxk > 0) ? (ox + dxk*cx) - pi.x : 0.0;
+                double ey = (dyk < 0) ? pi.y - (oy + (dyk + 1)*cy) : (dyk > 0) ? (oy + dyk*cy) - pi.y : 0.0;
+                double ez = (dzk < 0) ? pi.z - (oz + (dzk + 1)*cz) : (dzk > 0) ? (oz + dzk*cz) - pi.z : 0.0;
+                ex = ex > 0.0 ? ex : 0.0; ey = ey > 0.0 ? ey : 0.0; ez = ez > 0.0 ? ez : 0.0;
+                if (ex*ex + ey*ey + ez*ez > cull2) continue;
+                const bool adj = dxk >= -1 && dxk <= 1 && dyk >= -1 && dyk <= 1 && dzk >= -1 && dzk <= 1;
+                const int beg = D.cstart[cell], end = D.cstart[cell + 1];
+                for (int j = beg; j < end; ++j) {
+                    const uint32_t fj = D.flags_b[j];
+                    if (!flag_free(fj) || flag_mat(fj) != mat) continue;
+                    const double4 pj = D.pos_b[j];
+                    const double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                    const double r2 = x0*x0 + x1*x1 + x2*x2;
+                    if (r2 <= P.reach2) out.push(j);
+                    if (!adj) continue;
+                    const double h = pair_h(cp, P, ci, flag_cls(fj));
+                    const double rij = sqrt(r2);
+                    if ((rij/h) <= P.cellfac) add(j, pj, x0, x1, x2, rij, h);
+                }
+            }
+            out.flush();
+            D.wcnt[t] = out.cnt;
+            if (out.cnt > 4*P.wl_cap) atomicOr(&D.ctl->list_over, 2);
+            // record the wall as active if it has skin neighbours (one atomic per wave)
+            const bool rec = out.cnt != 0;
+            const unsigned long long msk = __ballot(rec);
+            const int lane = threadIdx.x & 63;
+            int base = 0;
+            if (msk) {
+                const int leader = __ffsll((long long)msk) - 1;
+                if (lane == leader) base = atomicAdd(&D.ctl->nwact, __popcll(msk));
+                base = __builtin_amdgcn_readlane(base, leader);
+            }
+            if (rec) D.wact[base + __popcll(msk & ((1ull << lane) - 1ull))] = t;
+        }
+    } else {
     const int ncell = (P.dim == 3) ? 27 : 9;
     for (int cc = 0; cc < ncell; ++cc) {
         int dxk = (cc % 3) - 1, dyk = ((cc / 3) % 3) - 1, dzk = (P.dim == 3) ? (cc / 9) - 1 : 0;
@@ -584,33 +725,10 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
             if (P.any_periodic) periodic_xij(D, P, h, key_i, j, x0, x1, x2);
             double rij = sqrt(x0*x0 + x1*x1 + x2*x2);
             if (!((rij/h) <= P.cellfac)) continue;      // outside the reference's pair set
-            ++npair;
-            // orientation of the reference's pair (P1 = smaller key / larger index); xij is
-            // x_P1 - x_P2.  Multiplying by -1 is an exact negation (rij unchanged).  The sign is
-            // formed without short-circuit branches: hipcc (ROCm 7.2, -O3) dropped the same-cell
-            // negation when this was written as `if (!(a || (b && c))) x = -x;`.
-            int kj = D.key[j];
-            int oj = D.orig_b[j];
-            int wall_first = pair_first(P, key_i, kj, orig_i, oj);
-            double sgn = wall_first ? 1.0 : -1.0;
-            x0 *= sgn; x1 *= sgn; x2 *= sgn;
-            double K = kernel_w<double>(P.dim, P.kernel_type, rij/h, h);
-            SK += K;
-            if (mat < 3) {
-                double4 vj = D.vel_b[j];
-                Pw += vj.w * K + (G[0]*x0 + G[1]*x1 + G[2]*x2)*pj.w*K;
-            }
-            if (SOLID && mat > 1) {
-                double sj[6];
-                ld6(D.sig_b, j, sj);
-                for (int q = 0; q < 6; ++q) sg[q] = sg[q] + sj[q]*K;
-            }
-            if (C.NoSlip) {
-                double4 vj = D.vel_b[j];
-                n0 += vj.x * K; n1 += vj.y * K; n2 += vj.z * K;
-            }
+            add(j, pj, x0, x1, x2, rij, h);
         }
         }
+    }
     }
     // StartAcceleration resets fixed P, NSv, Sigma (Domain.cpp:990-996); then divide by SumKernel
     if (SK != 0.0) {
@@ -648,17 +766,6 @@ __global__ void __launch_bounds__(kBlock) k_wall(Dev D0, Params P)
 // pair membership and cell codes of the fast fluid path
 // ------------------------------------------------------------------------------------------
 
-// Whether the cells of two keys are the same or neighbours: the reference pairs particles of
-// neighbouring cells only (MainNeighbourSearch, Domain.cpp:808-945).  Cells can be slightly
-// smaller than the support (the CellNo rounding of CellInitiate), so a pair closer than the
-// support may still be two cells apart and outside the reference's pair set.
-__device__ __noinline__ bool keys_adjacent(const Params &P, int a, int b)
-{
-    int ax, ay, az, bx, by, bz;
-    decode_key(P, a, ax, ay, az);
-    decode_key(P, b, bx, by, bz);
-    return abs(ax - bx) <= 1 && abs(ay - by) <= 1 && abs(az - bz) <= 1;
-}
 
 // Lists cover the W^DIM cells around the target's cell, W = 2 REACH + 1: REACH = 1 for lists
 // of one step (support radius), REACH = 2 for reused lists (support radius plus a skin of at
@@ -1158,27 +1265,6 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     if (shep_on) A.sd += mj* K;
 }
 
-// T = float (SOLID only): the FP32-mixed soil tier; gravity is added in FP64 afterwards.
-// A lane's list in slot groups of four (one 16-byte store per group): entry e of the lane at
-// out[(e/4)*gs] component e%4.  Entries past the capacity are counted, not stored.
-struct ListWriter {
-    int4 *out;
-    size_t gs;
-    int cap, cnt;
-    int w0, w1, w2, w3;
-    __device__ ListWriter(int4 *o, size_t g, int c) : out(o), gs(g), cap(c), cnt(0), w0(0), w1(0), w2(0), w3(0) {}
-    __device__ void push(int v)
-    {
-        int sl = cnt & 3;
-        w0 = (sl == 0) ? v : w0; w1 = (sl == 1) ? v : w1; w2 = (sl == 2) ? v : w2; w3 = (sl == 3) ? v : w3;
-        if (sl == 3 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-        ++cnt;
-    }
-    __device__ void flush()
-    {
-        if ((cnt & 3) != 0 && cnt < cap) out[(size_t)(cnt >> 2)*gs] = make_int4(w0, w1, w2, w3);
-    }
-};
 
 // Pair lists of the general kernels: each free target's pairs of this step, in the order the
 // force kernel's row walk meets them (rows of three cells, y then z; a row across a period in
@@ -1187,10 +1273,19 @@ struct ListWriter {
 // register-heavy loop.  Entries beyond the capacity are counted (ncnt), raise Ctl::nl_over (the
 // host grows the lists) and are found again by the force kernel.  A cross-material candidate
 // raises Ctl::xmat.
+//
+// REACH = 2 (list reuse in single-material runs): a step that rebuilds lists every candidate of
+// the target's material within the support radius plus the skin, over the 5^DIM cells around
+// the target's cell (rows of five cells, y then z); the steps until the next build keep the
+// lists, and the force kernel keeps the entries that are pairs of the reference's set at the
+// current positions.  A list that overflows makes every step rebuild (Ctl::list_over) until the
+// host has grown the lists.
+template <int REACH>
 __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
 {
     const Dev D = bind(D0);
     if (D.ctl->halt) return;
+    if (REACH == 2 && !D.ctl->rebuild) return;
     __shared__ double hcls[kMaxClasses];
     for (int k = threadIdx.x; k < P.nclasses; k += kBlock) hcls[k] = D.cls[k].h;
     __syncthreads();
@@ -1205,6 +1300,39 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
     decode_key(P, key_i, kx, ky, kz);
     ListWriter out(reinterpret_cast<int4 *>(D.nl) + t, (size_t)P.nfree, P.nl_cap);
     int xmat = 0, xc = 0;
+    if constexpr (REACH == 2) {
+        const int nr = (P.dim == 3) ? 25 : 5;
+        const int xa = kx > 1 ? kx - 2 : 0, xb = kx + 2 < P.nc[0] ? kx + 2 : P.nc[0] - 1;
+        for (int r = 0; r < nr; ++r) {
+            const int yy = ky + (r % 5) - 2, zz = (P.dim == 3) ? kz + (r / 5) - 2 : kz;
+            if (yy < 0 || yy >= P.nc[1] || zz < 0 || zz >= P.nc[2]) continue;
+            const int base = P.nc[0]*(yy + P.nc[1]*zz);
+            const int beg = D.cstart[base + xa], end = D.cstart[base + xb + 1];
+            for (int j0 = beg; j0 < end; j0 += 4) {
+                uint32_t fg[4];
+                double4 pg[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = (j0 + u < end) ? j0 + u : beg;
+                    fg[u] = D.flags_b[jj];
+                    pg[u] = D.pos_b[jj];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u;
+                    if (j >= end || j == i) continue;
+                    if (flag_mat(fg[u]) != mat) { xmat = 1; continue; }
+                    const double x0 = pi.x - pg[u].x, x1 = pi.y - pg[u].y, x2 = pi.z - pg[u].z;
+                    if (x0*x0 + x1*x1 + x2*x2 <= P.reach2) out.push(j);
+                }
+            }
+        }
+        out.flush();
+        D.ncnt[t] = out.cnt;
+        if (out.cnt > P.nl_cap) { atomicAdd(&D.ctl->nl_over, 1); atomicOr(&D.ctl->list_over, 1); }
+        if (wave_any(xmat != 0) && (threadIdx.x & 63) == 0) atomicOr(&D.ctl->xmat, 1);
+        return;
+    }
     const int nrows = (P.dim == 3) ? 9 : 3;
     for (int rs = 0; rs < 2*nrows; ++rs) {
         int beg = 0, end = 0;
@@ -1254,9 +1382,20 @@ __global__ void __launch_bounds__(kBlock) k_glist(Dev D0, Params P)
     if ((threadIdx.x & 63) == 0 && anyx) atomicOr(&D.ctl->xmat, 1);
 }
 
+// Whether list entry j of free target i is a pair of the reference's set at the current
+// positions, given their squared distance: inside the support (in_support) and in neighbouring
+// cells, which needs the current cell keys only beyond the smallest cell size.
+__device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, double r2)
+{
+    bool in = in_support(P, r2);
+    if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[j]);
+    return in;
+}
+
 // FLUID (SOLID kernels): runs with free fluid particles besides soil or solid ones; without
 // them the fluid pair function is not compiled in, which keeps the FP64 kernel within 256 VGPRs
-// (two waves per SIMD).
+// (two waves per SIMD).  T = float (SOLID only): the FP32-mixed soil tier; gravity is added in
+// FP64 afterwards.
 template <bool SOLID, class T, bool FLUID = true>
 __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
 {
@@ -1329,19 +1468,31 @@ __global__ void __launch_bounds__(kForceBlock) k_force(Dev D0, Params P)
     int xmat = 0;
     int cnt = 0;
     // listed pairs first (k_glist, same order as the walk below); the walk then runs only for
-    // a target whose pairs overflowed the list, and skips the `skip` pairs already taken
+    // a target whose pairs overflowed the list, and skips the `skip` pairs already taken.
+    // With list reuse the lists are skin lists: an entry counts if it is a pair of the
+    // reference's set at the current positions (skin_member); a skin list overflows only at a
+    // build, where the cells are current, and the target then takes the walk alone.
     int skip = 0;
     bool walk = act;
     if (P.glist && act) {
         const int total = D.ncnt[t], m = total < P.nl_cap ? total : P.nl_cap;
         const int *lst_g = D.nl;
-        for (int e = 0; e < m; ++e) {
-            int jj = lst_g[((size_t)(e >> 2)*P.nfree + t)*4 + (e & 3)];
-            if (flag_free(D.flags_b[jj])) ++nff; else ++nfw;
-            pair_same(jj);
+        if (P.verlet && total > P.nl_cap) {
+            skip = 0;
+        } else {
+            for (int e = 0; e < m; ++e) {
+                int jj = lst_g[((size_t)(e >> 2)*P.nfree + t)*4 + (e & 3)];
+                if (P.verlet) {
+                    const double4 pj = D.pos_b[jj];
+                    const double x0 = pi.x - pj.x, x1 = pi.y - pj.y, x2 = pi.z - pj.z;
+                    if (!skin_member(D, P, i, jj, x0*x0 + x1*x1 + x2*x2)) continue;
+                }
+                if (flag_free(D.flags_b[jj])) ++nff; else ++nfw;
+                pair_same(jj);
+            }
+            skip = m;
+            walk = total > P.nl_cap;
         }
-        skip = m;
-        walk = total > P.nl_cap;
     }
     int seen = 0;
     const int nrows = (P.dim == 3) ? 9 : 3;
@@ -1749,15 +1900,6 @@ __global__ void __launch_bounds__(kForceBlock) k_cross(Dev D0, Params P)
 // rij/h <= Cellfac test of the reference runs.
 // ------------------------------------------------------------------------------------------
 
-// Whether list entry j of free target i is a pair of the reference's set at the current
-// positions, given their squared distance: inside the support (in_support) and in neighbouring
-// cells, which needs the current cell keys only beyond the smallest cell size.
-__device__ inline bool skin_member(const Dev &D, const Params &P, int i, int j, double r2)
-{
-    bool in = in_support(P, r2);
-    if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[j]);
-    return in;
-}
 
 struct Acc11 { double a0, a1, a2, dd; };
 
@@ -3241,7 +3383,12 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sums(Dev D0, Params P)
     int kx, ky, kz;
     decode_key(P, D.key[w], kx, ky, kz);
     double a0 = P.gravity[0], a1 = P.gravity[1], a2 = P.gravity[2], dd = 0.0;
-    const int nrows = (P.dim == 3) ? 9 : 3;
+    // rows of three cells around the wall's cell; with list reuse the sort (and so the cell
+    // table) is that of the last build, so the rows span five cells and a candidate counts if
+    // its current cell neighbours the wall's (keys_adjacent beyond the smallest cell size)
+    const int R = P.verlet ? 2 : 1, W = 2*R + 1;
+    const int nrows = (P.dim == 3) ? W*W : W;
+    const int key_w = D.key[w];
     for (int rs = 0; rs < 2*nrows; ++rs) {
         int beg = 0, end = 0;
         if (P.any_periodic) {
@@ -3250,7 +3397,18 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sums(Dev D0, Params P)
             beg = (rs & 1) ? b1 : b0;
             end = (rs & 1) ? e1 : e0;
         } else if (!(rs & 1)) {
-            row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+            if (P.verlet) {
+                const int r = rs >> 1;
+                const int yy = ky + (r % W) - R, zz = (P.dim == 3) ? kz + (r / W) - R : kz;
+                if (yy >= 0 && yy < P.nc[1] && zz >= 0 && zz < P.nc[2]) {
+                    const int base = P.nc[0]*(yy + P.nc[1]*zz);
+                    const int xa = kx > 1 ? kx - 2 : 0, xb = kx + 2 < P.nc[0] ? kx + 2 : P.nc[0] - 1;
+                    beg = D.cstart[base + xa];
+                    end = D.cstart[base + xb + 1];
+                }
+            } else {
+                row_range(D, P, kx, ky, kz, rs >> 1, beg, end);
+            }
         }
         for (int j = beg; j < end; ++j) {
             const uint32_t fj = D.flags_b[j];
@@ -3259,7 +3417,9 @@ __global__ void __launch_bounds__(kBlock) k_fixed_sums(Dev D0, Params P)
             const double h = pair_h(cp, P, flag_cls(fj), cw);
             double x0 = pj.x - pw.x, x1 = pj.y - pw.y, x2 = pj.z - pw.z;
             if (P.any_periodic) periodic_xij(D, P, h, D.key[j], w, x0, x1, x2);
-            if (!support_test(P, x0*x0 + x1*x1 + x2*x2, h)) continue;
+            const double r2 = x0*x0 + x1*x1 + x2*x2;
+            if (!support_test(P, r2, h)) continue;
+            if (P.verlet && r2 > P.adj2 && !keys_adjacent(P, key_w, D.key[j])) continue;
             const double4 vj = D.vel_b[j], aj = D.aux[j];
             Acc A;
             A.a0 = A.a1 = A.a2 = A.dd = A.ddc = 0.0;
@@ -3448,10 +3608,14 @@ static void launch_wall_pass(const Dev &D, const Params &P, hipStream_t s)
         }
         return;
     }
-    if (P.any_solid)
-        hipLaunchKernelGGL(k_wall<true>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
-    else
-        hipLaunchKernelGGL(k_wall<false>, dim3(nblk(P.nfixed, kBlock)), dim3(kBlock), 0, s, D, P);
+    const dim3 g(nblk(P.nfixed, kBlock)), b(kBlock);
+    if (P.any_solid) {
+        if (P.verlet) hipLaunchKernelGGL((k_wall<true, 2>), g, b, 0, s, D, P);
+        else hipLaunchKernelGGL((k_wall<true, 1>), g, b, 0, s, D, P);
+    } else {
+        if (P.verlet) hipLaunchKernelGGL((k_wall<false, 2>), g, b, 0, s, D, P);
+        else hipLaunchKernelGGL((k_wall<false, 1>), g, b, 0, s, D, P);
+    }
 }
 
 static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s);
@@ -3504,8 +3668,12 @@ static void launch_list_pass(const Dev &D, const Params &P, hipStream_t s)
 
 static void launch_force_pass(const Dev &D, const Params &P, hipStream_t s)
 {
-    if (P.glist)
-        hipLaunchKernelGGL(k_glist, dim3(nblk(P.nfree, kBlock)), dim3(kBlock), 0, s, D, P);
+    if (P.glist) {
+        // with list reuse the lists are built only on steps that rebuild (a lean step does not
+        // queue the build)
+        if (!P.verlet) hipLaunchKernelGGL(k_glist<1>, dim3(nblk(P.nfree, kBlock)), dim3(kBlock), 0, s, D, P);
+        else if (!P.lean) hipLaunchKernelGGL(k_glist<2>, dim3(nblk(P.nfree, kBlock)), dim3(kBlock), 0, s, D, P);
+    }
     if (P.nl_cap > 0 && !P.glist) {
         if (P.dim == 3) {
             if (P.verlet) launch_list_pass<3, 2>(D, P, s);

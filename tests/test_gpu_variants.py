@@ -100,11 +100,13 @@ def test_variant_force_pass_and_trajectory(name):
 
 @pytest.mark.parametrize("name", ["takeda", "artificial_stress"])
 def test_general_pair_lists_are_exact(name, monkeypatch):
-    """The general kernels take each target's pairs from a list built in the walk order; the
-    walk itself serves only entries beyond the capacity.  Lists of capacity 8 (most targets
-    overflow, the host then grows them) and no lists at all give bitwise the same run."""
+    """Without list reuse the general kernels take each target's pairs from a list built every
+    step in the walk order; the walk itself serves only entries beyond the capacity.  Lists of
+    capacity 8 (most targets overflow, the host then grows them) and no lists at all give
+    bitwise the same run."""
     cfg, p, dt = _variant(**VARIANTS[name])
     runs = {}
+    monkeypatch.setenv("PSPH_NO_GREUSE", "1")
     for mode in ("lists", "cap8", "nolists"):
         monkeypatch.delenv("PSPH_NL_CAP", raising=False)
         monkeypatch.delenv("PSPH_NO_GLIST", raising=False)
@@ -126,3 +128,52 @@ def test_general_pair_lists_are_exact(name, monkeypatch):
         assert got[0] == ref[0] and got[4] == ref[4], mode
         for k in (1, 2, 3):
             np.testing.assert_array_equal(got[k], ref[k], err_msg=mode)
+
+
+_REUSE_MODES = {
+    "reuse": {},
+    "reuse_cap8": {"PSPH_NL_CAP": "8", "PSPH_WL_CAP": "4"},
+    "rebuild_every_step": {"PSPH_REBUILD_EVERY_STEP": "1"},
+}
+
+
+@pytest.mark.parametrize("name", ["quintic", "verlet_xsph_shepard", "artificial_stress"])
+def test_general_list_reuse_keeps_the_pair_set(name, monkeypatch):
+    """With list reuse the general kernels keep skin lists (candidates within the support plus
+    a skin) between builds, for free and wall targets, and keep the entries that are pairs of
+    the reference's set at the current positions.  Against per-step lists: the same pair counts
+    at every step, the first force pass within 1e-12 of the largest acceleration (the sums run
+    in another order), and 40-step trajectories within 1e-11 m and 1e-11 relative density.
+    Lists of capacity 8 (overflow at builds: the host grows them) and a rebuild at every step
+    give the same."""
+    cfg, p, dt = _variant(**VARIANTS[name])
+    env = ("PSPH_NO_GREUSE", "PSPH_NL_CAP", "PSPH_WL_CAP", "PSPH_REBUILD_EVERY_STEP")
+
+    def run(mode):
+        for k in env:
+            monkeypatch.delenv(k, raising=False)
+        if mode == "lists":
+            monkeypatch.setenv("PSPH_NO_GREUSE", "1")
+        for k, v in _REUSE_MODES.get(mode, {}).items():
+            monkeypatch.setenv(k, v)
+        print(f"[reuse] {name} {mode}", flush=True)
+        g = Solver(cfg, p, dt)
+        st1 = g.run(1, tout=0.0)
+        a1 = g.particles()["a"].copy()
+        pairs = []
+        for _ in range(40):
+            pairs.append(g.run(1).pairs)
+        q = g.particles()
+        out = (st1.pairs, a1, q["x"].copy(), q["Density"].copy(), pairs)
+        g.close()
+        return out
+
+    ref = run("lists")
+    fr = p["IsFree"] == 1
+    amax = np.abs(ref[1][fr]).max()
+    for mode in _REUSE_MODES:
+        got = run(mode)
+        assert got[0] == ref[0] and got[4] == ref[4], mode
+        assert np.abs(got[1][fr] - ref[1][fr]).max() <= 1e-12*amax, mode
+        np.testing.assert_allclose(got[2], ref[2], rtol=0, atol=1e-11, err_msg=mode)
+        np.testing.assert_allclose(got[3], ref[3], rtol=1e-11, atol=0, err_msg=mode)
