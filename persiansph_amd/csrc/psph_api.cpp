@@ -503,6 +503,7 @@ static int allocate(psph_handle *h, size_t n)
     AL(a.shep, sizeof(double2)*N);
     if (P.any_solid || P.any_sr) AL(a.sr, sizeof(double)*6*N);
     if (P.any_solid) { AL(a.rr, sizeof(double)*3*N); AL(a.tir, sizeof(double)*6*N); }
+    if (P.srec) AL(a.srec, sizeof(double2)*10*N);
     if (P.any_les_t0) AL(a.shr, sizeof(double)*2*N);
     AL(a.key, sizeof(int32_t)*N);
     AL(a.rank, sizeof(int32_t)*N);
@@ -1415,6 +1416,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     // made when needed, grow_buffers); PSPH_INOUT_SLACK sets the extra room (test switch)
     size_t slack = n/2 + 256;
     if (const char *e = std::getenv("PSPH_INOUT_SLACK")) slack = (size_t)std::max(0, std::atoi(e));
+    // pair records of free soil particles (single-material soil runs)
+    P.srec = 0;
+    if (P.any_solid && !P.multi && !std::getenv("PSPH_NO_SREC")) {
+        P.srec = 1;
+        for (const ClassParams &c : h->classes)
+            if (c.IsFree && c.Material != 3) P.srec = 0;
+    }
     rc = allocate(h, P.inout ? n + slack : n);
     if (rc) return rc;
     h->next_index = (int32_t)n;

@@ -171,6 +171,8 @@ struct Params {
     // work on rebuilding steps (sort, list build, tile union), and the tail halts (kHaltSort)
     // when the step after it must rebuild
     int lean;
+    // runs of one material whose free particles are all soil: the pair records of Dev::srec
+    int srec;
 };
 
 // One set of the persistent columns.
@@ -233,6 +235,9 @@ struct Dev {
     double *sr;        // StrainRate (6)
     double *rr;        // RotationRate (3: xy xz yz)
     double *tir;       // TIR (6)
+    // free soil particles' pair records (Params::srec), 10 double2 each: x y | z d | vx vy |
+    // vz m | Sigma (6) | TIR (6), one contiguous 160-B read per neighbour of CalcForce2233
+    double2 *srec;
     double *shr;       // ShearRate, SBar (2)
     int32_t *key;      // cell key of each particle in the current order
     int32_t *rank;     // arrival rank within its cell (counting sort)

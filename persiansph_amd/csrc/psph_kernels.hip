@@ -416,6 +416,14 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
                     if (c.Material > 1 && c.TI > 0.0) tir_tensor(P.dim, sg, c.TI, p.w, t);
                     double2 *t2 = reinterpret_cast<double2 *>(D.tir + 6*(size_t)i);
                     for (int q = 0; q < 3; ++q) t2[q] = make_double2(t[2*q], t[2*q+1]);
+                    if (P.srec) {
+                        // the particle's CalcForce2233 record as a neighbour (see Dev::srec)
+                        double2 *r = D.srec + 10*(size_t)i;
+                        r[0] = make_double2(p.x, p.y); r[1] = make_double2(p.z, p.w);
+                        r[2] = make_double2(v.x, v.y); r[3] = make_double2(v.z, mm.x);
+                        for (int q = 0; q < 3; ++q) r[4 + q] = make_double2(sg[2*q], sg[2*q+1]);
+                        for (int q = 0; q < 3; ++q) r[7 + q] = make_double2(t[2*q], t[2*q+1]);
+                    }
                 }
             }
         }
@@ -1146,13 +1154,30 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     // divisions by refined reciprocals and fused products (a few ulp per term, inside the bars)
 #pragma clang fp contract(fast)
     uint32_t fj = D.flags_b[j];
+    // a free neighbour's state from its pair record (runs of free soil particles, Dev::srec):
+    // its first 64 B are read with the flags, the tensors where they are used (the same two
+    // cache lines); a fixed neighbour's state from the columns
+    const double2 *rp = P.srec ? D.srec + 10*(size_t)j : nullptr;
+    double2 rec[4];
+    if (P.srec) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rec[q] = rp[q];
+    }
     int ci = flag_cls(fi), cj = flag_cls(fj);
     const ClassParams &Ci_ = cp[ci];
     const ClassParams &Cj_ = cp[cj];
-    double4 pj = D.pos_b[j];
-    double4 vj = D.vel_b[j];
-    double4 aj = D.aux[j];
     bool freej = flag_free(fj);
+    const bool use_rec = P.srec && freej;
+    double4 pj, vj, aj;
+    if (use_rec) {
+        pj = make_double4(rec[0].x, rec[0].y, rec[1].x, rec[1].y);
+        vj = make_double4(rec[2].x, rec[2].y, rec[3].x, 0.0);
+        aj = make_double4(rec[3].y, 0.0, 0.0, 0.0);
+    } else {
+        pj = D.pos_b[j];
+        vj = D.vel_b[j];
+        aj = D.aux[j];
+    }
     T h = (T)pair_h(cp, P, ci, cj);
     double dx0 = pi.x - pj.x, dx1 = pi.y - pj.y, dx2 = pi.z - pj.z;
     if (P.any_periodic) periodic_xij(D, P, pair_h(cp, P, ci, cj), D.key[i], j, dx0, dx1, dx2);
@@ -1194,7 +1219,8 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
                                               (T)((Ci_.TIInitDist + Cj_.TIInitDist)/(2.0*(double)h)), h));
         T sT = exp((T)((Ci_.TIn+Cj_.TIn)/2.0)*log(K*rwdp));
         double trj[6];
-        ld6(D.tir, j, trj);
+        if (use_rec) { for (int q = 0; q < 3; ++q) { const double2 t = rp[7 + q]; trj[2*q] = t.x; trj[2*q+1] = t.y; } }
+        else ld6(D.tir, j, trj);
         for (int k = 0; k < 6; ++k) TI6[k] = (T)(tri[k]+trj[k])*sT;
     }
     T b0, b1, b2;
@@ -1217,7 +1243,8 @@ __device__ __forceinline__ void pair2233(const Dev &D, const Params &P, const Cl
     T Mf[9];
     T sgiT[6], sgj[6];
     double sgjd[6];
-    ld6(D.sig_b, j, sgjd);
+    if (use_rec) { for (int q = 0; q < 3; ++q) { const double2 t = rp[4 + q]; sgjd[2*q] = t.x; sgjd[2*q+1] = t.y; } }
+    else ld6(D.sig_b, j, sgjd);
     for (int k = 0; k < 6; ++k) { sgiT[k] = (T)sgi[k]; sgj[k] = (T)sgjd[k]; }
     T Mi[9], Mj[9], T9[9];
     sym_to_full(sgiT, Mi);
