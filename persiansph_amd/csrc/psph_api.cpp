@@ -130,6 +130,7 @@ struct psph_handle {
     bool trans_ok = false;             // transient columns match the current order
     bool fixed_ok = false;             // fixed particles' pair sums formed for this stop
     bool no_lean = false;              // PSPH_NO_LEAN: queue every step in full (test switch)
+    bool no_fused_views = false;       // PSPH_NO_FUSED_VIEWS: every step's views by k_reorder (test switch)
     double body[3] = {0.0, 0.0, 0.0};  // body-force hook (psph_set_body_force)
     bool hook_b = false, hook_a = false;   // host hook stops (psph_set_hooks)
     int regrowths = 0;                 // device buffers re-created for more particles (in/out-flow)
@@ -196,6 +197,8 @@ static int dev_free(psph_handle *h, void **p)
 static int nl_alloc(psph_handle *h, int cap)
 {
     Params &P = h->P;
+    // the gathers tally a lane's pairs in 16-bit fields (pair_tally)
+    if (cap > 65535) return fail(PSPH_ERR_INVALID, "neighbour lists of %d entries per particle (at most 65535)", cap);
     int rc = dev_free(h, (void **)&h->nl);
     if (rc) return rc;
     size_t nf = P.nfree > 0 ? (size_t)P.nfree : 1;
@@ -792,12 +795,15 @@ static int read_state(psph_handle *h)
 // Queues one step from stage `from` (0: whole step; 1: from the wall pass; 2: Move and tail).
 // lean: list reuse, and the step is known not to rebuild, so the sort and list-build kernels
 // (no-ops on such steps) are not queued.  tail_lean: the step's tail halts (kHaltSort) when the
-// step after it must rebuild, because that step will be queued lean.
-static void enqueue_step(psph_handle *h, int from, bool lean, bool tail_lean)
+// step after it must rebuild, because that step will be queued lean.  fresh: the step before
+// it (in the same queue) wrote this lean step's views, so its k_reorder is not queued.
+// views_next: this step's k_integrate writes the views of the step queued after it.
+static void enqueue_step(psph_handle *h, int from, bool lean, bool tail_lean, bool fresh, bool views_next)
 {
     Params P = h->P;
     const Dev &D = h->D;
     P.lean = lean ? 1 : 0;
+    P.views_next = views_next ? 1 : 0;
     if (from == 0) {
         if (!lean) {
             timed(h, KScan, [&] {
@@ -807,7 +813,7 @@ static void enqueue_step(psph_handle *h, int from, bool lean, bool tail_lean)
             timed(h, KScatter, [&] { launch_scatter(D, P, h->stream); });
             timed(h, KCellsort, [&] { launch_cellsort(D, P, h->stream); });
         }
-        timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
+        if (!(lean && fresh)) timed(h, KReorder, [&] { launch_reorder(D, P, h->stream); });
     }
     if (from <= 1) {
         timed(h, KWall, [&] { launch_wall(D, P, h->stream, from); });
@@ -1369,6 +1375,13 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
                 if (first) { P.ab_fixed[0] = P.ab_free[0]; P.ab_fixed[1] = P.ab_free[1]; }
                 P.ab_one = P.ab_two && P.ab_fixed[0] == P.ab_free[0] && P.ab_fixed[1] == P.ab_free[1];
             }
+            const double *ab[2] = {P.ab_free, P.ab_fixed};
+            for (int k = 0; k < 2; ++k) {
+                P.ab_nha[k] = -ab[k][0]*0.5;
+                P.ab32[2*k] = -(float)ab[k][0]*0.5f;
+                P.ab32[2*k + 1] = (float)ab[k][1];
+                P.ab_on[k] = (ab[k][0] != 0.0 || ab[k][1] != 0.0) ? 1 : 0;
+            }
         }
     }
     P.soil32 = 0;
@@ -1437,6 +1450,7 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
     P.tiles = (P.fast11 && P.verlet && (!P.mixed32 || cells10) && !std::getenv("PSPH_NO_TILES")) ? 1 : 0;
     P.lean = 0;
     h->no_lean = std::getenv("PSPH_NO_LEAN") != nullptr;
+    h->no_fused_views = std::getenv("PSPH_NO_FUSED_VIEWS") != nullptr;
     P.tile_items = kTileItems;
     if (const char *e = std::getenv("PSPH_TILE_ITEMS")) P.tile_items = std::max(0, std::min(kTileItems, std::atoi(e)));
     h->D.tile = nullptr; h->D.items = nullptr; h->D.tl = nullptr;
@@ -1529,13 +1543,16 @@ int psph_run(psph_handle *h, int64_t max_steps, double tf, double tout, psph_sta
     // list reuse: steps after the first of a batch are queued lean (their tails halt before a
     // step that must rebuild); the first is lean when the control block says it keeps the lists
     const bool lean_ok = h->P.verlet && !h->no_lean;
+    // the fast fluid passes' views written by the step before (a lean step keeps the order)
+    const bool fuse_views = lean_ok && (h->P.fast11 || h->P.mixed32) && !h->no_fused_views;
     for (;;) {
         // queue a batch; every kernel is a no-op once the device has halted
         long long nq = std::min<long long>(batch, max_steps - queued);
         if (pending_finish && nq < 1) nq = 1;
         for (long long k = 0; k < nq; ++k) {
             const bool lean = lean_ok && (k > 0 || !C.rebuild);
-            enqueue_step(h, pending_finish ? pending_from : 0, lean, lean_ok);
+            enqueue_step(h, pending_finish ? pending_from : 0, lean, lean_ok, fuse_views && k > 0,
+                         fuse_views && k + 1 < nq);
             pending_finish = false;
             ++queued;
         }

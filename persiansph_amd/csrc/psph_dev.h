@@ -130,6 +130,11 @@ struct Params {
     // ab_one: the two pairs are equal
     int ab_two, ab_one;
     double ab_free[2], ab_fixed[2];
+    // the same pairs as the gathers use them: -alpha/2 (free-free, free-fixed), FP32 copies
+    // (-alpha/2, beta of free-free, then of free-fixed), and whether alpha or beta is nonzero
+    double ab_nha[2];
+    float ab32[4];
+    int ab_on[2];
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;
     // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4), and
@@ -171,6 +176,9 @@ struct Params {
     // work on rebuilding steps (sort, list build, tile union), and the tail halts (kHaltSort)
     // when the step after it must rebuild
     int lean;
+    // set per launch by the host: the step after this one is queued lean, and this step's
+    // k_integrate writes that step's free-particle views (k_reorder is then not queued for it)
+    int views_next;
     // runs of one material whose free particles are all soil: the pair records of Dev::srec
     int srec;
 };

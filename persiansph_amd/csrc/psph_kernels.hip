@@ -345,6 +345,29 @@ __device__ __forceinline__ void copy_tensor6(const double *src, double *dst, int
     b[0] = t0; b[1] = t1; b[2] = t2;
 }
 
+// The views of free particle i (state p, v, mm; class c) that the pair passes read: aux (mass,
+// mu, sound speed and P/rho^2: CalcForce11 :67-72, :139), and the fast fluid path's records,
+// FP64 (g0..g2) or FP32-mixed (f0..f2, positions relative to the origin of the cell the
+// particle had at the last sort).
+__device__ inline void free_views(const Dev &D, const Params &P, const ClassParams &c, int i, uint32_t f,
+                                  double4 p, double4 v, double2 mm)
+{
+    double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
+    double pr = v.w/(p.w*p.w);
+    D.aux[i] = make_double4(mm.x, mm.y, C, pr);
+    if (P.mixed32) {
+        double o0, o1, o2;
+        cell_origin(P, D.key_b[i], o0, o1, o2);
+        D.f0[i] = make_float4((float)(p.x - o0), (float)(p.y - o1), (float)(p.z - o2), (float)p.w);
+        D.f1[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)pr);
+        D.f2[i] = make_float4((float)C, (float)flag_cls(f), (float)mm.x, (float)(1.0/p.w));
+    } else if (P.fast11) {
+        D.g0[i] = p;
+        D.g1[i] = make_double4(v.x, v.y, v.z, pr);
+        D.g2[i] = make_double4(C, (double)flag_cls(f), mm.x, 1.0/p.w);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
 {
     const Dev D = bind(D0);
@@ -378,21 +401,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(Dev D0, Params P)
             // key_b[i] stays the cell of slot i at the last sort, the origin of the FP32 positions
             const ClassParams &c = cp[flag_cls(f)];
             if (flag_free(f)) {
-                // sound speed and P/rho^2 of a free particle (CalcForce11 :67-72, :139)
-                double C = sound_speed(c.PresEq, c.Cs, p.w, c.RefDensity);
-                double pr = v.w/(p.w*p.w);
-                D.aux[i] = make_double4(mm.x, mm.y, C, pr);
-                if (P.mixed32) {
-                    double o0, o1, o2;
-                    cell_origin(P, D.key_b[i], o0, o1, o2);
-                    D.f0[i] = make_float4((float)(p.x - o0), (float)(p.y - o1), (float)(p.z - o2), (float)p.w);
-                    D.f1[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)pr);
-                    D.f2[i] = make_float4((float)C, (float)flag_cls(f), (float)mm.x, (float)(1.0/p.w));
-                } else if (P.fast11) {
-                    D.g0[i] = p;
-                    D.g1[i] = make_double4(v.x, v.y, v.z, pr);
-                    D.g2[i] = make_double4(C, (double)flag_cls(f), mm.x, 1.0/p.w);
-                }
+                free_views(D, P, c, i, f, p, v, mm);
             } else {
                 D.aux[i] = make_double4(mm.x, mm.y, 0.0, 0.0);
             }
@@ -1930,56 +1939,64 @@ __global__ void __launch_bounds__(kForceBlock) k_cross(Dev D0, Params P)
 
 struct Acc11 { double a0, a1, a2, dd; };
 
-// Artificial-viscosity constants (alpha, beta) of a pair of a target of class ci with a
-// neighbour (meta: its class, or -1 - class for a fixed particle), and whether the pair has an
-// artificial viscosity term at all (on: alpha or beta nonzero).  ABTable reads them from the
-// class-pair table in LDS; ABTwo chooses between two register pairs by the neighbour's kind
-// (every free particle of one class, the same constants with every fixed class:
-// Params::ab_two); ABOne holds the one pair of constants of such a run whose free-free and
-// free-fixed constants are equal (Params::ab_one).
+// Artificial-viscosity constants of a pair of a target of class ci with a neighbour (meta: its
+// class, or -1 - class for a fixed particle): nha = -alpha/2 and beta, and whether the pair has
+// an artificial viscosity term at all (on: alpha or beta nonzero).  ABTable reads them from the
+// class-pair table in LDS; ABTwo chooses between the free-free and the free-fixed pair by the
+// neighbour's kind (every free particle of one class, the same constants with every fixed
+// class: Params::ab_two); ABOne holds the one pair of a run whose free-free and free-fixed
+// constants are equal (Params::ab_one).  ABTwo and ABOne read the kernel arguments, which the
+// host fills per precision (Params::ab_nha, ab32, ab_on), so that they stay scalar operands.
+template <class T> __device__ inline T ab_nha(const Params &P, int k);
+template <> __device__ inline double ab_nha<double>(const Params &P, int k) { return P.ab_nha[k]; }
+template <> __device__ inline float ab_nha<float>(const Params &P, int k) { return P.ab32[2*k]; }
+template <class T> __device__ inline T ab_beta(const Params &P, int k);
+template <> __device__ inline double ab_beta<double>(const Params &P, int k) { return k ? P.ab_fixed[1] : P.ab_free[1]; }
+template <> __device__ inline float ab_beta<float>(const Params &P, int k) { return P.ab32[2*k + 1]; }
+
 template <class T>
 struct ABTable {
     const T *pa, *pb;
-    __device__ void get(int ci, T meta, bool freej, T &alpha, T &beta) const {
+    __device__ void get(int ci, T meta, bool freej, T &nha, T &beta) const {
         const int cj = freej ? (int)meta : (int)((T)-1 - meta);
-        alpha = pa[ci*8 + cj];
+        nha = -pa[ci*8 + cj]*(T)0.5;
         beta = pb[ci*8 + cj];
     }
-    __device__ bool on(T alpha, T beta, bool) const { return alpha != (T)0 || beta != (T)0; }
+    __device__ bool on(T nha, T beta, bool) const { return nha != (T)0 || beta != (T)0; }
 };
 template <class T>
 struct ABTwo {
-    T af, bf, aw, bw;
-    bool onf, onw;
-    __device__ void get(int, T, bool freej, T &alpha, T &beta) const {
-        alpha = freej ? af : aw;
-        beta = freej ? bf : bw;
+    const Params &P;
+    __device__ void get(int, T, bool freej, T &nha, T &beta) const {
+        nha = freej ? ab_nha<T>(P, 0) : ab_nha<T>(P, 1);
+        beta = freej ? ab_beta<T>(P, 0) : ab_beta<T>(P, 1);
     }
-    __device__ bool on(T, T, bool freej) const { return freej ? onf : onw; }
+    __device__ bool on(T, T, bool freej) const { return freej ? P.ab_on[0] != 0 : P.ab_on[1] != 0; }
 };
 template <class T>
 struct ABOne {
-    T a, b;
-    bool any;
-    __device__ void get(int, T, bool, T &alpha, T &beta) const { alpha = a; beta = b; }
-    __device__ bool on(T, T, bool) const { return any; }
+    const Params &P;
+    __device__ void get(int, T, bool, T &nha, T &beta) const { nha = ab_nha<T>(P, 0); beta = ab_beta<T>(P, 0); }
+    __device__ bool on(T, T, bool) const { return P.ab_on[0] != 0; }
 };
 // ABM: 0 the class-pair table, 1 two pairs by the neighbour's kind, 2 one pair
 template <int ABM, class T> struct ABPick;
 template <class T> struct ABPick<2, T> {
-    static __device__ ABOne<T> make(const Params &P, const T *, const T *) {
-        return {(T)P.ab_free[0], (T)P.ab_free[1], P.ab_free[0] != 0.0 || P.ab_free[1] != 0.0};
-    }
+    static __device__ ABOne<T> make(const Params &P, const T *, const T *) { return {P}; }
 };
 template <class T> struct ABPick<1, T> {
-    static __device__ ABTwo<T> make(const Params &P, const T *, const T *) {
-        return {(T)P.ab_free[0], (T)P.ab_free[1], (T)P.ab_fixed[0], (T)P.ab_fixed[1],
-                P.ab_free[0] != 0.0 || P.ab_free[1] != 0.0, P.ab_fixed[0] != 0.0 || P.ab_fixed[1] != 0.0};
-    }
+    static __device__ ABTwo<T> make(const Params &P, const T *, const T *) { return {P}; }
 };
 template <class T> struct ABPick<0, T> {
     static __device__ ABTable<T> make(const Params &, const T *pa, const T *pb) { return {pa, pb}; }
 };
+
+// A lane's pair tally: free neighbours in the low 16 bits, fixed ones in the high 16 bits, one
+// add per pair (a lane evaluates fewer than 65,536 pairs: the lists hold at most 65,535
+// entries per target).
+__device__ inline unsigned pair_tally(bool freej) { return freej ? 1u : 0x10000u; }
+__device__ inline unsigned tally_free(unsigned npc) { return npc & 0xffffu; }
+__device__ inline unsigned tally_fixed(unsigned npc) { return npc >> 16; }
 
 // SKIN: j comes from a skin list and counts only if it is a pair of the reference's set at the
 // current positions (skin_member); i is the target (for its cell key).  gi0..gi2 are the
@@ -1989,9 +2006,9 @@ template <class T> struct ABPick<0, T> {
 // in the current order; it is called only for the rare key-adjacency part of the skin test.
 template <int DIM, bool SKIN, class AB, class JF>
 __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
-                                  double4 gi0, double4 gi1, double4 gi2, int ci, int i, JF jf,
+                                  double4 gi0, double4 gi1, double4 gi2, int ci, int ki, JF jf,
                                   double4 a0, double4 a1, double4 a2, double4 vw,
-                                  double vsf, double vsw, Acc11 &A, unsigned &nff, unsigned &nfw)
+                                  double vsf, double vsw, Acc11 &A, unsigned &npc)
 {
     const double cj_snd = a2.x, meta = a2.y, mrec = a2.z, invdj = a2.w;
     double x0 = gi0.x - a0.x, x1 = gi0.y - a0.y, x2 = gi0.z - a0.z;
@@ -1999,7 +2016,7 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
     double r2 = x0*x0 + x1*x1 + x2*x2;
     if (SKIN) {
         bool in = in_support(P, r2);
-        if (in && r2 > P.adj2) in = keys_adjacent(P, D.key[i], D.key[jf()]);
+        if (in && r2 > P.adj2) in = keys_adjacent(P, ki, D.key[jf()]);
         if (!in) return;
     }
     // the contribution: products and sums may fuse (each fused pair rounds once instead of
@@ -2022,19 +2039,18 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
     }
     (void)K;
     const bool freej = meta >= 0.0;
-    nff += freej ? 1u : 0u;
-    nfw += freej ? 0u : 1u;
+    npc += pair_tally(freej);
     // viscous velocity of the neighbour (the caller passes a free neighbour's v)
     const double w0 = vw.x, w1 = vw.y, w2 = vw.z;
-    double alpha, beta;
-    ab.get(ci, meta, freej, alpha, beta);
+    double nha, beta;
+    ab.get(ci, meta, freej, nha, beta);
     double u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     double dvx = u0*x0 + u1*x1 + u2*x2;
     double di = gi0.w, dj = a0.w;
     double PIij = 0.0;
-    if (ab.on(alpha, beta, freej) && dvx < 0.0) {
+    if (ab.on(nha, beta, freej) && dvx < 0.0) {
         double MU = P.h_uniform*dvx*rcp_nr(r2 + P.h2c);
-        PIij = (-alpha*0.5*(gi2.x + cj_snd)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
+        PIij = (nha*(gi2.x + cj_snd)*MU + beta*MU*MU)*rcp_nr(0.5*(di + dj));
     }
     double mi = gi2.z;
     double mj = freej ? mrec : mrec*mi;
@@ -2055,14 +2071,14 @@ __device__ inline void pair11_rec(const Dev &D, const Params &P, const AB &ab,
 // target's views, ci its class.  A wall neighbour's viscous velocity is read only for walls.
 template <int DIM, bool SKIN>
 __device__ inline void pair11_fast(const Dev &D, const Params &P, const double *pa, const double *pb,
-                                   double4 gi0, double4 gi1, double4 gi2, int ci, int i, int j,
-                                   double vsf, double vsw, Acc11 &A, unsigned &nff, unsigned &nfw)
+                                   double4 gi0, double4 gi1, double4 gi2, int ci, int ki, int j,
+                                   double vsf, double vsw, Acc11 &A, unsigned &npc)
 {
     double4 a0 = D.g0[j], a1 = D.g1[j], a2 = D.g2[j];
     double4 vw = a1;
     if (a2.y < 0.0) vw = D.gw[j];
-    pair11_rec<DIM, SKIN>(D, P, ABTable<double>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
-                          vsf, vsw, A, nff, nfw);
+    pair11_rec<DIM, SKIN>(D, P, ABTable<double>{pa, pb}, gi0, gi1, gi2, ci, ki, [j] { return j; }, a0, a1, a2, vw,
+                          vsf, vsw, A, npc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2187,7 +2203,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
         const double4 a_init = act ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
         A.a0 = a_init.x; A.a1 = a_init.y; A.a2 = a_init.z; A.dd = a_init.w;
     }
-    unsigned nff = 0, nfw = 0;
+    unsigned npc = 0;
+    const int ki = D.key[i];
     const int cap = P.nl_cap;
     const bool codes = P.nl_codes != 0;
     int cnt = act ? D.ncnt[t] : 0;
@@ -2199,8 +2216,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
         int sl = e & 3;
         if (sl == 0) w = lst[(size_t)(e >> 2)*gs];
         int v = (sl == 0) ? w.x : (sl == 1) ? w.y : (sl == 2) ? w.z : w.w;
-        pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, codes ? nl_index<REACH>(v) : v,
-                                         vsf, vsw, A, nff, nfw);
+        pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, ki, codes ? nl_index<REACH>(v) : v,
+                                         vsf, vsw, A, npc);
     }
     if (act && cnt > cap) {
         // the list overflowed: k_force11_nl_over continues this target from these sums
@@ -2215,7 +2232,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl(Dev D0, Params P)
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         dtb = dbits(cand);
     }
-    force_partials(D, dtb, nff, nfw);
+    force_partials(D, dtb, tally_free(npc), tally_fixed(npc));
 }
 
 // Ends the targets of the FP64 list pass whose neighbours overflowed the lists: the entries
@@ -2245,21 +2262,22 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_nl_over(Dev D0, Params 
         double4 s0 = D.ovs[2*q];
         Acc11 A;
         A.a0 = s0.x; A.a1 = s0.y; A.a2 = s0.z; A.dd = s0.w;
-        unsigned nff = 0, nfw = 0;
+        unsigned npc = 0;
+        const int ki = D.key[i];
         int kx, ky, kz;
-        decode_key(P, D.key[i], kx, ky, kz);
+        decode_key(P, ki, kx, ky, kz);
         int k = 0;
         walk_rows<DIM, REACH>(D, P, i, gi0.x, gi0.y, gi0.z, kx, ky, kz, cull2, [&](int j, int, double r2) {
             if (!((REACH == 2) ? (r2 <= reach2) : in_support(P, r2))) return;      // a list entry
-            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, vsf, vsw, A, nff, nfw);
+            if (k >= cap) pair11_fast<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, ki, j, vsf, vsw, A, npc);
             ++k;
         });
         D.acc[i] = make_double4(A.a0, A.a1, A.a2, A.dd);
         double na = sqrt(A.a0*A.a0 + A.a1*A.a1 + A.a2*A.a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         atomicMin(&D.ctl->dtmin_bits, dbits(cand));
-        if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
-        if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
+        if (tally_free(npc)) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)tally_free(npc));
+        if (tally_fixed(npc)) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)tally_fixed(npc));
     }
 }
 
@@ -2559,7 +2577,8 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
         const double4 a_init = (act && l == 0) ? a_start(D, P, i) : make_double4(0.0, 0.0, 0.0, 0.0);
         A.a0 = a_init.x; A.a1 = a_init.y; A.a2 = a_init.z; A.dd = a_init.w;
     }
-    unsigned nff = 0, nfw = 0;
+    unsigned npc = 0;
+    const int ki = D.key[i];
     if (staged) {
         // slots from the lane's slot words, records from LDS (a wall's viscous velocity only
         // for walls; a free neighbour's is its v)
@@ -2584,14 +2603,14 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
                 const double2 wa = swa[slot - uf], wb = swb[slot - uf];
                 w = make_double4(wa.x, wa.y, wb.x, wb.y);
             }
-            pair11_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
-                                  a0, a1, a2, w, vsf, vsw, A, nff, nfw);
+            pair11_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, ki, [&] { return items[slot]; },
+                                  a0, a1, a2, w, vsf, vsw, A, npc);
         }
     } else {
         const size_t gs = (size_t)P.nfree;
         for (int e = l; e < m; e += kTileLanes) {
             const int j = nl_index<2>(D.nl[(((size_t)(e >> 2))*gs + (size_t)t)*4 + (e & 3)]);
-            pair11_fast<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, vsf, vsw, A, nff, nfw);
+            pair11_fast<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, ki, j, vsf, vsw, A, npc);
         }
     }
     // the lanes' sums meet in a fixed order
@@ -2614,7 +2633,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force11_tile(Dev D0
             dtb = dbits(cand);
         }
     }
-    force_partials(D, dtb, nff, nfw);
+    force_partials(D, dtb, tally_free(npc), tally_fixed(npc));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2659,7 +2678,7 @@ template <int DIM, bool SKIN, class AB, class JF>
 __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &ab,
                                       float4 gi0, float4 gi1, float4 gi2, int ci, int i, JF jf,
                                       float4 a0, float4 a1, float4 a2, float4 vw,
-                                      float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
+                                      float o0, float o1, float o2, Acc32 &A, unsigned &npc,
                                       float kw, float kg, float kgh, float invh, float h, float h2c,
                                       float vsf, float vsw, const Band32 &band)
 {
@@ -2685,18 +2704,17 @@ __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &a
     }
     (void)kw;
     const bool freej = meta >= 0.0f;
-    nff += freej ? 1u : 0u;
-    nfw += freej ? 0u : 1u;
+    npc += pair_tally(freej);
     const float w0 = vw.x, w1 = vw.y, w2 = vw.z;         // viscous velocity (a free neighbour's v)
-    float alpha, beta;
-    ab.get(ci, meta, freej, alpha, beta);
+    float nha, beta;
+    ab.get(ci, meta, freej, nha, beta);
     float u0 = gi1.x - a1.x, u1 = gi1.y - a1.y, u2 = gi1.z - a1.z;
     float dvx = u0*x0 + u1*x1 + u2*x2;
     float di = gi0.w, dj = a0.w;
     float PIij = 0.0f;
-    if (ab.on(alpha, beta, freej) && dvx < 0.0f) {
+    if (ab.on(nha, beta, freej) && dvx < 0.0f) {
         float MU = h*dvx*__builtin_amdgcn_rcpf(r2 + h2c);
-        PIij = (-alpha*0.5f*(gi2.x + cj_snd)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
+        PIij = (nha*(gi2.x + cj_snd)*MU + beta*MU*MU)*__builtin_amdgcn_rcpf(0.5f*(di + dj));
     }
     float mi = gi2.z;
     float mj = freej ? mrec : mrec*mi;
@@ -2717,14 +2735,14 @@ __device__ inline void pair11_f32_rec(const Dev &D, const Params &P, const AB &a
 template <int DIM, bool SKIN>
 __device__ inline void pair11_f32(const Dev &D, const Params &P, const float *pa, const float *pb,
                                   float4 gi0, float4 gi1, float4 gi2, int ci, int i, int j,
-                                  float o0, float o1, float o2, Acc32 &A, unsigned &nff, unsigned &nfw,
+                                  float o0, float o1, float o2, Acc32 &A, unsigned &npc,
                                   float kw, float kg, float kgh, float invh, float h, float h2c,
                                   float vsf, float vsw, const Band32 &band)
 {
     const float4 a0 = D.f0[j], a1 = D.f1[j], a2 = D.f2[j];
     const float4 vw = (a2.y < 0.0f) ? D.fw[j] : a1;
     pair11_f32_rec<DIM, SKIN>(D, P, ABTable<float>{pa, pb}, gi0, gi1, gi2, ci, i, [j] { return j; }, a0, a1, a2, vw,
-                              o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
+                              o0, o1, o2, A, npc, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
 }
 
 // FP32-mixed fluid force pass over the neighbour lists: the list's cell code gives the cell
@@ -2752,7 +2770,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
-    unsigned nff = 0, nfw = 0;
+    unsigned npc = 0;
     const int cap = P.nl_cap;
     int cnt = act ? D.ncnt[t] : 0;
     int m = cnt < cap ? cnt : cap;
@@ -2763,7 +2781,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
         code_offset<REACH>(c, dx, dy, dz);
         float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
         pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
-                                    A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
+                                    A, npc, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
     };
     int4 w = make_int4(0, 0, 0, 0);
     for (int e = 0; e < m; ++e) {
@@ -2788,7 +2806,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         dtb = dbits(cand);
     }
-    force_partials(D, dtb, nff, nfw);
+    force_partials(D, dtb, tally_free(npc), tally_fixed(npc));
 }
 
 // Ends the overflowed targets of the FP32 list and tile passes (see k_force11_nl_over).  After
@@ -2826,7 +2844,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
             A.ddc = (float)D.ovs[2*q+1].x;
             s0 = make_double4(0.0, 0.0, 0.0, 0.0);
         }
-        unsigned nff = 0, nfw = 0;
+        unsigned npc = 0;
         int kx, ky, kz;
         decode_key(P, D.key[i], kx, ky, kz);
         double4 pi = D.pos_b[i];
@@ -2838,7 +2856,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
                 code_offset<REACH>(c, dx, dy, dz);
                 float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
                 pair11_f32<DIM, REACH == 2>(D, P, pa, pb, gi0, gi1, gi2, ci, i, j, p0, p1, p2,
-                                            A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
+                                            A, npc, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
             }
             ++k;
         });
@@ -2848,8 +2866,8 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
         double na = sqrt(a0*a0 + a1*a1 + a2*a2);
         double cand = P.sqrt_h_a*sqrt(P.h_uniform/na);
         atomicMin(&D.ctl->dtmin_bits, dbits(cand));
-        if (nff) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)nff);
-        if (nfw) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)nfw);
+        if (tally_free(npc)) atomicAdd(&D.ctl->pairs_ff, (unsigned long long)tally_free(npc));
+        if (tally_fixed(npc)) atomicAdd(&D.ctl->pairs_fw, (unsigned long long)tally_fixed(npc));
     }
 }
 
@@ -2915,7 +2933,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
-    unsigned nff = 0, nfw = 0;
+    unsigned npc = 0;
     if (staged) {
         // a wall's viscous velocity read only for walls; a free neighbour's is its v
         const auto ab = ABPick<ABM, float>::make(P, pa, pb);
@@ -2938,7 +2956,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             const float o0 = (float)((kj & 1023) - kix)*csx, o1 = (float)(((kj >> 10) & 1023) - kiy)*csy;
             const float o2 = (DIM == 3) ? (float)((kj >> 20) - kiz)*csz : 0.0f;
             pair11_f32_rec<DIM, true>(D, P, ab, gi0, gi1, gi2, ci, i, [&] { return items[slot]; },
-                                      a0, a1, a2, w, o0, o1, o2, A, nff, nfw, kw, kg, kgh, invh, h, h2c,
+                                      a0, a1, a2, w, o0, o1, o2, A, npc, kw, kg, kgh, invh, h, h2c,
                                       vsf, vsw, band);
         }
     } else {
@@ -2949,7 +2967,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             code_offset<2>(nl_code<2>(v), dx, dy, dz);
             const float p0 = dx*csx, p1 = dy*csy, p2 = (DIM == 3) ? dz*csz : 0.0f;
             pair11_f32<DIM, true>(D, P, pa, pb, gi0, gi1, gi2, ci, i, nl_index<2>(v), p0, p1, p2,
-                                  A, nff, nfw, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
+                                  A, npc, kw, kg, kgh, invh, h, h2c, vsf, vsw, band);
         }
     }
     double b0 = (double)A.a0, b1 = (double)A.a1, b2 = (double)A.a2, bd = (double)A.dd + (double)(-A.ddc);
@@ -2974,7 +2992,7 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
             dtb = dbits(cand);
         }
     }
-    force_partials(D, dtb, nff, nfw);
+    force_partials(D, dtb, tally_free(npc), tally_fixed(npc));
 }
 
 // ------------------------------------------------------------------------------------------
@@ -3264,9 +3282,12 @@ __global__ void __launch_bounds__(kBlock) k_integrate(Dev D0, Params P)
         }
         p.w = rho;
         v.w = P_;
+        if (!SOLID && P.views_next && P.dim == 2) v.z = 0.0;   // as k_reorder leaves a 2-D step's v
         D.pos_b[i] = p;
         D.vel_b[i] = v;
         D.vh_b[i] = vh;
+        // the next step keeps this order (queued lean): its views, in place of its k_reorder
+        if (!SOLID && P.views_next) free_views(D, P, c, i, f, p, v, mm);
     }
     if (P.verlet && flag_free(f)) {
         // list reuse: a particle more than half the skin away from where the lists were built

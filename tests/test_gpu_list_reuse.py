@@ -182,27 +182,35 @@ def test_tile_gather_matches_list_gather(monkeypatch, case, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_lean_steps_equal_full_steps(monkeypatch, precision):
+@pytest.mark.parametrize("case", list(CASES))
+def test_lean_steps_equal_full_steps(monkeypatch, case, precision):
     """Steps that keep their lists are queued without the sort and list-build kernels, which are
-    no-ops on such steps; a tail before a step that must rebuild hands it back to the host.
-    Queuing every step in full (PSPH_NO_LEAN) gives the same run bit for bit, with the same
-    list builds, over calls of several lengths."""
-    cfg, p, dt = _small_3d(1)
+    no-ops on such steps; a tail before a step that must rebuild hands it back to the host; and
+    a lean step after another step of the same queue takes the views that step's integration
+    wrote instead of running k_reorder.  Queuing every step in full (PSPH_NO_LEAN), and lean
+    steps that each run k_reorder (PSPH_NO_FUSED_VIEWS), give the same run bit for bit, with
+    the same list builds, over calls of several lengths (a small skin: many rebuilds)."""
+    cfg, p, dt = CASES[case]()
     if precision == "fp32":
+        cfg = type(cfg).from_buffer_copy(cfg)
         cfg.precision = FP32_MIXED
     res = {}
-    for name, env in (("lean", {}), ("full", {"PSPH_NO_LEAN": "1"})):
-        monkeypatch.delenv("PSPH_NO_LEAN", raising=False)
-        print(f"[lean] {precision} {name}", flush=True)
-        g = _solver(monkeypatch, env, cfg, p, dt)
+    for name, env in (("lean", {}), ("full", {"PSPH_NO_LEAN": "1"}), ("unfused", {"PSPH_NO_FUSED_VIEWS": "1"})):
+        for k in ("PSPH_NO_LEAN", "PSPH_NO_FUSED_VIEWS"):
+            monkeypatch.delenv(k, raising=False)
+        print(f"[lean] {case} {precision} {name}", flush=True)
+        g = _solver(monkeypatch, dict(env, PSPH_SKIN="0.01"), cfg, p, dt)
         builds = 0
         for chunk in (1, 7, 40, 300):
             st, b = _run(g, chunk, chunk=chunk)
             builds += b
         res[name] = (builds, st, g.particles())
         g.close()
-    (b0, s0, A), (b1, s1, B) = res["lean"], res["full"]
-    assert b0 == b1 and b0 >= 3, (b0, b1)
-    assert s0.time == s1.time and s0.pairs == s1.pairs
-    for f in ("x", "v", "Density", "a"):
-        np.testing.assert_array_equal(A[f], B[f], err_msg=f)
+    b0, s0, A = res["lean"]
+    assert b0 >= 3, b0
+    for name in ("full", "unfused"):
+        b1, s1, B = res[name]
+        assert b0 == b1, (name, b0, b1)
+        assert s0.time == s1.time and s0.pairs == s1.pairs, name
+        for f in ("x", "v", "Density", "a"):
+            np.testing.assert_array_equal(A[f], B[f], err_msg=f"{name} {f}")
