@@ -1425,6 +1425,16 @@ int psph_upload(psph_handle *h, const psph_particle *p, size_t n, double time, d
             P.verlet_force = std::getenv("PSPH_REBUILD_EVERY_STEP") ? 1 : 0;
         }
     }
+    if (P.mixed32) {
+        P.f32.kw = (float)P.kw; P.f32.kg = (float)P.kg; P.f32.kgh = (float)P.kgh;
+        P.f32.invh = (float)P.invh; P.f32.h = (float)P.h_uniform; P.f32.h2c = (float)P.h2c;
+        for (int d = 0; d < 3; ++d) P.f32.cs[d] = (float)P.csize[d];
+        P.f32.band_lo = (float)(P.sup2_lo*(1.0 - 1e-4));
+        P.f32.band_hi = (float)(P.sup2_hi*(1.0 + 1e-4));
+        P.f32.band_adj = (float)(P.adj2*(1.0 - 1e-4));
+        P.f32.mu2_ff = 2.0f*(float)P.mu_ff;
+        P.f32.mu2_wall = 2.0f*(float)P.mu_ff_wall;
+    }
     // in/out-flow runs may grow by recycling: room for half as many particles again (more is
     // made when needed, grow_buffers); PSPH_INOUT_SLACK sets the extra room (test switch)
     size_t slack = n/2 + 256;

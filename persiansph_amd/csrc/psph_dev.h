@@ -137,6 +137,12 @@ struct Params {
     int ab_on[2];
     // FP32-mixed tier (fast fluid configuration only)
     int mixed32;
+    // its constants rounded to FP32 once by the host, so that the gathers take them as scalar
+    // operands: kernel constants, 1/h, h, 0.01 h^2, cell size, the skin test's FP32 band
+    // (r^2 bounds and the adjacency bound, widened by 1e-4), and 2 mu of both pair kinds
+    struct {
+        float kw, kg, kgh, invh, h, h2c, cs[3], band_lo, band_hi, band_adj, mu2_ff, mu2_wall;
+    } f32;
     // neighbour lists of the fast fluid path: capacity per free target (a multiple of 4), and
     // whether entries carry the neighbour cell code (needs fewer than 2^27 particles)
     int nl_cap, nl_codes;

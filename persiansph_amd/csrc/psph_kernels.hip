@@ -2650,9 +2650,7 @@ struct Acc32 { float a0, a1, a2, dd, ddc; };
 // band the FP64 test runs on the FP64 positions.
 struct Band32 {
     float lo, hi, adj;
-    __device__ explicit Band32(const Params &P)
-        : lo((float)(P.sup2_lo*(1.0 - 1e-4))), hi((float)(P.sup2_hi*(1.0 + 1e-4))),
-          adj((float)(P.adj2*(1.0 - 1e-4))) {}
+    __device__ explicit Band32(const Params &P) : lo(P.f32.band_lo), hi(P.f32.band_hi), adj(P.f32.band_adj) {}
 };
 
 __device__ inline bool skin_member64(const Dev &D, const Params &P, int i, int j)
@@ -2763,10 +2761,10 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl(Dev D0, Params P
     int i = act ? D.free_list[t] : 0;
     float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
     int ci = (int)gi2.y;
-    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
-    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
-    const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
+    const float csx = P.f32.cs[0], csy = P.f32.cs[1], csz = P.f32.cs[2];
+    const float kw = P.f32.kw, kg = P.f32.kg, kgh = P.f32.kgh, invh = P.f32.invh;
+    const float h = P.f32.h, h2c = P.f32.h2c;
+    const float vsf = P.f32.mu2_ff*gi2.w, vsw = P.f32.mu2_wall*gi2.w;   // 2 mu / d_i
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
@@ -2823,9 +2821,9 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
     __shared__ float pa[64], pb[64];
     if (threadIdx.x < 64) { pa[threadIdx.x] = (float)P.pair_alpha[threadIdx.x]; pb[threadIdx.x] = (float)P.pair_beta[threadIdx.x]; }
     __syncthreads();
-    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
-    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
+    const float csx = P.f32.cs[0], csy = P.f32.cs[1], csz = P.f32.cs[2];
+    const float kw = P.f32.kw, kg = P.f32.kg, kgh = P.f32.kgh, invh = P.f32.invh;
+    const float h = P.f32.h, h2c = P.f32.h2c;
     const int cap = P.nl_cap;
     const Band32 band(P);
     const double reach2 = P.reach2;
@@ -2835,7 +2833,7 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
         int i = D.free_list[t];
         float4 gi0 = D.f0[i], gi1 = D.f1[i], gi2 = D.f2[i];
         int ci = (int)gi2.y;
-        const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
+        const float vsf = P.f32.mu2_ff*gi2.w, vsw = P.f32.mu2_wall*gi2.w;   // 2 mu / d_i
         double4 s0 = D.ovs[2*q];
         Acc32 A;
         A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
@@ -2877,8 +2875,12 @@ __global__ void __launch_bounds__(kForceBlock) k_force11_f32_nl_over(Dev D0, Par
 // offset from the two cells.  Lane l of a target adds entries l, l + kTileLanes, ... in list order
 // in FP32 (Kahan-compensated continuity per lane); the lanes' sums are widened and meet in FP64
 // in the fixed order of k_force11_tile.  Overflowed targets go to k_force11_f32_nl_over.
+// slot words a lane of the FP32 tile gather holds in registers (16 entries per lane, 128 per
+// target; longer lists read the rest in the loop): fewer than the FP64 gather's, so that the
+// kernel fits 80 VGPRs and three workgroups share a CU
+constexpr int kSlotWords32 = 4;
 template <int DIM, int ABM>
-__global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0, Params P)
+__global__ void __launch_bounds__(kTileTargets*kTileLanes, 6) k_force32_tile(Dev D0, Params P)
 {
     const Dev D = bind(D0);
     if (D.ctl->halt) return;
@@ -2905,9 +2907,9 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
     const uint64_t *tq = reinterpret_cast<const uint64_t *>(D.tl + (size_t)tile*P.tl_stride) +
                          tt*kTileLanes + l;
     constexpr size_t kWordStride = (size_t)kTileTargets*kTileLanes;
-    uint64_t qw[kSlotWords];
+    uint64_t qw[kSlotWords32];
 #pragma unroll
-    for (int G = 0; G < kSlotWords; ++G)
+    for (int G = 0; G < kSlotWords32; ++G)
         qw[G] = (staged && l + 4*kTileLanes*G < m) ? tq[G*kWordStride] : 0ull;
     auto pack = [&](int key) {
         int kx, ky, kz;
@@ -2926,10 +2928,10 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
     }
     __syncthreads();
     const int ci = (int)gi2.y;
-    const float csx = (float)P.csize[0], csy = (float)P.csize[1], csz = (float)P.csize[2];
-    const float kw = (float)P.kw, kg = (float)P.kg, kgh = (float)P.kgh, invh = (float)P.invh;
-    const float h = (float)P.h_uniform, h2c = (float)P.h2c;
-    const float vsf = 2.0f*(float)P.mu_ff*gi2.w, vsw = 2.0f*(float)P.mu_ff_wall*gi2.w;   // 2 mu / d_i
+    const float csx = P.f32.cs[0], csy = P.f32.cs[1], csz = P.f32.cs[2];
+    const float kw = P.f32.kw, kg = P.f32.kg, kgh = P.f32.kgh, invh = P.f32.invh;
+    const float h = P.f32.h, h2c = P.f32.h2c;
+    const float vsf = P.f32.mu2_ff*gi2.w, vsw = P.f32.mu2_wall*gi2.w;   // 2 mu / d_i
     const Band32 band(P);
     Acc32 A;
     A.a0 = 0.0f; A.a1 = 0.0f; A.a2 = 0.0f; A.dd = 0.0f; A.ddc = 0.0f;
@@ -2945,8 +2947,8 @@ __global__ void __launch_bounds__(kTileTargets*kTileLanes) k_force32_tile(Dev D0
                 const int G = g >> 2;
                 q = qw[0];
 #pragma unroll
-                for (int k = 1; k < kSlotWords; ++k) q = (G == k) ? qw[k] : q;
-                if (G >= kSlotWords) q = tq[(size_t)G*kWordStride];
+                for (int k = 1; k < kSlotWords32; ++k) q = (G == k) ? qw[k] : q;
+                if (G >= kSlotWords32) q = tq[(size_t)G*kWordStride];
             }
             const int slot = (int)((q >> (16*(g & 3))) & 0xffffu);
             const float4 a0 = s0[slot], a1 = s1[slot], a2 = s2[slot];
