@@ -404,15 +404,29 @@ __global__ void __launch_bounds__(256) k_edge_mlp(int M, LdEdgeCat al, const uin
 
 // ---- elementwise / graph kernels ----------------------------------------------------------
 
-// a[i] = mean of E over the CSR segment of i (bf16 out, FP32 sums); one wave per node
+// a[i] = mean of E over the CSR segment of i (bf16 out, FP32 sums in edge order); one wave per
+// node, eight edge rows in flight per wave (the loads of a group are issued before its adds)
 __global__ void __launch_bounds__(256) k_segsum(int n, const int32_t *rp, const uint16_t *E, uint16_t *a)
 {
     int i = blockIdx.x*4 + (threadIdx.x >> 6);
     if (i >= n) return;
     int lane = threadIdx.x & 63;
     float s0 = 0.0f, s1 = 0.0f;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) {
-        uint32_t v = *(const uint32_t *)(E + (size_t)e*128 + 2*lane);
+    const uint16_t *col = E + 2*lane;
+    int e = rp[i];
+    const int e1 = rp[i + 1];
+    for (; e + 8 <= e1; e += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = *(const uint32_t *)(col + (size_t)(e + k)*128);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            s0 += bf2f((uint16_t)(v[k] & 0xffffu));
+            s1 += bf2f((uint16_t)(v[k] >> 16));
+        }
+    }
+    for (; e < e1; ++e) {
+        uint32_t v = *(const uint32_t *)(col + (size_t)e*128);
         s0 += bf2f((uint16_t)(v & 0xffffu));
         s1 += bf2f((uint16_t)(v >> 16));
     }
