@@ -700,23 +700,27 @@ __global__ void __launch_bounds__(256) k_node_in(int n, int dim, const double *v
     X[t] = f2bf(f);
 }
 
-// edge inputs (padded to 32 columns): (x_recv - x_send) / R (dim), |x_recv - x_send| / R
+// edge inputs (padded to 32 columns): (x_recv - x_send) / R (dim), |x_recv - x_send| / R; one
+// thread per edge, its 64-B row written as four 16-B stores
 __global__ void __launch_bounds__(256) k_edge_in(int ne, int dim, const int32_t *recv, const int32_t *send,
                                                  const double *x, double inv_r, uint16_t *U)
 {
-    size_t t = (size_t)blockIdx.x*256 + threadIdx.x;
-    if (t >= (size_t)ne*32) return;
-    size_t e = t >> 5;
-    int c = (int)(t & 31);
-    int i = recv[e], j = send[e];
-    float f = 0.0f;
-    if (c < dim) f = (float)((x[3*i + c] - x[3*j + c])*inv_r);
-    else if (c == dim) {
-        double s = 0.0;
-        for (int d = 0; d < dim; ++d) { double a = x[3*i + d] - x[3*j + d]; s += a*a; }
-        f = (float)(sqrt(s)*inv_r);
-    }
-    U[t] = f2bf(f);
+    size_t e = (size_t)blockIdx.x*256 + threadIdx.x;
+    if (e >= (size_t)ne) return;
+    const int i = recv[e], j = send[e];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; ++d) a[d] = x[3*i + d] - x[3*j + d];
+    double s = 0.0;
+    for (int d = 0; d < dim; ++d) s += a[d]*a[d];
+    uint32_t w[4] = {0u, 0u, 0u, 0u};          // columns 0..7 as bf16 pairs
+    for (int d = 0; d < dim; ++d)
+        w[d >> 1] |= (uint32_t)f2bf((float)(a[d]*inv_r)) << (16*(d & 1));
+    w[dim >> 1] |= (uint32_t)f2bf((float)(sqrt(s)*inv_r)) << (16*(dim & 1));
+    uint4 *row = reinterpret_cast<uint4 *>(U + e*32);
+    row[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    row[1] = make_uint4(0u, 0u, 0u, 0u);
+    row[2] = make_uint4(0u, 0u, 0u, 0u);
+    row[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // state after k solver steps: x += dx, v += dv, rho += drho for free particles, from the
@@ -880,7 +884,7 @@ void node_in(hipStream_t s, int n, int dim, const double *v, const double *rho, 
 { hipLaunchKernelGGL(k_node_in, dim3(nb((size_t)n*32, 256)), dim3(256), 0, s, n, dim, v, rho, isfree, rp, fs, X); }
 void edge_in(hipStream_t s, int ne, int dim, const int32_t *recv, const int32_t *send, const double *x, double inv_r,
              uint16_t *U)
-{ hipLaunchKernelGGL(k_edge_in, dim3(nb((size_t)ne*32, 256)), dim3(256), 0, s, ne, dim, recv, send, x, inv_r, U); }
+{ hipLaunchKernelGGL(k_edge_in, dim3(nb((size_t)ne, 256)), dim3(256), 0, s, ne, dim, recv, send, x, inv_r, U); }
 void apply_out(hipStream_t s, int n, int dim, const float *y, const int32_t *isfree, const FeatScale &fs,
                double *x, double *v, double *rho)
 { hipLaunchKernelGGL(k_apply, dim3(nb(n, 256)), dim3(256), 0, s, n, dim, y, isfree, fs, x, v, rho); }
