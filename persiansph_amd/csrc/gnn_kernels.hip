@@ -631,13 +631,15 @@ __global__ void __launch_bounds__(256) k_g_edges(int n, GraphGrid g, const doubl
     }
 }
 
-// receiver/sender arrays and the reverse-edge index (binary search in the sender's row)
+// receiver/sender arrays and the reverse-edge index (binary search in the sender's row); one
+// wave per node, its lanes taking the node's edges, so the searches run side by side
 __global__ void __launch_bounds__(256) k_g_finish(int n, const int32_t *rp, const int32_t *cols,
                                                   int32_t *recv, int32_t *send, int32_t *rev)
 {
-    int i = blockIdx.x*256 + threadIdx.x;
+    int i = blockIdx.x*4 + (threadIdx.x >> 6);
     if (i >= n) return;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+    const int lane = threadIdx.x & 63;
+    for (int e = rp[i] + lane; e < rp[i + 1]; e += 64) {
         int j = cols[e];
         recv[e] = i;
         send[e] = j;
@@ -875,7 +877,7 @@ void graph_edges(hipStream_t s, int pass, int n, const GraphGrid &g, const doubl
         hipLaunchKernelGGL(k_g_edges<1>, dim3(nb(n, 256)), dim3(256), 0, s, n, g, x, cstart, sorted, deg, rp, cols);
 }
 void graph_finish(hipStream_t s, int n, const int32_t *rp, const int32_t *cols, int32_t *recv, int32_t *send, int32_t *rev)
-{ hipLaunchKernelGGL(k_g_finish, dim3(nb(n, 256)), dim3(256), 0, s, n, rp, cols, recv, send, rev); }
+{ hipLaunchKernelGGL(k_g_finish, dim3(nb(n, 4)), dim3(256), 0, s, n, rp, cols, recv, send, rev); }
 void scan1(hipStream_t s, int n, const int32_t *in, int32_t *out)
 { hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, n, in, out); }
 
